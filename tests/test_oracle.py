@@ -1,0 +1,263 @@
+"""CPU tests of the oracle against the reference's own artefacts.
+
+Pins (see oracle/oracle.cpp header): the loader against the reference's
+loader compiled from its sources (golden ref_load.json), the SwapPairList pop
+order against the reference comparator in std::priority_queue (ref_pq.json),
+the VENOM trajectory against reordering.py itself (venom_ref.json), the gain
+arithmetic against the closed form, the greedy against a literal dense
+restatement, and the temp.mtx / output.txt known answer.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import literal_ref as L
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MTX = os.path.join(GOLD, "mtx")
+
+
+def _digest(m):
+    h = hashlib.sha256()
+    h.update(np.array([m.nrows, m.ncols, m.nnz], np.int64).tobytes())
+    h.update(m.row.astype(np.int32).tobytes())
+    h.update(m.col.astype(np.int32).tobytes())
+    h.update(m.val.astype(np.float32).tobytes())
+    return h.hexdigest()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _built():
+    oracle.build()
+
+
+def test_loader_matches_reference_loader():
+    for rec in json.load(open(os.path.join(GOLD, "ref_load.json"))):
+        m = oracle.mtx_read(os.path.join(MTX, rec["file"]), rec["M"])
+        assert (m.nrows, m.ncols, m.nnz) == (rec["nrows"], rec["ncols"], rec["nnz"])
+        assert _digest(m) == rec["sha256"], rec["file"]
+        if "row" in rec:
+            assert m.row.tolist() == rec["row"]
+            assert m.col.tolist() == rec["col"]
+            assert np.array_equal(m.val, np.array(rec["val"], np.float32))
+
+
+def test_pq_order_matches_reference_comparator():
+    for rec in json.load(open(os.path.join(GOLD, "ref_pq.json"))):
+        got = oracle.pq_pop_order(np.array(rec["score"]), rec["M"]).tolist()
+        assert got == rec["order"]
+        assert L.pop_order(rec["score"], rec["M"]) == rec["order"]
+
+
+def test_venom_matches_reference_script():
+    for c in json.load(open(os.path.join(GOLD, "venom_ref.json"))):
+        m = oracle.Matrix(c["nrows"], c["ncols"], c["row"], c["col"], c["val"])
+        assert list(oracle.venom_eval(m, c["M"], c["V"])) == c["evalonly"]
+        st = oracle.venom_reorder(m, c["M"], c["V"], 10)
+        assert st.tolist() == c["output"], (c["name"], c["V"])
+        pos = sorted(set(zip(m.row.tolist(), m.col.tolist())))
+        assert [list(p) for p in pos] == c["out_pos"]
+
+
+def test_temp_mtx_known_answer():
+    # reference fixture: output.txt = "0,0" for --evalonly --m 8 --v <= 32
+    m = oracle.mtx_read(os.path.join(MTX, "temp.mtx"), 8)
+    assert open(os.path.join(MTX, "temp_output.txt")).read().strip() == "0,0"
+    for V in (1, 8, 16, 32):
+        assert oracle.venom_eval(m, 8, V) == (0, 0)
+    cnt, zero = oracle.locate(m, 2, 8)
+    assert int((cnt != 0).sum()) == 0
+    a = m.copy()
+    st, _ = oracle.nm_reorder(a, 2, 8, 0, 10)
+    assert st.tolist() == [0, 0, 0, 0, 0]
+    assert np.array_equal(a.row, m.row) and np.array_equal(a.col, m.col)
+
+
+def _closed_form(l, r, b1, b2, N, M):
+    """Per-row gain, closed form: only the exchanged bits matter."""
+    x = (l >> (M - 1 - b1)) & 1
+    y = (r >> (M - 1 - b2)) & 1
+    pl, pr = bin(l).count("1"), bin(r).count("1")
+    if x == y:
+        return 0, 0
+    if x == 1:
+        return int(pl == N + 1), -int(pr == N)
+    return -int(pl == N), int(pr == N + 1)
+
+
+@pytest.mark.parametrize("M,N", [(4, 2), (8, 2), (4, 1), (8, 3)])
+def test_swap_gain_closed_form_exhaustive(M, N):
+    # every (l, r) bitmap pair as one row: seg 0 holds l, seg 1 holds r
+    pairs = [(l, r) for l in range(1 << M) for r in range(1 << M)]
+    rng = np.random.default_rng(M * 10 + N)
+    rows, cols = [], []
+    for i, (l, r) in enumerate(pairs):
+        for j in range(M):
+            if (l >> (M - 1 - j)) & 1:
+                rows.append(i); cols.append(j)
+            if (r >> (M - 1 - j)) & 1:
+                rows.append(i); cols.append(M + j)
+    R = len(pairs)
+    m = oracle.from_tuples(R, R, np.array(rows), np.array(cols), np.ones(len(rows)), M)
+    got = oracle.swap_score(m, N, M, 0, 1).reshape(M * M, 2)
+    exp = np.zeros((M * M, 2), np.int64)
+    for (l, r) in pairs:
+        for b1 in range(M):
+            for b2 in range(M):
+                g = _closed_form(l, r, b1, b2, N, M)
+                exp[b1 * M + b2] += g
+    assert np.array_equal(got, exp)
+    # random row subsets, so a per-row error cannot cancel in the full sum
+    for _ in range(5):
+        keep = rng.random(R) < 0.3
+        idx = np.nonzero(keep)[0]
+        sel = np.isin(np.array(rows), idx)
+        m2 = oracle.from_tuples(R, R, np.array(rows)[sel], np.array(cols)[sel],
+                                np.ones(int(sel.sum())), M)
+        got2 = oracle.swap_score(m2, N, M, 0, 1).reshape(M * M, 2)
+        exp2 = np.zeros((M * M, 2), np.int64)
+        for i in idx:
+            l, r = pairs[i]
+            for b1 in range(M):
+                for b2 in range(M):
+                    exp2[b1 * M + b2] += _closed_form(l, r, b1, b2, N, M)
+        assert np.array_equal(got2, exp2)
+
+
+def _rand(rng, R, dens, M, sym=False, zero_frac=0.05):
+    mask = rng.random((R, R)) < dens
+    if sym:
+        mask |= mask.T
+    r, c = np.nonzero(mask)
+    v = rng.uniform(-1, 1, r.shape[0]).astype(np.float32)
+    v[rng.random(r.shape[0]) < zero_frac] = 0.0
+    return oracle.from_tuples(R, R, r, c, v, M)
+
+
+def test_locate_brute_force():
+    rng = np.random.default_rng(7)
+    for t in range(20):
+        M = (4, 8)[t % 2]
+        R = int(rng.integers(1, 12)) * M
+        m = _rand(rng, R, rng.uniform(0.05, 0.7), M)
+        cnt, zero = oracle.locate(m, 2, M)
+        bm = L.dense_bitmaps(R, M, m.row.tolist(), m.col.tolist(), m.val.tolist())
+        for s in range(R // M):
+            assert cnt[s] == sum(1 for row in bm if bin(row[s]).count("1") > 2)
+            assert zero[s] == sum(1 for row in bm if row[s] == 0)
+
+
+@pytest.mark.parametrize("sched", [0, 1, 2, 3, 8])
+def test_greedy_matches_literal_restatement(sched):
+    rng = np.random.default_rng(100 + sched)
+    for t in range(10):
+        M = (4, 8)[t % 2]
+        N = 2 if M == 8 else int(rng.integers(1, 3))
+        R = int(rng.integers(2, 7)) * M
+        m = _rand(rng, R, rng.uniform(0.1, 0.6), M, sym=t % 3 == 0)
+        a = m.copy()
+        st, _ = oracle.nm_reorder(a, N, M, sched, 10)
+        rows, cols = [int(x) for x in m.row], [int(x) for x in m.col]
+        exp = L.nm_reorder(R, rows, cols, [float(x) for x in m.val], N, M, sched, 10)
+        assert st.tolist() == exp
+        assert a.row.tolist() == rows and a.col.tolist() == cols
+    assert oracle.ub_events() == 0
+
+
+def test_greedy_invariants():
+    rng = np.random.default_rng(5)
+    for t in range(12):
+        M = (4, 8)[t % 2]
+        R = int(rng.integers(4, 30)) * M
+        m = _rand(rng, R, rng.uniform(0.02, 0.3), M, sym=t % 2 == 0)
+        for sched in (0, 1, 2, 3):
+            a = m.copy()
+            st, trace = oracle.nm_reorder(a, 2, M, sched, 10, trace_cap=100000)
+            # relabel is one symmetric permutation applied to rows and columns
+            perm = -np.ones(R, np.int64)
+            for src, dst in ((m.row, a.row), (m.col, a.col)):
+                for s, d in zip(src.tolist(), dst.tolist()):
+                    assert perm[s] in (-1, d)
+                    perm[s] = d
+            used = perm[perm >= 0]
+            assert len(set(used.tolist())) == len(used)
+            assert np.array_equal(a.val, m.val)
+            # reported final count equals a fresh recount of the output
+            cnt, _ = oracle.locate(a, 2, M)
+            assert st[1] == int((cnt != 0).sum()) and st[3] == int(cnt.sum())
+            cnt0, _ = oracle.locate(m, 2, M)
+            assert st[0] == int((cnt0 != 0).sum()) and st[2] == int(cnt0.sum())
+            assert trace is not None
+    assert oracle.ub_events() == 0
+
+
+def test_writer_format_and_roundtrip(tmp_path):
+    m = oracle.from_tuples(10, 10, np.array([0, 3, 9]), np.array([1, 2, 9]),
+                           np.array([0.125, -0.004, 2.5]), 8)
+    p = tmp_path / "w.mtx"
+    oracle.mtx_write(p, m)
+    assert p.read_text() == ("%%MatrixMarket matrix coordinate real general\n"
+                             "16 16 3\n1 2 0.12\n4 3 -0.00\n10 10 2.50\n")
+    back = oracle.mtx_read(p, 8)
+    assert (back.nrows, back.nnz) == (16, 3)
+    assert back.val.tolist() == [np.float32(0.12), np.float32(-0.0), np.float32(2.5)]
+
+
+def test_quantisation_roundtrips():
+    for v, a, b in [(0.004, 0.0, 0.0), (0.126, 0.13, 0.13), (1.25, 1.25, 1.2),
+                    (12.345, 12.35, 12.0), (-0.994, -0.99, -0.99)]:
+        assert oracle.quant_cpp_write(v) == np.float32(a)
+        assert oracle.quant_py_roundtrip(v) == np.float32(b)
+
+
+def test_duplicate_value_quirk():
+    # duplicates: row/col compacted, values left in sorted order (Q12)
+    m = oracle.from_tuples(4, 4, np.array([0, 0, 1, 2]), np.array([1, 1, 2, 3]),
+                           np.array([5.0, 6.0, 7.0, 8.0]), 4)
+    assert m.nnz == 3
+    assert m.row.tolist() == [0, 1, 2] and m.col.tolist() == [1, 2, 3]
+    assert m.val[1] in (5.0, 6.0) and m.val[2] == 7.0
+
+
+def test_hybrid_fixtures_consistent():
+    for name in ("GD01_c", "GD95_a", "lund_b", "bcsstm02", "small"):
+        m = oracle.mtx_read(os.path.join(MTX, name + ".mtx"), 8)
+        for V in (1, 8, 32):
+            st, out = oracle.hybrid(m, 2, 8, V, 2, 0)
+            if m.nnz == 0:
+                continue
+            assert out.nnz == m.nnz
+            # final fields equal fresh evaluation of the final matrix
+            cnt, _ = oracle.locate(out, 2, 8)
+            assert st[4] == float((cnt != 0).sum()) and st[6] == float(cnt.sum())
+            assert (st[9], st[11]) == oracle.venom_eval(out, 8, V)
+            assert (st[8], st[10]) == oracle.venom_eval(m, 8, V)
+            assert st[14] == st[12] + st[13]
+            # output order = sorted at last reload (positions unique)
+            key = out.row.astype(np.int64) * out.ncols + out.col
+            assert len(np.unique(key)) == out.nnz
+
+
+def test_retained_nnz_brute_force():
+    import itertools
+    rng = np.random.default_rng(11)
+    for t in range(10):
+        M = (4, 8)[t % 2]
+        R = int(rng.integers(1, 6)) * M
+        m = _rand(rng, R, rng.uniform(0.1, 0.8), M)
+        for V in (1, 2, 4, 8):
+            bm = L.dense_bitmaps(R, M, m.row.tolist(), m.col.tolist(), m.val.tolist())
+            tot = 0
+            for s in range(R // M):
+                for b0 in range(0, R, V):
+                    best = 0
+                    for S in itertools.combinations(range(M), min(4, M)):
+                        mask = sum(1 << (M - 1 - j) for j in S)
+                        best = max(best, sum(min(2, bin(bm[i][s] & mask).count("1"))
+                                             for i in range(b0, min(b0 + V, R))))
+                    tot += best
+            assert oracle.retained_nnz(m, V, 2, M) == tot
