@@ -1,0 +1,205 @@
+/*
+ * vnm.h -- C ABI of the MI355X V:N:M reorder + V:N:M SpMM library (libvnm.so).
+ *
+ * Plain C types only.  Every call returns an int status: VNM_OK (0) or a
+ * negative VNM_E_* code; vnm_last_error() gives the message of the last
+ * failure on the calling thread.  Nothing is swallowed: HIP failures surface
+ * as VNM_E_HIP.  Device work runs on the context's own HIP stream.
+ *
+ * Each entry point names the reference interface it replaces
+ * (PictureResearch/vnm-graph-reordering, file:line).
+ */
+#ifndef VNM_H
+#define VNM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VNM_OK          0
+#define VNM_E_IO       -1  /* file cannot be opened/written                */
+#define VNM_E_FORMAT   -2  /* bad MatrixMarket banner, size line or entry   */
+#define VNM_E_ARG      -3  /* invalid argument (N, M, V, sched, sizes)      */
+#define VNM_E_HIP      -4  /* a HIP runtime call or kernel launch failed    */
+#define VNM_E_OOM      -5  /* host or device allocation failed             */
+#define VNM_E_UB_PATH  -6  /* input reached a path the reference leaves undefined */
+#define VNM_E_NODEV    -7  /* no usable GPU                                 */
+
+typedef struct vnm_ctx vnm_ctx;
+typedef struct vnm_graph vnm_graph;
+typedef struct vnm_packed vnm_packed;
+
+/* Host COO, the reference's Mtx<float> host arrays (mtx/Mtx.h:64-65):
+ * 0-based ids, entries in the reference's COO order, dims padded to M. */
+typedef struct {
+    int32_t nrows;
+    int32_t ncols;
+    int64_t nnz;
+    int32_t *row;
+    int32_t *col;
+    float   *val;
+} vnm_coo;
+
+const char *vnm_last_error(void);
+
+/* ---------------------------------------------------------------- host I/O */
+
+/* Replaces readMtx (mtx/readMtx.hpp:354-425) + Mtx::init_from_file's host part
+ * (mtx/Mtx.h:98-112): banner/size parsing, symmetric mirroring, (row, col)
+ * sort, duplicate removal (values left in sorted order, as the reference
+ * does), padding of both dims to a multiple of M when nrows % M != 0.
+ * Out-of-range indices are VNM_E_FORMAT (the reference writes out of bounds).
+ * The arrays are allocated by the library; release with vnm_coo_free. */
+int vnm_mtx_read(const char *path, int M, vnm_coo *out);
+
+/* Replaces Mtx::write_to_file (mtx/Mtx.h:412-435): "%%MatrixMarket matrix
+ * coordinate real general", "nrows ncols nnz", then "%d %d %.2f" per entry
+ * in COO order (1-based). */
+int vnm_mtx_write(const char *path, const vnm_coo *coo);
+
+/* The loader's ordering applied to raw tuples (mtx/readMtx.hpp:152-219,419,
+ * mtx/Mtx.h:105-112). */
+int vnm_coo_from_tuples(int32_t nrows, int32_t ncols, int symmetric, int64_t n,
+                        const int32_t *row, const int32_t *col, const float *val,
+                        int M, vnm_coo *out);
+void vnm_coo_free(vnm_coo *coo);
+
+/* Seeded generators (the reference downloads SuiteSparse instead,
+ * scripts/tab5_small.sh:920-941).  Output is loader-ordered and padded. */
+int vnm_gen_er(int64_t n, int64_t k, uint64_t seed, float lo, float hi, int M, vnm_coo *out);
+int vnm_gen_rmat(int scale, int64_t undirected_edges, double a, double b, double c,
+                 uint64_t seed, int M, vnm_coo *out);
+int vnm_gen_chung_lu(int64_t n, double gamma, double mean_degree, uint64_t seed, int M,
+                     vnm_coo *out);
+void vnm_gen_normal(uint64_t seed, int64_t count, float *out);
+
+/* ---------------------------------------------------------------- context */
+
+/* One context per host thread: owns a HIP stream on `device` and the
+ * device memory of the graphs created on it.  No global or CWD state (the
+ * reference shares ./temp.mtx and ./output.txt, main.cu:118-152). */
+int  vnm_ctx_create(int device, vnm_ctx **out);
+void vnm_ctx_destroy(vnm_ctx *ctx);
+/* The context's hipStream_t, for event timing by the caller. */
+void *vnm_ctx_stream(vnm_ctx *ctx);
+/* Number of kernels launched on the context so far, and the name of the
+ * kernel family the last N:M / VENOM / SpMM call spent most time in. */
+int64_t vnm_ctx_launches(vnm_ctx *ctx);
+
+/* ---------------------------------------------------------------- graph */
+
+/* Replaces Mtx::convert_to_bsr_and_sync_device (mtx/Mtx.h:308-379): uploads
+ * the COO and builds the device segment stores.  M in {4, 8, 16}. */
+int  vnm_graph_create(vnm_ctx *ctx, const vnm_coo *coo, int M, vnm_graph **out);
+void vnm_graph_destroy(vnm_graph *g);
+/* Current COO, in the order the reference writes it (mtx/Mtx.h:429-431):
+ * the order at the last (re)load, ids remapped.  Arrays library-allocated. */
+int  vnm_graph_download(vnm_graph *g, vnm_coo *out);
+/* Map from the ids at vnm_graph_create to current ids (int32[nrows]). */
+int  vnm_graph_perm(vnm_graph *g, int32_t *perm);
+
+/* ---------------------------------------------------------------- N:M primitives */
+
+/* Replaces LocateinvalidSegment / LocateAllZeroSegment + locate_invalid_segment
+ * (reorder/reorder.h:262-317,436-496,767-797): cnt[seg] = rows whose bitmap in
+ * seg has more than N ones, zero[seg] = rows whose bitmap is 0 (padded rows
+ * included).  Host arrays of ceil(nrows/M) ints; either may be NULL. */
+int vnm_locate(vnm_graph *g, int N, int32_t *cnt, int32_t *zero);
+
+/* Replaces GetSegSwapScore + get_segment_swap_score (reorder/reorder.h:75-131,
+ * 498-535), batched: for k < count, scores[k*2*M*M + (b1*M+b2)*2 + {0,1}] =
+ * {lgain, rgain} of exchanging column b1 of seg1 with column b2 of seg2[k]. */
+int vnm_swap_score(vnm_graph *g, int N, int seg1, const int32_t *seg2, int count,
+                   int32_t *scores);
+
+/* Pop order of the reference SwapPairList (std::priority_queue with the
+ * non-strict Comparator, reorder/swappair.h:112-120) for `count` score vectors
+ * of 2*M*M ints, computed on the GPU by the same heap emulation the greedy
+ * kernel uses: order[k*M*M + j] = index of the j-th popped pair. */
+int vnm_pq_pop_order(vnm_ctx *ctx, const int32_t *scores, int count, int M, int32_t *order);
+
+/* ---------------------------------------------------------------- reorder */
+
+typedef struct {
+    int32_t init_size, final_size, init_cnt, final_cnt, rounds;
+} vnm_nm_stats;
+
+/* Replaces reorderUtil::run_onepair / _double / _reverse / _reverse_double
+ * (sched 0..3, reorder/reorder.h:1805-2474) and run (sched >= 8, :963-1100),
+ * called as main.cu:19-53 does (maxiter = 10).  The whole greedy runs on the
+ * GPU; the stats are the reference's return vector. */
+int vnm_nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *st);
+
+/* Replaces `python3 reordering.py --evalonly --m M --v V` (reordering.py:125-198). */
+int vnm_venom_eval(vnm_graph *g, int V, int32_t *inv, int32_t *inv_strict);
+
+typedef struct {
+    int32_t init_inv, init_inv_strict, final_inv, final_inv_strict, rounds;
+} vnm_venom_stats;
+
+/* Replaces `python3 reordering.py --m M --v V [--maxiter]` (reordering.py:
+ * 183-265) and the reload of temp.mtx that follows it (main.cu:268): the
+ * VENOM row reorder, then the loader's re-sort and the %.2f/%.1e text
+ * quantisation of the values. */
+int vnm_venom_reorder(vnm_graph *g, int V, int maxiter, vnm_venom_stats *st);
+
+/* The 16 numbers of the reference's stdout CSV line (main.cu:331-335). */
+typedef struct {
+    float   nrows, nnz, density;
+    float   init_size, final_size, init_cnt, final_cnt, improve_rate;
+    int32_t init_mb, final_mb, init_vnm, final_vnm;
+    int32_t nm_rounds, vnm_rounds, total_rounds;
+    float   milliseconds;
+} vnm_hybrid_stats;
+
+/* Replaces hybrid_reordering (main.cu:195-337) with the temp.mtx / output.txt
+ * process round trips done in device memory.  init_mb/init_vnm evaluate the
+ * input matrix (the reference reads a stale ./temp.mtx, main.cu:207). */
+int vnm_hybrid(vnm_graph *g, int N, int V, int rounds, int sched, vnm_hybrid_stats *st);
+
+/* Replaces find_best_2toX (main.cu:72-114) over schedules 0..nsched-1, each on
+ * a fresh copy of the graph; improve-rate ties go to the lowest index (the
+ * reference compares wall times). */
+int vnm_find_best_sched(vnm_graph *g, int N, int nsched, int32_t *best);
+
+/* ---------------------------------------------------------------- V:N:M */
+
+/* Entries kept by the V:N:M (N = 2) format: per V x M block, the 4-column
+ * subset maximising sum_rows min(2, |row & S|) (ties: lexicographically
+ * smallest subset), summed over blocks.  Not in the reference. */
+int vnm_retained_nnz(vnm_graph *g, int V, int N, int64_t *out);
+
+#define VNM_DT_F32   0
+#define VNM_DT_BF16  1
+#define VNM_DT_F16   2
+#define VNM_DT_FP8   3   /* OCP e4m3, per-block FP32 scale */
+
+typedef struct {
+    int32_t V, M, nrows, ncols;
+    int32_t nblocks;          /* stored V x M blocks                     */
+    int32_t dtype;
+    int64_t retained_nnz;
+    int64_t bytes;            /* device bytes of the packed format       */
+} vnm_packed_info;
+
+/* Replaces the cuSPARSELt prune/compress path (kernel/spmm.cu:402-447) and
+ * the BSR layout of bsr_wmma (kernel/spmm.cu:44-101): packs the current
+ * matrix into the VENOM format (per stored V x M block: 4 column selectors,
+ * V x 2 values + 2-bit indices). */
+int  vnm_pack(vnm_graph *g, int V, int dtype, vnm_packed **out);
+int  vnm_packed_info_get(const vnm_packed *p, vnm_packed_info *info);
+void vnm_packed_destroy(vnm_packed *p);
+
+/* Replaces evalCuSPARSELtMatmul / bsr_wmma's product (kernel/spmm.cu:44-101,
+ * 324-500): C[nrows x n] = A_vnm * B[ncols x n], B and C row-major device
+ * arrays (ldb, ldc in elements), B in b_dtype (F32, BF16 or FP8 with one FP32
+ * scale per 128 rows in b_scales), C FP32, FP32 accumulation. */
+int vnm_spmm(vnm_packed *p, const void *B, int ldb, int n, int b_dtype,
+             const float *b_scales, float *C, int ldc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VNM_H */

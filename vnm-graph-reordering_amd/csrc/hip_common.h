@@ -1,0 +1,105 @@
+// Device-side helpers shared by the HIP kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vnm_internal.h"
+
+namespace vnm {
+
+#define VNM_HIP_TRY(expr)                                                              \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            return set_error(VNM_E_HIP, "%s failed: %s (%s:%d)", #expr,                \
+                             hipGetErrorString(_e), __FILE__, __LINE__);               \
+    } while (0)
+
+#define VNM_LAUNCH_CHECK(name)                                                         \
+    do {                                                                               \
+        hipError_t _e = hipGetLastError();                                             \
+        if (_e != hipSuccess)                                                          \
+            return set_error(VNM_E_HIP, "launch of %s failed: %s", name,               \
+                             hipGetErrorString(_e));                                   \
+    } while (0)
+
+// Loads/stores of memory another thread of the same workgroup (or a later
+// kernel phase) updated through L2 atomics: agent-scope relaxed accesses skip
+// the non-coherent vector L1.
+template <typename T>
+__device__ __forceinline__ T ld_l2(const T *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void st_l2(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Priority key of a SwapPair under the reference Comparator
+// (reorder/swappair.h:112-120): Comparator(p1, p2) == key(p1) <= key(p2).
+// The gain sum dominates; equal sums compare right gains.
+__device__ __forceinline__ int64_t swap_key(int lg, int rg) {
+    return ((int64_t)(lg + rg) << 32) | (int64_t)(uint32_t)(rg ^ (int)0x80000000);
+}
+
+// std::priority_queue<SwapPair*> with the non-strict Comparator, emulated
+// exactly as libstdc++ pushes (__push_heap) and pops (pop_heap ->
+// __adjust_heap -> __push_heap).  The heap holds element indices; keys live in
+// a separate array.  `less(a, b)` is key[a] <= key[b].
+struct HeapRef {
+    uint8_t *h;          // heap slots (element indices)
+    const int64_t *key;  // keys by element index
+    int n;               // current size
+};
+
+__device__ __forceinline__ bool heap_less(const HeapRef &H, int a, int b) {
+    return H.key[a] <= H.key[b];
+}
+
+__device__ __forceinline__ void heap_sift_up(HeapRef &H, int hole, int top, int value) {
+    int parent = (hole - 1) / 2;
+    while (hole > top && heap_less(H, H.h[parent], value)) {
+        H.h[hole] = H.h[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    H.h[hole] = (uint8_t)value;
+}
+
+__device__ __forceinline__ void heap_push(HeapRef &H, int value) {
+    H.h[H.n] = (uint8_t)value;
+    H.n += 1;
+    heap_sift_up(H, H.n - 1, 0, value);
+}
+
+// Returns the popped element (the former top).
+__device__ __forceinline__ int heap_pop(HeapRef &H) {
+    int top_el = H.h[0];
+    int n = H.n;
+    if (n > 1) {
+        int last = n - 1;
+        int value = H.h[last];
+        H.h[last] = H.h[0];
+        // __adjust_heap(first, 0, len = last, value)
+        int len = last, hole = 0, second = 0;
+        while (second < (len - 1) / 2) {
+            second = 2 * (second + 1);
+            if (heap_less(H, H.h[second], H.h[second - 1])) second--;
+            H.h[hole] = H.h[second];
+            hole = second;
+        }
+        if ((len & 1) == 0 && second == (len - 2) / 2) {
+            second = 2 * (second + 1);
+            H.h[hole] = H.h[second - 1];
+            hole = second - 1;
+        }
+        heap_sift_up(H, hole, 0, value);
+    }
+    H.n = n - 1;
+    return top_el;
+}
+
+__device__ __forceinline__ int popc32(uint32_t x) { return __builtin_popcount(x); }
+
+}  // namespace vnm

@@ -1,0 +1,426 @@
+"""Python binding of libvnm (ctypes over the C ABI in include/vnm.h).
+
+Mirrors the reference's operator surface for the reorder path:
+  Mtx(path, M)                      -> mtx/Mtx.h:21 (loads, pads, uploads)
+  Mtx.write_to_file(path)           -> mtx/Mtx.h:412-435
+  ReorderUtil(mtx, maxiter).run_*() -> reorder/reorder.h:319-390, 963-2474
+  hybrid_reordering(...)            -> main.cu:195-337
+  find_best_2toX(...)               -> main.cu:72-114
+  eval_venom / VNM_reordering       -> main.cu:116-193 (reordering.py)
+Errors raise VnmError carrying the library's status code and message; a
+missing or unloadable libvnm.so raises at import of the first call, there is
+no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libvnm.so")
+CLI_PATH = os.path.join(os.path.dirname(_HERE), "bin", "spmm")
+
+OK, E_IO, E_FORMAT, E_ARG, E_HIP, E_OOM, E_UB_PATH, E_NODEV = 0, -1, -2, -3, -4, -5, -6, -7
+DT_F32, DT_BF16, DT_F16, DT_FP8 = 0, 1, 2, 3
+
+EXPORTS = [
+    "vnm_last_error", "vnm_mtx_read", "vnm_mtx_write", "vnm_coo_from_tuples", "vnm_coo_free",
+    "vnm_gen_er", "vnm_gen_rmat", "vnm_gen_chung_lu", "vnm_gen_normal",
+    "vnm_ctx_create", "vnm_ctx_destroy", "vnm_ctx_stream", "vnm_ctx_launches",
+    "vnm_graph_create", "vnm_graph_destroy", "vnm_graph_download", "vnm_graph_perm",
+    "vnm_locate", "vnm_swap_score", "vnm_pq_pop_order", "vnm_nm_reorder",
+    "vnm_venom_eval", "vnm_venom_reorder", "vnm_hybrid", "vnm_find_best_sched",
+    "vnm_retained_nnz", "vnm_pack", "vnm_packed_info_get", "vnm_packed_destroy", "vnm_spmm",
+]
+
+
+class VnmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Coo(ctypes.Structure):
+    _fields_ = [("nrows", ctypes.c_int32), ("ncols", ctypes.c_int32), ("nnz", ctypes.c_int64),
+                ("row", ctypes.POINTER(ctypes.c_int32)), ("col", ctypes.POINTER(ctypes.c_int32)),
+                ("val", ctypes.POINTER(ctypes.c_float))]
+
+
+class NMStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("init_size", "final_size", "init_cnt", "final_cnt", "rounds")]
+
+    def as_list(self):
+        return [self.init_size, self.final_size, self.init_cnt, self.final_cnt, self.rounds]
+
+
+class VenomStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("init_inv", "init_inv_strict", "final_inv", "final_inv_strict", "rounds")]
+
+    def as_list(self):
+        return [getattr(self, n) for n, _ in self._fields_]
+
+
+class HybridStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("nrows", "nnz", "density", "init_size", "final_size", "init_cnt",
+                 "final_cnt", "improve_rate")] + \
+               [(n, ctypes.c_int32) for n in
+                ("init_mb", "final_mb", "init_vnm", "final_vnm", "nm_rounds", "vnm_rounds",
+                 "total_rounds")] + [("milliseconds", ctypes.c_float)]
+
+    def as_tuple(self):
+        return tuple(getattr(self, n) for n, _ in self._fields_)
+
+    def csv(self):
+        # main.cu:331-335, std::cout default float formatting
+        def f(x):
+            return "%g" % x
+        t = self
+        return ", ".join([f(t.nrows), f(t.nnz), f(t.density), f(t.init_size), f(t.final_size),
+                          f(t.init_cnt), f(t.final_cnt), f(t.improve_rate), str(t.init_mb),
+                          str(t.final_mb), str(t.init_vnm), str(t.final_vnm), str(t.nm_rounds),
+                          str(t.vnm_rounds), str(t.total_rounds), f(t.milliseconds)])
+
+
+class PackedInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("V", "M", "nrows", "ncols", "nblocks", "dtype")] + \
+               [("retained_nnz", ctypes.c_int64), ("bytes", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libvnm.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise VnmError(E_HIP, f"{LIB_PATH} is not built (run vnm-graph-reordering_amd/build.py)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    i32p, vp = P(ctypes.c_int32), ctypes.c_void_p
+    L.vnm_last_error.restype = ctypes.c_char_p
+    L.vnm_mtx_read.argtypes = [ctypes.c_char_p, ctypes.c_int, P(Coo)]
+    L.vnm_mtx_write.argtypes = [ctypes.c_char_p, P(Coo)]
+    L.vnm_coo_from_tuples.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int, ctypes.c_int64,
+                                      i32p, i32p, P(ctypes.c_float), ctypes.c_int, P(Coo)]
+    L.vnm_coo_free.argtypes = [P(Coo)]
+    L.vnm_coo_free.restype = None
+    L.vnm_gen_er.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float,
+                             ctypes.c_float, ctypes.c_int, P(Coo)]
+    L.vnm_gen_rmat.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_uint64, ctypes.c_int, P(Coo)]
+    L.vnm_gen_chung_lu.argtypes = [ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_uint64, ctypes.c_int, P(Coo)]
+    L.vnm_gen_normal.argtypes = [ctypes.c_uint64, ctypes.c_int64, P(ctypes.c_float)]
+    L.vnm_gen_normal.restype = None
+    L.vnm_ctx_create.argtypes = [ctypes.c_int, P(vp)]
+    L.vnm_ctx_destroy.argtypes = [vp]
+    L.vnm_ctx_destroy.restype = None
+    L.vnm_ctx_stream.argtypes = [vp]
+    L.vnm_ctx_stream.restype = vp
+    L.vnm_ctx_launches.argtypes = [vp]
+    L.vnm_ctx_launches.restype = ctypes.c_int64
+    L.vnm_graph_create.argtypes = [vp, P(Coo), ctypes.c_int, P(vp)]
+    L.vnm_graph_destroy.argtypes = [vp]
+    L.vnm_graph_destroy.restype = None
+    L.vnm_graph_download.argtypes = [vp, P(Coo)]
+    L.vnm_graph_perm.argtypes = [vp, i32p]
+    L.vnm_locate.argtypes = [vp, ctypes.c_int, i32p, i32p]
+    L.vnm_swap_score.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, i32p]
+    L.vnm_pq_pop_order.argtypes = [vp, i32p, ctypes.c_int, ctypes.c_int, i32p]
+    L.vnm_nm_reorder.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(NMStats)]
+    L.vnm_venom_eval.argtypes = [vp, ctypes.c_int, i32p, i32p]
+    L.vnm_venom_reorder.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(VenomStats)]
+    L.vnm_hybrid.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             P(HybridStats)]
+    L.vnm_find_best_sched.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p]
+    L.vnm_retained_nnz.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(ctypes.c_int64)]
+    L.vnm_pack.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(vp)]
+    L.vnm_packed_info_get.argtypes = [vp, P(PackedInfo)]
+    L.vnm_packed_destroy.argtypes = [vp]
+    L.vnm_packed_destroy.restype = None
+    L.vnm_spmm.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != OK:
+        raise VnmError(rc, lib().vnm_last_error().decode())
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+# ------------------------------------------------------------------ host COO
+
+class HostCoo:
+    """Host COO in the reference's order (numpy arrays)."""
+
+    def __init__(self, nrows, ncols, row, col, val):
+        self.nrows, self.ncols = int(nrows), int(ncols)
+        self.row = np.ascontiguousarray(row, np.int32)
+        self.col = np.ascontiguousarray(col, np.int32)
+        self.val = np.ascontiguousarray(val, np.float32)
+
+    @property
+    def nnz(self):
+        return int(self.row.shape[0])
+
+    def c(self):
+        return Coo(self.nrows, self.ncols, self.nnz, _p(self.row, ctypes.c_int32),
+                   _p(self.col, ctypes.c_int32), _p(self.val, ctypes.c_float))
+
+    def copy(self):
+        return HostCoo(self.nrows, self.ncols, self.row.copy(), self.col.copy(), self.val.copy())
+
+
+def _take(c):
+    n = int(c.nnz)
+    out = HostCoo(c.nrows, c.ncols,
+                  np.ctypeslib.as_array(c.row, (max(n, 1),))[:n].copy(),
+                  np.ctypeslib.as_array(c.col, (max(n, 1),))[:n].copy(),
+                  np.ctypeslib.as_array(c.val, (max(n, 1),))[:n].copy())
+    lib().vnm_coo_free(ctypes.byref(c))
+    return out
+
+
+def mtx_read(path, M):
+    c = Coo()
+    _check(lib().vnm_mtx_read(str(path).encode(), M, ctypes.byref(c)))
+    return _take(c)
+
+
+def mtx_write(path, coo):
+    _check(lib().vnm_mtx_write(str(path).encode(), ctypes.byref(coo.c())))
+
+
+def coo_from_tuples(nrows, ncols, row, col, val, M, symmetric=False):
+    row = np.ascontiguousarray(row, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float32)
+    c = Coo()
+    _check(lib().vnm_coo_from_tuples(nrows, ncols, int(symmetric), row.shape[0],
+                                     _p(row, ctypes.c_int32), _p(col, ctypes.c_int32),
+                                     _p(val, ctypes.c_float), M, ctypes.byref(c)))
+    return _take(c)
+
+
+def gen_er(n, k, seed, lo=-1.0, hi=1.0, M=8):
+    c = Coo()
+    _check(lib().vnm_gen_er(n, k, seed, lo, hi, M, ctypes.byref(c)))
+    return _take(c)
+
+
+def gen_rmat(scale, undirected_edges, seed, a=0.57, b=0.19, c_=0.19, M=8):
+    c = Coo()
+    _check(lib().vnm_gen_rmat(scale, undirected_edges, a, b, c_, seed, M, ctypes.byref(c)))
+    return _take(c)
+
+
+def gen_chung_lu(n, gamma, mean_degree, seed, M=8):
+    c = Coo()
+    _check(lib().vnm_gen_chung_lu(n, gamma, mean_degree, seed, M, ctypes.byref(c)))
+    return _take(c)
+
+
+def gen_normal(seed, shape):
+    out = np.empty(int(np.prod(shape)), np.float32)
+    lib().vnm_gen_normal(seed, out.shape[0], _p(out, ctypes.c_float))
+    return out.reshape(shape)
+
+
+# ------------------------------------------------------------------ device
+
+class Context:
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().vnm_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def stream(self):
+        return lib().vnm_ctx_stream(self.h)
+
+    @property
+    def launches(self):
+        return lib().vnm_ctx_launches(self.h)
+
+    def close(self):
+        if self.h:
+            lib().vnm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pq_pop_order(self, scores, M):
+        scores = np.ascontiguousarray(scores, np.int32).reshape(-1, 2 * M * M)
+        out = np.zeros((scores.shape[0], M * M), np.int32)
+        _check(lib().vnm_pq_pop_order(self.h, _p(scores, ctypes.c_int32), scores.shape[0], M,
+                                      _p(out, ctypes.c_int32)))
+        return out
+
+
+class Graph:
+    """Device-resident matrix (the reference's Mtx<float> device side)."""
+
+    def __init__(self, ctx, coo, M):
+        self.ctx = ctx
+        self.M = M
+        h = ctypes.c_void_p()
+        _check(lib().vnm_graph_create(ctx.h, ctypes.byref(coo.c()), M, ctypes.byref(h)))
+        self.h = h
+        self.nrows, self.ncols = coo.nrows, coo.ncols
+
+    def close(self):
+        if self.h:
+            lib().vnm_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nseg(self):
+        return (self.nrows + self.M - 1) // self.M
+
+    def download(self):
+        c = Coo()
+        _check(lib().vnm_graph_download(self.h, ctypes.byref(c)))
+        return _take(c)
+
+    def perm(self):
+        out = np.zeros(self.nrows, np.int32)
+        _check(lib().vnm_graph_perm(self.h, _p(out, ctypes.c_int32)))
+        return out
+
+    def locate(self, N):
+        cnt = np.zeros(self.nseg, np.int32)
+        zero = np.zeros(self.nseg, np.int32)
+        _check(lib().vnm_locate(self.h, N, _p(cnt, ctypes.c_int32), _p(zero, ctypes.c_int32)))
+        return cnt, zero
+
+    def swap_score(self, N, seg1, seg2):
+        seg2 = np.ascontiguousarray(np.atleast_1d(seg2), np.int32)
+        out = np.zeros((seg2.shape[0], 2 * self.M * self.M), np.int32)
+        _check(lib().vnm_swap_score(self.h, N, seg1, _p(seg2, ctypes.c_int32), seg2.shape[0],
+                                    _p(out, ctypes.c_int32)))
+        return out
+
+    def nm_reorder(self, N, sched, maxiter=10):
+        st = NMStats()
+        _check(lib().vnm_nm_reorder(self.h, N, sched, maxiter, ctypes.byref(st)))
+        return st.as_list()
+
+    def venom_eval(self, V):
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().vnm_venom_eval(self.h, V, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def venom_reorder(self, V, maxiter=10):
+        st = VenomStats()
+        _check(lib().vnm_venom_reorder(self.h, V, maxiter, ctypes.byref(st)))
+        return st.as_list()
+
+    def hybrid(self, N, V, rounds, sched):
+        st = HybridStats()
+        _check(lib().vnm_hybrid(self.h, N, V, rounds, sched, ctypes.byref(st)))
+        return st
+
+    def find_best_sched(self, N, nsched=4):
+        b = ctypes.c_int32()
+        _check(lib().vnm_find_best_sched(self.h, N, nsched, ctypes.byref(b)))
+        return b.value
+
+    def retained_nnz(self, V, N=2):
+        out = ctypes.c_int64()
+        _check(lib().vnm_retained_nnz(self.h, V, N, ctypes.byref(out)))
+        return out.value
+
+    def pack(self, V, dtype):
+        return Packed(self, V, dtype)
+
+
+class Packed:
+    def __init__(self, graph, V, dtype):
+        h = ctypes.c_void_p()
+        _check(lib().vnm_pack(graph.h, V, dtype, ctypes.byref(h)))
+        self.h = h
+        self.graph = graph
+
+    def info(self):
+        i = PackedInfo()
+        _check(lib().vnm_packed_info_get(self.h, ctypes.byref(i)))
+        return i
+
+    def spmm(self, B_ptr, ldb, n, b_dtype, C_ptr, ldc, b_scales_ptr=None):
+        _check(lib().vnm_spmm(self.h, B_ptr, ldb, n, b_dtype, b_scales_ptr, C_ptr, ldc))
+
+    def close(self):
+        if self.h:
+            lib().vnm_packed_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ reference-shaped API
+
+class Mtx:
+    """Mtx<float>(mtxfile, aligned) (mtx/Mtx.h:21): load, pad, upload."""
+
+    def __init__(self, ctx, mtxfile=None, aligned=8, coo=None):
+        self.host = coo if coo is not None else mtx_read(mtxfile, aligned)
+        self.graph = Graph(ctx, self.host, aligned)
+
+    @property
+    def nrows(self):
+        return self.host.nrows
+
+    @property
+    def nnz(self):
+        return self.host.nnz
+
+    def write_to_file(self, path):
+        mtx_write(path, self.graph.download())
+
+
+class ReorderUtil:
+    """reorderUtil<float>(Mtx*, maxiter) (reorder/reorder.h:319-390)."""
+
+    def __init__(self, mtx, maxiter=10, N=2):
+        self.mtx, self.maxiter, self.N = mtx, maxiter, N
+
+    def _run(self, sched):
+        return self.mtx.graph.nm_reorder(self.N, sched, self.maxiter)
+
+    def run_onepair(self, verbose=False, stepmsg=False):
+        return self._run(0)
+
+    def run_onepair_double(self, verbose=False, stepmsg=False):
+        return self._run(1)
+
+    def run_onepair_reverse(self, verbose=False, stepmsg=False):
+        return self._run(2)
+
+    def run_onepair_reverse_double(self, verbose=False, stepmsg=False):
+        return self._run(3)
+
+    def run(self, verbose=False, stepmsg=False):
+        return self._run(8)
