@@ -135,3 +135,18 @@ def test_kat_temp_mtx(ctx):
     assert g.nm_reorder(2, 0) == [0, 0, 0, 0, 0]
     out = g.download()
     assert np.array_equal(out.row, m.row) and np.array_equal(out.col, m.col)
+
+
+def test_greedy_rmat_fullsize_golden(ctx):
+    # R-MAT 65,536 nodes, nnz 1,048,576 (seed 2): stats and relabelled COO
+    # equal the oracle's run, recorded by tests/golden/make_fullsize_golden.py
+    import hashlib
+    gold = json.load(open(os.path.join(GOLD, "fullsize.json")))["rmat16_seed2_nm_sched0"]
+    m = vnm.gen_rmat(16, 524288, 2)
+    g = vnm.Graph(ctx, m, 8)
+    assert g.nm_reorder(2, 0, 10) == gold["stats"]
+    out = g.download()
+    h = hashlib.sha256()
+    for a in (out.row, out.col, out.val):
+        h.update(np.ascontiguousarray(a).tobytes())
+    assert h.hexdigest() == gold["sha256"]

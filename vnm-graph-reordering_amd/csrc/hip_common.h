@@ -45,16 +45,18 @@ __device__ __forceinline__ int64_t swap_key(int lg, int rg) {
 
 // std::priority_queue<SwapPair*> with the non-strict Comparator, emulated
 // exactly as libstdc++ pushes (__push_heap) and pops (pop_heap ->
-// __adjust_heap -> __push_heap).  The heap holds element indices; keys live in
-// a separate array.  `less(a, b)` is key[a] <= key[b].
+// __adjust_heap -> __push_heap).  The heap holds element indices; each
+// element carries its dense rank among the list's keys (the number of
+// strictly smaller keys), so rank[a] <= rank[b] exactly when
+// key[a] <= key[b], i.e. when Comparator(a, b) holds.
 struct HeapRef {
-    uint8_t *h;          // heap slots (element indices)
-    const int64_t *key;  // keys by element index
-    int n;               // current size
+    uint8_t *h;           // heap slots (element indices)
+    const uint8_t *rank;  // ranks by element index
+    int n;                // current size
 };
 
 __device__ __forceinline__ bool heap_less(const HeapRef &H, int a, int b) {
-    return H.key[a] <= H.key[b];
+    return H.rank[a] <= H.rank[b];
 }
 
 __device__ __forceinline__ void heap_sift_up(HeapRef &H, int hole, int top, int value) {

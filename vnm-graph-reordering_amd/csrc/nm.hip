@@ -317,14 +317,19 @@ __global__ void k_pq_pop_order(const int32_t *__restrict__ scores, int count, in
                                int32_t *__restrict__ order) {
     extern __shared__ unsigned char smem[];
     const int MM = M * M;
-    int64_t *keys = (int64_t *)smem;
-    uint8_t *heaps = (uint8_t *)(keys + (size_t)blockDim.x * MM);
+    uint8_t *ranks = smem;
+    uint8_t *heaps = smem + (size_t)blockDim.x * MM;
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= count) return;
-    int64_t *key = keys + (size_t)threadIdx.x * MM;
-    for (int i = 0; i < MM; ++i)
-        key[i] = swap_key(scores[(size_t)q * 2 * MM + 2 * i], scores[(size_t)q * 2 * MM + 2 * i + 1]);
-    HeapRef H{heaps + (size_t)threadIdx.x * MM, key, 0};
+    const int32_t *sc = scores + (size_t)q * 2 * MM;
+    uint8_t *rank = ranks + (size_t)threadIdx.x * MM;
+    for (int i = 0; i < MM; ++i) {
+        int64_t ki = swap_key(sc[2 * i], sc[2 * i + 1]);
+        int r = 0;
+        for (int j = 0; j < MM; ++j) r += swap_key(sc[2 * j], sc[2 * j + 1]) < ki;
+        rank[i] = (uint8_t)r;
+    }
+    HeapRef H{heaps + (size_t)threadIdx.x * MM, rank, 0};
     for (int i = 0; i < MM; ++i) heap_push(H, i);
     for (int k = 0; k < MM; ++k) order[(size_t)q * MM + k] = heap_pop(H);
 }
@@ -349,6 +354,7 @@ __global__ void k_compose(int32_t *__restrict__ total, const int32_t *__restrict
 // ------------------------------------------------------------ greedy
 
 constexpr int GT = 1024;         // threads of the persistent workgroup
+constexpr int NW = GT / 64;      // its waves
 constexpr int KMAX = 9;          // primary + up to M targets (M <= 8)
 constexpr int MMAX = 8;
 constexpr int NPAIR_T = KMAX * (KMAX - 1) / 2 + 1;  // unordered pairs + one self pair
@@ -360,8 +366,7 @@ struct NMArgs {
     const int32_t *cptr, *crow;
     int32_t *perm, *inv, *cnt, *nzr;
     uint32_t *tab;       // KMAX slots of R bytes, zero on entry and exit
-    int32_t *both;       // NPAIR_T * 2 * 64 ints of scratch
-    int32_t *out;        // stats[5], status, ub, nswaps
+    int32_t *out;        // stats[5], status, work, nswaps
     int32_t *trace;      // optional: 9 ints per swap
     int64_t trace_cap;
 };
@@ -373,18 +378,26 @@ struct GreedyShared {
     int L, totcnt, round, maxiter, init_size, init_cnt, status;
     ColSpan cols[KMAX * MMAX];
     int pre[KMAX * MMAX + 1];
+    int cb[KMAX + 1];
     int E[KMAX][MMAX];
+    // per unordered pair (j < j2) and one self pair: the "both rows" sums
+    //   Bl[b1*M+b2] = sum over rows in both of y_b2 * (x_b1 ? A : B)
+    //   Br[b1*M+b2] = sum over rows in both of x_b1 * (y_b2 ? D : C)
+    int both[NPAIR_T][2][MMAX * MMAX];
     int newcnt[KMAX], newnzr[KMAX];
-    int64_t key[NQ][MMAX * MMAX];
+    uint8_t rank[NQ][MMAX * MMAX];
     uint8_t heap[NQ][MMAX * MMAX];
     uint8_t popped[NQ][POPP];
     int heapn[NQ];
     uint8_t vis[KMAX * MMAX];
     int swv1[64], swv2[64], swlg[64], swrg[64], swp[64], swt[64], swpc[64], swtc[64];
     int nsw;
-    uint64_t wtop[16][KMAX + 1];
-    uint64_t wbest[16];
+    uint64_t wtop[NW][KMAX + 1];
+    uint64_t wbest[NW];
+    uint64_t topk[KMAX + 1];
+    int64_t wkeys[NW][64];
     int64_t trace_len;
+    int64_t work;
 };
 
 __device__ __forceinline__ uint64_t key_fwd(int s, int c) {
@@ -398,17 +411,23 @@ __device__ __forceinline__ int seg_of_key(uint64_t k, bool fwd) {
     return fwd ? (int)(0xFFFFFFFFu - lo) : (int)lo;
 }
 
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+    uint32_t lo = __shfl((uint32_t)v, src, 64), hi = __shfl((uint32_t)(v >> 32), src, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+    const int lane = threadIdx.x & 63;
     for (int d = 32; d >= 1; d >>= 1) {
-        uint64_t o = __shfl_xor(v, d, 64);
+        uint64_t o = shfl_u64(v, lane ^ d);
         v = o > v ? o : v;
     }
     return v;
 }
 
-// Top-k (k <= KMAX+1 <= 10) keys of the block; result in out[0..k-1] (0 = none).
-// tk holds each thread's descending local top list.
-__device__ void block_topk(uint64_t (&tk)[KMAX + 1], int k, GreedyShared &S, uint64_t *out) {
+// Top-k (k <= KMAX + 1) keys of the block into S.topk (0 = none); tk holds
+// each thread's descending local list.
+__device__ void block_topk(uint64_t (&tk)[KMAX + 1], int k, GreedyShared &S) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int it = 0; it < k; ++it) {
         uint64_t m = wave_max_u64(tk[0]);
@@ -421,10 +440,9 @@ __device__ void block_topk(uint64_t (&tk)[KMAX + 1], int k, GreedyShared &S, uin
     }
     __syncthreads();
     if (w == 0) {
-        // lanes 0..15 hold the waves' lists
         uint64_t lst[KMAX + 1];
 #pragma unroll
-        for (int i = 0; i <= KMAX; ++i) lst[i] = (lane < 16 && i < k) ? S.wtop[lane][i] : 0ull;
+        for (int i = 0; i <= KMAX; ++i) lst[i] = (lane < NW && i < k) ? S.wtop[lane][i] : 0ull;
         for (int it = 0; it < k; ++it) {
             uint64_t m = wave_max_u64(lst[0]);
             if (m != 0 && lst[0] == m) {
@@ -432,7 +450,7 @@ __device__ void block_topk(uint64_t (&tk)[KMAX + 1], int k, GreedyShared &S, uin
                 for (int i = 0; i < KMAX; ++i) lst[i] = lst[i + 1];
                 lst[KMAX] = 0;
             }
-            if (lane == 0) out[it] = m;
+            if (lane == 0) S.topk[it] = m;
         }
     }
     __syncthreads();
@@ -444,7 +462,7 @@ __device__ uint64_t block_max_u64(uint64_t v, GreedyShared &S) {
     if (lane == 0) S.wbest[w] = v;
     __syncthreads();
     if (w == 0) {
-        uint64_t x = lane < 16 ? S.wbest[lane] : 0ull;
+        uint64_t x = lane < NW ? S.wbest[lane] : 0ull;
         x = wave_max_u64(x);
         if (lane == 0) S.wbest[0] = x;
     }
@@ -459,24 +477,24 @@ __device__ __forceinline__ int tri_index(int a, int b, int K) {
     return a * (2 * K - a - 1) / 2 + (b - a - 1);
 }
 
-// lgain/rgain of element e = b1*M + b2 of ordered pair (a, b) of slots.
-__device__ __forceinline__ void pair_gain(const GreedyShared &S, const int32_t *both, int K, int M,
-                                          int a, int b, int e, int &lg, int &rg) {
+// lgain/rgain of element e = b1*M + b2 of ordered pair (a, b) of slots:
+//   score(a, b)[b1][b2] = (E_a[b1] - Bl(a,b)[b1][b2], E_b[b2] - Br(a,b)[b1][b2])
+// and score(b, a) is score(a, b) transposed with the two gains exchanged.
+__device__ __forceinline__ void pair_gain(const GreedyShared &S, int K, int M, int a, int b, int e,
+                                          int &lg, int &rg) {
     const int b1 = e / M, b2 = e % M;
-    const int stride = 2 * MMAX * MMAX;
     if (a == b) {
-        const int32_t *p = both + (size_t)(NPAIR_T - 1) * stride;
-        lg = S.E[a][b1] - ld_l2(&p[e]);
-        rg = S.E[a][b2] - ld_l2(&p[MMAX * MMAX + e]);
+        lg = S.E[a][b1] - S.both[NPAIR_T - 1][0][e];
+        rg = S.E[a][b2] - S.both[NPAIR_T - 1][1][e];
     } else if (a < b) {
-        const int32_t *p = both + (size_t)tri_index(a, b, K) * stride;
-        lg = S.E[a][b1] - ld_l2(&p[e]);
-        rg = S.E[b][b2] - ld_l2(&p[MMAX * MMAX + e]);
+        const int pi = tri_index(a, b, K);
+        lg = S.E[a][b1] - S.both[pi][0][e];
+        rg = S.E[b][b2] - S.both[pi][1][e];
     } else {
-        const int32_t *p = both + (size_t)tri_index(b, a, K) * stride;
+        const int pi = tri_index(b, a, K);
         const int t = b2 * M + b1;
-        lg = S.E[a][b1] - ld_l2(&p[MMAX * MMAX + t]);
-        rg = S.E[b][b2] - ld_l2(&p[t]);
+        lg = S.E[a][b1] - S.both[pi][1][t];
+        rg = S.E[b][b2] - S.both[pi][0][t];
     }
 }
 
@@ -496,102 +514,156 @@ __device__ void q_pair(int q, int K, int &a, int &b) {
     b = 1 + r + (r + 1 >= a ? 1 : 0);
 }
 
-// Round phases that touch every entry of the selected segments' columns.
-__device__ void greedy_score_phase(const NMArgs &A, GreedyShared &S) {
-    const int M = A.M, N = A.N, R = A.R, K = S.K;
-    const int nc = K * M;
-    // zero per-round accumulators
-    for (int t = threadIdx.x; t < KMAX * MMAX; t += blockDim.x) (&S.E[0][0])[t] = 0;
-    const int nb = NPAIR_T * 2 * MMAX * MMAX;
-    for (int t = threadIdx.x; t < nb; t += blockDim.x) st_l2(&A.both[t], 0);
-    int total = table_columns(S.sel, K, M, A.inv, A.cptr, S.cols, S.pre);
-    table_scatter(A.tab, nullptr, M, R, S.cols, S.pre, nc, total, A.crow);
+// Decode entry e of slot j: its column bit b and row r.
+__device__ __forceinline__ void slot_entry(const GreedyShared &S, const int32_t *crow, int j, int M,
+                                           int e, int &b, int &r) {
+    b = entry_col(S.pre + j * M, M, e);
+    const int t = j * M + b;
+    r = crow[S.cols[t].start + (e - S.pre[t])];
+}
+
+// Chunk table of the round: slot j owns 64-entry chunks [cb[j], cb[j+1]).
+__device__ int chunk_table(GreedyShared &S, int K, int M) {
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int j = 0; j < K; ++j) {
+            S.cb[j] = acc;
+            acc += (S.pre[j * M + M] - S.pre[j * M] + 63) / 64;
+        }
+        S.cb[K] = acc;
+    }
     __syncthreads();
-    const int stride = 2 * MMAX * MMAX;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        int t = entry_col(S.pre, nc, e);
-        int j = t / M, b = t % M;
-        int r = A.crow[S.cols[t].start + (e - S.pre[t])];
-        uint32_t l = slot_byte(A.tab, j, R, r);
-        if ((l >> (M - 1 - b)) != 1u) continue;  // not the row's owner entry
+    return S.cb[K];
+}
+
+__device__ __forceinline__ int chunk_slot(const GreedyShared &S, int K, int c) {
+    int j = 0;
+    while (j + 1 < K && S.cb[j + 1] <= c) ++j;
+    return j;
+}
+
+// Gains of every pair of the round's segments (the GetSegSwapScore sums,
+// reorder/reorder.h:75-131, in closed form).  Waves take 64-entry chunks of
+// one slot each; the owner entries of a chunk are folded in with ballots, so
+// each (b1, b2) counter takes one LDS atomic per chunk instead of one per row,
+// and the partner-slot bytes of a chunk are loaded together.
+__device__ void greedy_score_phase(const NMArgs &A, GreedyShared &S) {
+    const int M = A.M, N = A.N, R = A.R, K = S.K, MM = M * M;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int nc = K * M;
+    for (int t = threadIdx.x; t < KMAX * MMAX; t += GT) (&S.E[0][0])[t] = 0;
+    for (int t = threadIdx.x; t < NPAIR_T * 2 * MMAX * MMAX; t += GT) (&S.both[0][0][0])[t] = 0;
+    const int total = table_columns(S.sel, K, M, A.inv, A.cptr, S.cols, S.pre);
+    table_scatter(A.tab, nullptr, M, R, S.cols, S.pre, nc, total, A.crow);
+    const int nch = chunk_table(S, K, M);  // (barrier inside)
+    const int b1 = lane / M, b2 = lane % M;
+    for (int c = w; c < nch; c += NW) {
+        const int j = chunk_slot(S, K, c);
+        const int e1 = S.pre[j * M + M];
+        const int e = S.pre[j * M] + (c - S.cb[j]) * 64 + lane;
+        uint32_t l = 0;
+        int r = 0;
+        bool act = false;
+        if (e < e1) {
+            int b;
+            slot_entry(S, A.crow, j, M, e, b, r);
+            l = slot_byte(A.tab, j, R, r);
+            act = (l >> (M - 1 - b)) == 1u;  // the row's leading-bit entry
+            if (!act) l = 0;
+        }
+        uint32_t rrs[KMAX];
+#pragma unroll
+        for (int j2 = 0; j2 < KMAX; ++j2)
+            rrs[j2] = (act && j2 > j && j2 < K) ? slot_byte(A.tab, j2, R, r) : 0u;
         const int pl = popc32(l);
-        if (pl == N + 1)
-            for (int q = 0; q < M; ++q)
-                if ((l >> (M - 1 - q)) & 1u) atomicAdd(&S.E[j][q], 1);
-        const int Aq = pl == N + 1, Bq = pl == N;
-        // pairs (j, j2) with j2 > j, rows present in both segments
-        for (int j2 = j + 1; j2 <= K; ++j2) {
-            uint32_t rr;
-            int32_t *p;
-            if (j2 == K) {
-                if (!S.self_pair) break;
-                rr = l;
-                p = A.both + (size_t)(NPAIR_T - 1) * stride;
-            } else {
-                rr = slot_byte(A.tab, j2, R, r);
-                if (rr == 0) continue;
-                p = A.both + (size_t)tri_index(j, j2, K) * stride;
-            }
+        const uint64_t Am = __ballot(act && pl == N + 1), Bm = __ballot(act && pl == N);
+        uint64_t Lb1 = 0, Lb2 = 0, Lme = 0;
+        for (int k = 0; k < M; ++k) {
+            const uint64_t m = __ballot((l >> (M - 1 - k)) & 1u);
+            if (k == b1) Lb1 = m;
+            if (k == b2) Lb2 = m;
+            if (k == lane) Lme = m;
+        }
+        if (lane < M) {
+            const int v = __popcll(Lme & Am);
+            if (v) atomicAdd(&S.E[j][lane], v);
+        }
+#pragma unroll
+        for (int j2 = 0; j2 < KMAX; ++j2) {
+            if (j2 <= j || j2 >= K) continue;
+            const uint32_t rr = rrs[j2];
             const int pr = popc32(rr);
-            const int Cq = pr == N, Dq = pr == N + 1;
-            // S_l[b1][b2] -= y(b2) * (x(b1) ? A : B);  S_r[b1][b2] -= x(b1) * (y(b2) ? D : C)
-            if (Aq | Bq)
-                for (int b2 = 0; b2 < M; ++b2) {
-                    if (!((rr >> (M - 1 - b2)) & 1u)) continue;
-                    for (int b1 = 0; b1 < M; ++b1) {
-                        int x = (l >> (M - 1 - b1)) & 1u;
-                        if (x ? Aq : Bq) atomicAdd(&p[b1 * M + b2], 1);
-                    }
-                }
-            if (Cq | Dq)
-                for (int b1 = 0; b1 < M; ++b1) {
-                    if (!((l >> (M - 1 - b1)) & 1u)) continue;
-                    for (int b2 = 0; b2 < M; ++b2) {
-                        int y = (rr >> (M - 1 - b2)) & 1u;
-                        if (y ? Dq : Cq) atomicAdd(&p[MMAX * MMAX + b1 * M + b2], 1);
-                    }
-                }
+            const uint64_t Cm = __ballot(pr == N), Dm = __ballot(pr == N + 1);
+            uint64_t Rb = 0;
+            for (int k = 0; k < M; ++k) {
+                const uint64_t m = __ballot((rr >> (M - 1 - k)) & 1u);
+                if (k == b2) Rb = m;
+            }
+            if (lane < MM) {
+                const int vl = __popcll(Rb & ((Lb1 & Am) | (~Lb1 & Bm)));
+                const int vr = __popcll(Lb1 & ((Rb & Dm) | (~Rb & Cm)));
+                const int pi = tri_index(j, j2, K);
+                if (vl) atomicAdd(&S.both[pi][0][lane], vl);
+                if (vr) atomicAdd(&S.both[pi][1][lane], vr);
+            }
+        }
+        if (S.self_pair && lane < MM) {  // seg1 == seg2: r = l for every row
+            const int vl = __popcll(Lb2 & ((Lb1 & Am) | (~Lb1 & Bm)));
+            const int vr = __popcll(Lb1 & ((Lb2 & Am) | (~Lb2 & Bm)));
+            if (vl) atomicAdd(&S.both[NPAIR_T - 1][0][lane], vl);
+            if (vr) atomicAdd(&S.both[NPAIR_T - 1][1][lane], vr);
         }
     }
     __syncthreads();
     table_clear(A.tab, nullptr, M, R, S.cols, S.pre, nc, total, A.crow);
+    if (threadIdx.x == 0) S.work += total;
     __syncthreads();
 }
 
+// cnt / nzr of the round's segments after the swaps (locate_invalid_segment).
 __device__ void greedy_recount_phase(const NMArgs &A, GreedyShared &S) {
     const int M = A.M, N = A.N, R = A.R, K = S.K;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int nc = K * M;
     if (threadIdx.x < KMAX) { S.newcnt[threadIdx.x] = 0; S.newnzr[threadIdx.x] = 0; }
-    int total = table_columns(S.sel, K, M, A.inv, A.cptr, S.cols, S.pre);
+    const int total = table_columns(S.sel, K, M, A.inv, A.cptr, S.cols, S.pre);
     table_scatter(A.tab, nullptr, M, R, S.cols, S.pre, nc, total, A.crow);
-    __syncthreads();
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        int t = entry_col(S.pre, nc, e);
-        int j = t / M, b = t % M;
-        int r = A.crow[S.cols[t].start + (e - S.pre[t])];
-        uint32_t l = slot_byte(A.tab, j, R, r);
-        if ((l >> (M - 1 - b)) != 1u) continue;
-        atomicAdd(&S.newnzr[j], 1);
-        if (popc32(l) > N) atomicAdd(&S.newcnt[j], 1);
+    const int nch = chunk_table(S, K, M);
+    for (int c = w; c < nch; c += NW) {
+        const int j = chunk_slot(S, K, c);
+        const int e1 = S.pre[j * M + M];
+        const int e = S.pre[j * M] + (c - S.cb[j]) * 64 + lane;
+        bool act = false;
+        uint32_t l = 0;
+        if (e < e1) {
+            int b, r;
+            slot_entry(S, A.crow, j, M, e, b, r);
+            l = slot_byte(A.tab, j, R, r);
+            act = (l >> (M - 1 - b)) == 1u;
+        }
+        const uint64_t o = __ballot(act), v = __ballot(act && popc32(l) > N);
+        if (lane == 0) {
+            if (o) atomicAdd(&S.newnzr[j], (int)__popcll(o));
+            if (v) atomicAdd(&S.newcnt[j], (int)__popcll(v));
+        }
     }
     __syncthreads();
     table_clear(A.tab, nullptr, M, R, S.cols, S.pre, nc, total, A.crow);
+    if (threadIdx.x == 0) S.work += total;
     __syncthreads();
 }
 
 // First element of ordered pair q (slots a -> b) in pop order whose two
 // vertices are unvisited; -1 when the queue runs empty.
-__device__ int first_free(GreedyShared &S, int q, int a, int b, int M, int &last) {
+__device__ int first_free(GreedyShared &S, int q, int a, int b, int M) {
     const int npre = min(POPP, M * M);
     for (int k = 0; k < npre; ++k) {
         int e = S.popped[q][k];
-        last = e;
         if (!S.vis[a * M + e / M] && !S.vis[b * M + e % M]) return e;
     }
-    HeapRef H{S.heap[q], S.key[q], S.heapn[q]};
+    HeapRef H{S.heap[q], S.rank[q], S.heapn[q]};
     while (H.n > 0) {
         int e = heap_pop(H);
-        last = e;
         if (!S.vis[a * M + e / M] && !S.vis[b * M + e % M]) { S.heapn[q] = H.n; return e; }
     }
     S.heapn[q] = 0;
@@ -604,9 +676,9 @@ __device__ __forceinline__ bool list_before(int sa, int ca, int sb, int cb, bool
     return ca != cb ? ca < cb : sa > sb;
 }
 
-__device__ void greedy_apply(const NMArgs &A, GreedyShared &S, bool rev, bool drain) {
-    // single thread
-    const int M = A.M, K = S.K;
+// The sequential part of one round (reorder/reorder.h:1837-1932), one thread.
+__device__ void greedy_apply(GreedyShared &S, int M, bool rev, bool drain) {
+    const int K = S.K;
     S.nsw = 0;
     for (int i = 0; i < K * M; ++i) S.vis[i] = 0;
     auto add_swap = [&](int a, int b, int e, int lg, int rg, int pc, int tc) {
@@ -623,7 +695,7 @@ __device__ void greedy_apply(const NMArgs &A, GreedyShared &S, bool rev, bool dr
         const int b = S.self_pair ? 0 : 1;
         const int e = S.popped[0][0];
         int lg, rg;
-        pair_gain(S, A.both, K, M, 0, b, e, lg, rg);
+        pair_gain(S, K, M, 0, b, e, lg, rg);
         add_swap(0, b, e, lg, rg, 0, 0);
         return;
     }
@@ -638,16 +710,15 @@ __device__ void greedy_apply(const NMArgs &A, GreedyShared &S, bool rev, bool dr
             if (pc <= 0 || all) break;  // reorder/reorder.h:1878
             const int t = tgt[i];
             const int q = q_index(cur, t, K);
-            int last = 0;
-            int e = first_free(S, q, cur, t, M, last);
+            int e = first_free(S, q, cur, t, M);
             if (e < 0) {
-                // top() of an empty queue: the reference reads the last popped
-                // pair, which has a visited endpoint (reorder/reorder.h:1908)
+                // top() of an empty queue: the reference would read the last
+                // popped pair, which has a visited endpoint (:1908)
                 S.status = VNM_E_UB_PATH;
                 return;
             }
             int lg, rg;
-            pair_gain(S, A.both, K, M, cur, t, e, lg, rg);
+            pair_gain(S, K, M, cur, t, e, lg, rg);
             S.vis[cur * M + e / M] = 1;
             S.vis[t * M + e % M] = 1;
             pc -= lg;
@@ -656,7 +727,6 @@ __device__ void greedy_apply(const NMArgs &A, GreedyShared &S, bool rev, bool dr
             if (tc > 0) { upd[nu] = t; ucnt[nu] = tc; ++nu; }
         }
         if (nu < 2) break;  // while (inv_seg_list.size() >= 2)
-        // order the surviving targets as the invalid list would (insertion)
         for (int i = 1; i < nu; ++i) {
             int s = upd[i], c = ucnt[i], k = i - 1;
             while (k >= 0 && list_before(S.sel[s], c, S.sel[upd[k]], ucnt[k], rev)) {
@@ -678,6 +748,7 @@ __device__ void greedy_apply(const NMArgs &A, GreedyShared &S, bool rev, bool dr
 __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
     __shared__ GreedyShared S;
     const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
     const int M = A.M, R = A.R, nseg = A.nseg, MM = M * M;
     const bool plain = A.sched >= 8;
     const bool rev = (A.sched == 2 || A.sched == 3);
@@ -686,31 +757,28 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
     const int min_size = plain ? 2 : 1;
 
     // initial list size and count (reorder/reorder.h:1808-1810)
+    if (tid == 0) { S.L = 0; S.totcnt = 0; S.work = 0; }
+    __syncthreads();
     {
-        uint64_t lsum = 0, csum = 0;
+        int lsum = 0, csum = 0;
         for (int s = tid; s < nseg; s += GT) {
             int c = A.cnt[s];
             lsum += c != 0;
-            csum += (uint64_t)c;
+            csum += c;
         }
-        uint64_t a = block_max_u64(0, S);  // warm the shared slots
-        (void)a;
-        // reuse wave reductions via atomics on shared ints
-        if (tid == 0) { S.L = 0; S.totcnt = 0; }
-        __syncthreads();
-        if (lsum) atomicAdd(&S.L, (int)lsum);
-        if (csum) atomicAdd(&S.totcnt, (int)csum);
-        __syncthreads();
-        if (tid == 0) {
-            S.init_size = S.L;
-            S.init_cnt = S.totcnt;
-            S.maxiter = A.maxiter > S.L ? A.maxiter : S.L;  // :1819
-            S.round = 1;
-            S.status = 0;
-            S.trace_len = 0;
-        }
-        __syncthreads();
+        if (lsum) atomicAdd(&S.L, lsum);
+        if (csum) atomicAdd(&S.totcnt, csum);
     }
+    __syncthreads();
+    if (tid == 0) {
+        S.init_size = S.L;
+        S.init_cnt = S.totcnt;
+        S.maxiter = A.maxiter > S.L ? A.maxiter : S.L;  // :1819
+        S.round = 1;
+        S.status = 0;
+        S.trace_len = 0;
+    }
+    __syncthreads();
 
     while (true) {
         if (tid == 0) S.cont = (S.round <= S.maxiter && S.L >= min_size && S.status == 0);
@@ -719,12 +787,12 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
 
         // ---- phase A: the round's segments
         if (!plain && S.L == 1) {
-            // the single invalid segment and the segment with the most empty
+            // the single invalid segment, and the segment with the most empty
             // rows, ties to the larger id (reorder/reorder.h:767-797)
             uint64_t best_l = 0, best_z = 0;
             for (int s = tid; s < nseg; s += GT) {
                 int c = ld_l2(&A.cnt[s]);
-                if (c != 0) best_l = ((uint64_t)1 << 32) | (uint32_t)s | 0x8000000000000000ull;
+                if (c != 0) best_l = 0x8000000000000000ull | (uint32_t)s;
                 uint64_t z = ((uint64_t)(uint32_t)(R - ld_l2(&A.nzr[s])) << 32) | (uint32_t)s;
                 best_z = z > best_z ? z : best_z;
             }
@@ -759,8 +827,7 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
                 }
             }
             uint64_t bl = block_max_u64(best_l, S);
-            __shared__ uint64_t topk[KMAX + 1];
-            block_topk(tk, k, S, topk);
+            block_topk(tk, k, S);
             if (tid == 0) {
                 int primary = seg_of_key(bl, !rev);
                 S.size1 = 0;
@@ -769,8 +836,8 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
                 S.selcnt[0] = ld_l2(&A.cnt[primary]);
                 int K = 1;
                 for (int i = 0; i < k && K < k; ++i) {
-                    if (topk[i] == 0) break;
-                    int s = seg_of_key(topk[i], tfwd);
+                    if (S.topk[i] == 0) break;
+                    int s = seg_of_key(S.topk[i], tfwd);
                     if (s == primary) continue;
                     S.sel[K] = s;
                     S.selcnt[K] = ld_l2(&A.cnt[s]);
@@ -784,21 +851,32 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
         // ---- phase C: gains of every pair of the round's segments
         greedy_score_phase(A, S);
 
-        // ---- phase D: priority queues (keys, pushes, first pops)
+        // ---- phase D: priority queues: dense ranks (one wave per list), then
+        // one thread per list pushes all pairs and pops the first POPP
         {
             const int K = S.K;
             const int nq = S.size1 ? 1 : (K - 1) * (K - 1);
-            if (tid < nq) {
+            for (int q = w; q < nq; q += NW) {
                 int a, b;
                 if (S.size1) { a = 0; b = S.self_pair ? 0 : 1; }
-                else q_pair(tid, K, a, b);
-                int64_t *key = S.key[tid];
-                for (int e = 0; e < MM; ++e) {
+                else q_pair(q, K, a, b);
+                int64_t key = 0;
+                if (lane < MM) {
                     int lg, rg;
-                    pair_gain(S, A.both, K, M, a, b, e, lg, rg);
-                    key[e] = swap_key(lg, rg);
+                    pair_gain(S, K, M, a, b, lane, lg, rg);
+                    key = swap_key(lg, rg);
+                    S.wkeys[w][lane] = key;
                 }
-                HeapRef H{S.heap[tid], key, 0};
+                __builtin_amdgcn_wave_barrier();
+                int rnk = 0;
+                if (lane < MM)
+                    for (int j = 0; j < MM; ++j) rnk += S.wkeys[w][j] < key;
+                if (lane < MM) S.rank[q][lane] = (uint8_t)rnk;
+                __builtin_amdgcn_wave_barrier();
+            }
+            __syncthreads();
+            if (tid < nq) {
+                HeapRef H{S.heap[tid], S.rank[tid], 0};
                 for (int e = 0; e < MM; ++e) heap_push(H, e);
                 const int npre = min(POPP, MM);
                 for (int k = 0; k < npre; ++k) S.popped[tid][k] = (uint8_t)heap_pop(H);
@@ -808,7 +886,7 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
         }
 
         // ---- phase E: the sequential greedy (one thread)
-        if (tid == 0) greedy_apply(A, S, rev, drain);
+        if (tid == 0) greedy_apply(S, M, rev, drain);
         __syncthreads();
         if (S.status != 0) break;
 
@@ -834,7 +912,6 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
         }
         __syncthreads();
         if (tid == 0) S.trace_len += S.nsw;
-        __threadfence_block();
         __syncthreads();
 
         // ---- phase G: recount the touched segments (locate_invalid_segment)
@@ -860,7 +937,7 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
         A.out[3] = S.totcnt;
         A.out[4] = S.round - 1;
         A.out[5] = S.status;
-        A.out[6] = 0;
+        A.out[6] = (int32_t)min(S.work, (int64_t)0x7FFFFFFF);
         A.out[7] = (int32_t)S.trace_len;
     }
 }
@@ -979,7 +1056,7 @@ int nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *stats)
     NMArgs A;
     A.R = g->nrows; A.M = g->M; A.N = N; A.nseg = g->nseg(); A.sched = sched; A.maxiter = maxiter;
     A.cptr = g->cptr.p; A.crow = g->crow.p; A.perm = g->perm.p; A.inv = g->inv.p;
-    A.cnt = g->cnt.p; A.nzr = g->nzr.p; A.tab = g->tab.p; A.both = g->ws.p; A.out = out;
+    A.cnt = g->cnt.p; A.nzr = g->nzr.p; A.tab = g->tab.p; A.out = out;
     A.trace = nullptr; A.trace_cap = 0;
     hipLaunchKernelGGL(k_nm_greedy, dim3(1), dim3(GT), 0, st, A);
     g->ctx->launches++;
@@ -1070,7 +1147,7 @@ int vnm_pq_pop_order(vnm_ctx *ctx, const int32_t *scores, int count, int M, int3
     hipStream_t st = ctx->stream;
     VNM_HIP_TRY(hipMemcpyAsync(ds.p, scores, (size_t)count * 2 * MM * 4, hipMemcpyHostToDevice, st));
     const int bt = 64;
-    size_t shm = (size_t)bt * MM * (8 + 1);
+    size_t shm = (size_t)bt * MM * 2;
     hipLaunchKernelGGL(k_pq_pop_order, dim3((count + bt - 1) / bt), dim3(bt), shm, st, ds.p, count,
                        M, dord.p);
     ctx->launches++;
