@@ -88,6 +88,7 @@ def lib():
         L.orc_quant_py_roundtrip.argtypes = [ctypes.c_float]
         L.orc_quant_py_roundtrip.restype = ctypes.c_float
         L.orc_ub_events.restype = ctypes.c_long
+        L.orc_reload.argtypes = [P(Coo), ctypes.c_int]
         _lib = L
     return _lib
 
@@ -257,6 +258,16 @@ def quant_cpp_write(v):
 
 def quant_py_roundtrip(v):
     return lib().orc_quant_py_roundtrip(v)
+
+
+def reload(m, M):
+    """The loader-side effects of the temp.mtx round trip (in place)."""
+    c = m.c()
+    _check(lib().orc_reload(ctypes.byref(c), M), "reload")
+    n = int(c.nnz)
+    m.nrows, m.ncols = c.nrows, c.ncols
+    m.row, m.col, m.val = m.row[:n].copy(), m.col[:n].copy(), m.val[:n].copy()
+    return m
 
 
 def ub_events():

@@ -88,6 +88,9 @@ int orc_venom_eval(const orc_coo *coo, int M, int V, int32_t *inv,
  * positions are the previous ones, as the script writes them. */
 int orc_venom_reorder(orc_coo *coo, int M, int V, int maxiter, int32_t *stats);
 
+/* The C++ reload of temp.mtx after the VENOM script (main.cu:268). */
+int orc_reload(orc_coo *coo, int M);
+
 /* ---- Hybrid driver (main.cu:195-337) */
 typedef struct {
     float   nrows, nnz, density;

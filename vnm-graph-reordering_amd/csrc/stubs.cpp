@@ -2,10 +2,6 @@
 #include "vnm_internal.h"
 using namespace vnm;
 extern "C" {
-int vnm_venom_eval(vnm_graph *, int, int32_t *, int32_t *) { return set_error(VNM_E_ARG, "vnm_venom_eval: not built"); }
-int vnm_venom_reorder(vnm_graph *, int, int, vnm_venom_stats *) { return set_error(VNM_E_ARG, "vnm_venom_reorder: not built"); }
-int vnm_hybrid(vnm_graph *, int, int, int, int, vnm_hybrid_stats *) { return set_error(VNM_E_ARG, "vnm_hybrid: not built"); }
-int vnm_find_best_sched(vnm_graph *, int, int, int32_t *) { return set_error(VNM_E_ARG, "not built"); }
 int vnm_retained_nnz(vnm_graph *, int, int, int64_t *) { return set_error(VNM_E_ARG, "not built"); }
 int vnm_pack(vnm_graph *, int, int, vnm_packed **) { return set_error(VNM_E_ARG, "not built"); }
 int vnm_packed_info_get(const vnm_packed *, vnm_packed_info *) { return set_error(VNM_E_ARG, "not built"); }
