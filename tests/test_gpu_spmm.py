@@ -1,0 +1,139 @@
+"""GPU parity of the V:N:M packer and SpMM.
+
+Retained nnz is integer work: bit-exact against the oracle.  The SpMM is
+compared with the oracle's FP64 product of the same pruned matrix (FP32
+values, FP32 B).  Tolerances (relative, normwise ||dC||_F / ||C||_F and
+max|dC| / max|C|):
+  A FP32, B FP32 : 1e-5   (FP32 accumulation only)
+  A BF16, B BF16 : 1e-2   (bf16 rounding of both operands)
+  A FP8,  B FP8  : 1e-1   (e4m3, per-block / per-128-row scales)
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import vnm
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = vnm.Context(0)
+    yield c
+    c.close()
+
+
+def _om(m):
+    return oracle.Matrix(m.nrows, m.ncols, m.row, m.col, m.val)
+
+
+def _rand(rng, R, dens, M, zero_frac=0.05):
+    mask = rng.random((R, R)) < dens
+    r, c = np.nonzero(mask)
+    v = rng.uniform(-1, 1, r.shape[0]).astype(np.float32)
+    v[rng.random(r.shape[0]) < zero_frac] = 0.0
+    return vnm.coo_from_tuples(R, R, r, c, v, M)
+
+
+def test_retained_nnz_matches_oracle(ctx):
+    rng = np.random.default_rng(41)
+    cases = [(_rand(rng, int(rng.integers(1, 30)) * M, rng.uniform(0.02, 0.6), M), M)
+             for M in (4, 8) for _ in range(5)]
+    cases.append((vnm.gen_er(2048, 41943, 1), 8))
+    cases.append((vnm.gen_rmat(13, 60000, 3), 8))
+    cases.append((vnm.gen_rmat(12, 40000, 4, M=4), 4))
+    cases.append((vnm.mtx_read(os.path.join(GOLD, "mtx", "lund_b.mtx"), 8), 8))
+    for m, M in cases:
+        g = vnm.Graph(ctx, m, M)
+        for V in (1, 8, 64, 128):
+            assert g.retained_nnz(V) == oracle.retained_nnz(_om(m), V, 2, M), (M, V)
+
+
+def _errs(C, ref):
+    d = C.astype(np.float64) - ref
+    fro = np.linalg.norm(d) / max(np.linalg.norm(ref), 1e-300)
+    mx = np.abs(d).max() / max(np.abs(ref).max(), 1e-300)
+    return fro, mx
+
+
+def _e4m3_rows(B):
+    """FP8 e4m3 B with one FP32 scale per 128 rows (absmax / 448)."""
+    K = B.shape[0]
+    nb = (K + 127) // 128
+    scales = np.ones(nb, np.float32)
+    Bs = np.empty_like(B)
+    for i in range(nb):
+        blk = B[i * 128:(i + 1) * 128]
+        a = np.abs(blk).max()
+        scales[i] = a / 448.0 if a > 0 else 1.0
+        Bs[i * 128:(i + 1) * 128] = blk / scales[i]
+    q = torch.from_numpy(Bs).to(torch.float8_e4m3fn)
+    return q, scales
+
+
+def _run(ctx, m, M, V, n, adt, bdt, seed=5):
+    g = vnm.Graph(ctx, m, M)
+    p = g.pack(V, adt)
+    info = p.info()
+    B = vnm.gen_normal(seed, (m.ncols, n))
+    dev = torch.device("cuda:0")
+    scales_d = None
+    if bdt == vnm.DT_F32:
+        Bd = torch.from_numpy(B).to(dev)
+    elif bdt == vnm.DT_BF16:
+        Bd = torch.from_numpy(B).to(torch.bfloat16).to(dev)
+    else:
+        q, sc = _e4m3_rows(B)
+        Bd = q.to(dev)
+        scales_d = torch.from_numpy(sc).to(dev)
+    C = torch.full((m.nrows, n), float("nan"), dtype=torch.float32, device=dev)
+    p.spmm(Bd.data_ptr(), n, n, bdt, C.data_ptr(), n,
+           scales_d.data_ptr() if scales_d is not None else None)
+    torch.cuda.synchronize()
+    ref = oracle.spmm_vnm_f64(_om(m), V, M, B)
+    return C.cpu().numpy(), ref, info
+
+
+@pytest.mark.parametrize("n", [64, 128, 256])
+def test_spmm_fp32_exactish(ctx, n):
+    rng = np.random.default_rng(n)
+    for M in (4, 8):
+        m = _rand(rng, 40 * M, 0.05, M)
+        C, ref, info = _run(ctx, m, M, 16, n, vnm.DT_F32, vnm.DT_F32)
+        assert info.retained_nnz == oracle.retained_nnz(_om(m), 16, 2, M)
+        fro, mx = _errs(C, ref)
+        assert fro < 1e-5 and mx < 1e-5, (fro, mx)
+
+
+def test_spmm_rmat_bf16_and_fp8(ctx):
+    m = vnm.gen_rmat(13, 60000, 3)  # includes hub rows longer than one work item
+    C, ref, info = _run(ctx, m, 8, 128, 256, vnm.DT_F32, vnm.DT_F32)
+    assert max(_errs(C, ref)) < 1e-5
+    C, ref, _ = _run(ctx, m, 8, 128, 256, vnm.DT_BF16, vnm.DT_BF16)
+    fro, mx = _errs(C, ref)
+    assert fro < 1e-2 and mx < 1e-2, (fro, mx)
+    C, ref, _ = _run(ctx, m, 8, 128, 256, vnm.DT_FP8, vnm.DT_FP8)
+    fro, mx = _errs(C, ref)
+    assert fro < 1e-1 and mx < 1e-1, (fro, mx)
+
+
+def test_spmm_m4_bf16(ctx):
+    m = vnm.gen_rmat(12, 40000, 4, M=4)
+    C, ref, _ = _run(ctx, m, 4, 64, 128, vnm.DT_BF16, vnm.DT_BF16)
+    fro, mx = _errs(C, ref)
+    assert fro < 1e-2 and mx < 1e-2, (fro, mx)
+
+
+def test_spmm_empty_rows_written(ctx):
+    # rows without kept entries must come out exactly 0 (C starts as NaN)
+    m = vnm.coo_from_tuples(64, 64, np.array([0, 5]), np.array([3, 9]),
+                            np.array([1.5, -2.0], np.float32), 8)
+    C, ref, _ = _run(ctx, m, 8, 8, 64, vnm.DT_F32, vnm.DT_F32)
+    assert np.isfinite(C).all()
+    assert np.allclose(C, ref, rtol=1e-6, atol=1e-6)

@@ -43,6 +43,7 @@ template struct DBuf<int64_t>;
 template struct DBuf<uint64_t>;
 template struct DBuf<uint8_t>;
 template struct DBuf<uint16_t>;
+template struct DBuf<unsigned long long>;
 
 __global__ void k_iota(int32_t *p, int32_t n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -1,0 +1,740 @@
+// V:N:M (N = 2) packer and SpMM for gfx950.
+//
+// Replaces the reference's SpMM paths (kernel/spmm.cu: the 16x16 WMMA BSR
+// kernel :44-161 and the cuSPARSELt 2:4 prune/compress/matmul :324-500,
+// which densify A first, kernel/eval.h:12-18).
+//
+// Format (VENOM with row compression).  A is cut into V x M blocks (V rows,
+// one segment of M columns).  Each stored block keeps 4 of its M columns --
+// the subset maximising sum over rows of min(2, |row & S|), ties to the
+// lexicographically smallest subset -- and each row keeps at most 2 values
+// inside them (2:4): the two largest |v|, ties to the lower column.  Blocks
+// store their segment, the 4 column offsets (4 bits each) and, for FP8, one
+// FP32 scale (absmax / 448).  Rows of a block that keep anything store one
+// "row entry": two values and two 2-bit selector indices.  Row entries are
+// laid out row-major (CSR over rows), so the SpMM accumulates each output row
+// in registers and writes it once.
+//
+//   bytes(A) = 4 (R + 1) + E (4 + 1 + 2 s_v) + nb (4 + 2 + 4) + 8 * items
+//
+// SpMM: C[R x n] = A_vnm B, FP32 accumulation.  Work items are (row, up to
+// 64 entries); a wave takes one item, each lane owns n / 64 consecutive
+// columns and gathers the selected B rows with vector loads.  Items of the
+// same row add into C with FP32 atomics (rows longer than 64 entries only).
+// Item indices are remapped so that consecutive items run on the same XCD
+// (workgroups are dealt round-robin over the 8 XCDs), keeping the B rows of
+// neighbouring (reordered, hence similar) rows in one L2.
+#include "graph.h"
+#include "hip_common.h"
+#include "sortlib.h"
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+struct vnm_packed {
+    vnm_ctx *ctx = nullptr;
+    int32_t V = 0, M = 0, nrows = 0, ncols = 0, dtype = 0;
+    int32_t nblocks = 0, nent = 0, nitems = 0;
+    int64_t retained = 0;
+    vnm::DBuf<int32_t> rowptr, ent_blk, blk_seg, items;   // items: (row, begin) pairs
+    vnm::DBuf<uint8_t> ent_idx, ent_val;                   // values: 2 per entry, dtype-sized
+    vnm::DBuf<uint16_t> blk_sel;
+    vnm::DBuf<float> blk_scale;
+    vnm::DBuf<uint8_t> row_multi;                          // row spans > 1 item
+};
+
+namespace vnm {
+
+constexpr int ITEM = 64;  // entries per work item
+
+static int grid_of(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, 1 << 20));
+}
+
+static int bits_for(int64_t x) {
+    int b = 1;
+    while ((1ll << b) <= x) ++b;
+    return b;
+}
+
+// ------------------------------------------------------------ value codecs
+
+__host__ __device__ inline uint16_t f32_to_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// OCP FP8 E4M3 (bias 7, no infinities, 0x7F = NaN, max 448): round to
+// nearest even, saturating to +-448.
+__host__ __device__ inline uint8_t f32_to_e4m3(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    const uint8_t sign = (uint8_t)((u >> 24) & 0x80u);
+    const uint32_t a = u & 0x7FFFFFFFu;
+    if (a > 0x7F800000u) return (uint8_t)(sign | 0x7F);  // NaN
+    float af;
+    memcpy(&af, &a, 4);
+    if (af >= 448.0f) return (uint8_t)(sign | 0x7E);      // saturate
+    const int e = (int)(a >> 23) - 127;                    // unbiased exponent
+    if (e < -6) {
+        // subnormal range: value = m * 2^-9, m in [0, 7]
+        float q = af * 512.0f;                             // exact scaling
+        // round half to even
+        float r = q - floorf(q);
+        uint32_t m = (uint32_t)floorf(q);
+        if (r > 0.5f || (r == 0.5f && (m & 1u))) m += 1;
+        return (uint8_t)(sign | m);                        // m == 8 -> smallest normal 0x08
+    }
+    // normal: keep 3 mantissa bits, round half to even on the 20 dropped bits
+    uint32_t mant = a & 0x7FFFFFu;
+    uint32_t keep = mant >> 20, rest = mant & 0xFFFFFu;
+    uint32_t exp8 = (uint32_t)(e + 7);
+    if (rest > 0x80000u || (rest == 0x80000u && (keep & 1u))) {
+        keep += 1;
+        if (keep == 8) { keep = 0; exp8 += 1; }
+    }
+    uint32_t code = (exp8 << 3) | keep;
+    if (code > 0x7E) code = 0x7E;
+    return (uint8_t)(sign | code);
+}
+
+__host__ __device__ inline float e4m3_to_f32(uint8_t c) {
+    const uint32_t s = c & 0x80u, e = (c >> 3) & 0xFu, m = c & 7u;
+    float v;
+    if (e == 0) v = (float)m * (1.0f / 512.0f);
+    else if (e == 15 && m == 7) v = __builtin_nanf("");
+    else {
+        uint32_t u = ((e + 120u) << 23) | (m << 20);
+        memcpy(&v, &u, 4);
+    }
+    return s ? -v : v;
+}
+
+// ------------------------------------------------------------ packer kernels
+
+__global__ void k_nz_keys(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                          const float *__restrict__ val, int64_t nnz, int32_t ncolseg,
+                          uint64_t *__restrict__ keys, int32_t *__restrict__ idx) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool ok = val[i] != 0.0f && col[i] < ncolseg;
+        keys[i] = ok ? (((uint64_t)(uint32_t)row[i] << 32) | (uint32_t)col[i]) : ~0ull;
+        idx[i] = (int32_t)i;
+    }
+}
+
+__global__ void k_pk_heads(const uint64_t *__restrict__ k, int64_t n, int M, int32_t *__restrict__ head) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int h = 1;
+        if (i > 0) {
+            uint64_t a = k[i], b = k[i - 1];
+            h = ((a >> 32) != (b >> 32)) || ((uint32_t)a / M != (uint32_t)b / M);
+        }
+        head[i] = h;
+    }
+}
+
+__global__ void k_pk_runs(const uint64_t *__restrict__ k, const int32_t *__restrict__ head,
+                          const int32_t *__restrict__ excl, int64_t n, int M, int V, int nblk,
+                          uint32_t *__restrict__ bits, int32_t *__restrict__ rstart,
+                          int32_t *__restrict__ rlen, uint64_t *__restrict__ bkey,
+                          int32_t *__restrict__ rid) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = k[i];
+        const uint32_t c = (uint32_t)key, r = (uint32_t)(key >> 32);
+        const int u = excl[i] + head[i] - 1;
+        atomicOr(&bits[u], 1u << (M - 1 - (c % M)));
+        atomicAdd(&rlen[u], 1);
+        if (head[i]) {
+            rstart[u] = (int32_t)i;
+            // block key: (segment, row / V), then the row (stable order)
+            bkey[u] = ((uint64_t)((uint64_t)(c / M) * nblk + r / V) << 32) | r;
+            rid[u] = u;
+        }
+    }
+}
+
+__global__ void k_blk_heads(const uint64_t *__restrict__ bk, int U, int32_t *__restrict__ head) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < U) head[i] = (i == 0) || ((bk[i] >> 32) != (bk[i - 1] >> 32));
+}
+
+// One wave per block: best 4-column subset (lexicographic order of ascending
+// column tuples; ties to the smallest).  nsub = C(M, 4) <= 70.
+__global__ void k_blk_select(const uint64_t *__restrict__ bk, const int32_t *__restrict__ rid_sorted,
+                             const int32_t *__restrict__ bhead_excl, const int32_t *__restrict__ bhead,
+                             int U, int nb, int M, int nblk, const uint32_t *__restrict__ bits,
+                             const uint32_t *__restrict__ subs, int nsub,
+                             int32_t *__restrict__ blk_first, int32_t *__restrict__ blk_seg,
+                             uint16_t *__restrict__ blk_sel, uint32_t *__restrict__ blk_mask,
+                             int32_t *__restrict__ run_blk, unsigned long long *__restrict__ retained) {
+    const int lane = threadIdx.x & 63;
+    const int b = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (b >= nb) return;
+    const int first = blk_first[b];
+    int last = U;
+    if (b + 1 < nb) last = blk_first[b + 1];
+    // score of subsets lane and lane + 64
+    int best_score = -1, best_idx = 0x7FFFFFFF;
+    for (int s = lane; s < nsub; s += 64) {
+        const uint32_t S = subs[s];
+        int sc = 0;
+        for (int q = first; q < last; ++q) sc += min(2, popc32(bits[rid_sorted[q]] & S));
+        if (sc > best_score) { best_score = sc; best_idx = s; }
+    }
+    // wave argmax (score desc, index asc)
+    for (int d = 32; d >= 1; d >>= 1) {
+        int os = __shfl_xor(best_score, d, 64), oi = __shfl_xor(best_idx, d, 64);
+        if (os > best_score || (os == best_score && oi < best_idx)) { best_score = os; best_idx = oi; }
+    }
+    const uint32_t S = subs[best_idx];
+    if (lane == 0) {
+        const uint64_t key = bk[first] >> 32;
+        blk_seg[b] = (int32_t)(key / (uint64_t)nblk);
+        // 4 column offsets (ascending), 4 bits each
+        uint16_t sel = 0;
+        int k = 0;
+        for (int c = 0; c < M && k < 4; ++c)
+            if (S & (1u << (M - 1 - c))) sel |= (uint16_t)(c << (4 * k++));
+        blk_sel[b] = sel;
+        blk_mask[b] = S;
+        atomicAdd(retained, (unsigned long long)best_score);
+    }
+    for (int q = first + lane; q < last; q += 64) run_blk[rid_sorted[q]] = b;
+}
+
+__global__ void k_blk_first(const int32_t *__restrict__ head, const int32_t *__restrict__ excl, int U,
+                            int32_t *__restrict__ blk_first) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < U && head[i]) blk_first[excl[i]] = i;
+}
+
+// Per run: the kept values (<= 2) among the block's selected columns.
+__global__ void k_run_keep(const uint64_t *__restrict__ keys, const int32_t *__restrict__ eidx,
+                           const float *__restrict__ val, int U, int M,
+                           const int32_t *__restrict__ rstart, const int32_t *__restrict__ rlen,
+                           const int32_t *__restrict__ run_blk, const uint32_t *__restrict__ blk_mask,
+                           float *__restrict__ kv, uint8_t *__restrict__ kidx,
+                           int32_t *__restrict__ nkeep, uint32_t *__restrict__ blk_absmax) {
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= U) return;
+    const int b = run_blk[u];
+    const uint32_t S = blk_mask[b];
+    // selector index of column offset c within S (ascending order)
+    float v0 = 0.f, v1 = 0.f;
+    int c0 = -1, c1 = -1;
+    const int beg = rstart[u], len = rlen[u];
+    for (int q = beg; q < beg + len; ++q) {
+        const int c = (int)((uint32_t)keys[q] % (uint32_t)M);
+        if (!(S & (1u << (M - 1 - c)))) continue;
+        const float v = val[eidx[q]];
+        const float a = fabsf(v);
+        // entries come in ascending column order: strict > keeps the lower
+        // column on ties
+        if (c0 < 0 || a > fabsf(v0)) { v1 = v0; c1 = c0; v0 = v; c0 = c; }
+        else if (c1 < 0 || a > fabsf(v1)) { v1 = v; c1 = c; }
+    }
+    int n = (c0 >= 0) + (c1 >= 0);
+    uint8_t code = 0;
+    if (n > 0) {
+        int i0 = popc32(S >> (M - c0)), i1 = c1 >= 0 ? popc32(S >> (M - c1)) : 0;
+        code = (uint8_t)(i0 | (i1 << 2) | (c1 >= 0 ? 0x10 : 0));
+        float m = fmaxf(fabsf(v0), c1 >= 0 ? fabsf(v1) : 0.f);
+        atomicMax(&blk_absmax[b], __float_as_uint(m));
+    }
+    kv[2 * u] = v0;
+    kv[2 * u + 1] = c1 >= 0 ? v1 : 0.f;
+    kidx[u] = code;
+    nkeep[u] = n;
+}
+
+__global__ void k_row_counts(const uint64_t *__restrict__ keys, const int32_t *__restrict__ rstart,
+                             const int32_t *__restrict__ nkeep, int U, int32_t *__restrict__ rcnt,
+                             uint8_t *__restrict__ flag) {
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= U) return;
+    const bool keep = nkeep[u] > 0;
+    flag[u] = keep;
+    if (keep) atomicAdd(&rcnt[(int)(keys[rstart[u]] >> 32)], 1);
+}
+
+__global__ void k_iota_i(int32_t *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+
+__global__ void k_scales(const uint32_t *__restrict__ absmax, int nb, int fp8, float *__restrict__ sc) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    float m = __uint_as_float(absmax[b]);
+    sc[b] = (fp8 && m > 0.f) ? m / 448.0f : 1.0f;
+}
+
+__global__ void k_emit(const int32_t *__restrict__ sel_runs, int E, const int32_t *__restrict__ run_blk,
+                       const float *__restrict__ kv, const uint8_t *__restrict__ kidx,
+                       const float *__restrict__ sc, int dtype, int32_t *__restrict__ ent_blk,
+                       uint8_t *__restrict__ ent_idx, uint8_t *__restrict__ ent_val) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int u = sel_runs[e];
+    const int b = run_blk[u];
+    ent_blk[e] = b;
+    ent_idx[e] = kidx[u];
+    const float v0 = kv[2 * u], v1 = kv[2 * u + 1];
+    if (dtype == VNM_DT_F32) {
+        float *p = (float *)ent_val;
+        p[2 * e] = v0;
+        p[2 * e + 1] = v1;
+    } else if (dtype == VNM_DT_BF16) {
+        uint16_t *p = (uint16_t *)ent_val;
+        p[2 * e] = f32_to_bf16(v0);
+        p[2 * e + 1] = f32_to_bf16(v1);
+    } else if (dtype == VNM_DT_F16) {
+        __half *p = (__half *)ent_val;
+        p[2 * e] = __float2half(v0);
+        p[2 * e + 1] = __float2half(v1);
+    } else {
+        const float s = sc[b];
+        ent_val[2 * e] = f32_to_e4m3(v0 / s);
+        ent_val[2 * e + 1] = f32_to_e4m3(v1 / s);
+    }
+}
+
+__global__ void k_items_count(const int32_t *__restrict__ rowptr, int R, int32_t *__restrict__ nit,
+                              uint8_t *__restrict__ multi) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const int len = rowptr[r + 1] - rowptr[r];
+    const int n = (len + ITEM - 1) / ITEM;
+    nit[r] = n;
+    multi[r] = n > 1;
+}
+
+__global__ void k_items_fill(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ nit_excl,
+                             const int32_t *__restrict__ nit, int R, int32_t *__restrict__ items) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const int base = nit_excl[r];
+    for (int k = 0; k < nit[r]; ++k) {
+        items[2 * (base + k)] = r;
+        items[2 * (base + k) + 1] = rowptr[r] + k * ITEM;
+    }
+}
+
+// ------------------------------------------------------------ SpMM
+
+template <int DT>
+__device__ __forceinline__ float load_aval(const uint8_t *p, int i) {
+    if constexpr (DT == VNM_DT_F32) return ((const float *)p)[i];
+    else if constexpr (DT == VNM_DT_BF16) return bf16_to_f32(((const uint16_t *)p)[i]);
+    else if constexpr (DT == VNM_DT_F16) return __half2float(((const __half *)p)[i]);
+    else return e4m3_to_f32(p[i]);
+}
+
+// CPL consecutive B values of row c starting at column col0, as floats.
+template <int BT, int CPL>
+__device__ __forceinline__ void load_brow(const void *B, int ldb, int c, int col0, float (&x)[CPL]) {
+    if constexpr (BT == VNM_DT_F32) {
+        const float *p = (const float *)B + (size_t)c * ldb + col0;
+        if constexpr (CPL == 4) {
+            float4 t = *(const float4 *)p;
+            x[0] = t.x; x[1] = t.y; x[2] = t.z; x[3] = t.w;
+        } else if constexpr (CPL == 2) {
+            float2 t = *(const float2 *)p;
+            x[0] = t.x; x[1] = t.y;
+        } else {
+#pragma unroll
+            for (int i = 0; i < CPL; ++i) x[i] = p[i];
+        }
+    } else if constexpr (BT == VNM_DT_BF16) {
+        const uint16_t *p = (const uint16_t *)B + (size_t)c * ldb + col0;
+        if constexpr (CPL == 4) {
+            uint2 t = *(const uint2 *)p;
+            x[0] = __uint_as_float(t.x << 16); x[1] = __uint_as_float(t.x & 0xFFFF0000u);
+            x[2] = __uint_as_float(t.y << 16); x[3] = __uint_as_float(t.y & 0xFFFF0000u);
+        } else if constexpr (CPL == 2) {
+            uint32_t t = *(const uint32_t *)p;
+            x[0] = __uint_as_float(t << 16); x[1] = __uint_as_float(t & 0xFFFF0000u);
+        } else {
+#pragma unroll
+            for (int i = 0; i < CPL; ++i) x[i] = bf16_to_f32(p[i]);
+        }
+    } else {
+        const uint8_t *p = (const uint8_t *)B + (size_t)c * ldb + col0;
+        if constexpr (CPL == 4) {
+            uint32_t t = *(const uint32_t *)p;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) x[i] = e4m3_to_f32((uint8_t)(t >> (8 * i)));
+        } else {
+#pragma unroll
+            for (int i = 0; i < CPL; ++i) x[i] = e4m3_to_f32(p[i]);
+        }
+    }
+}
+
+template <int AT, int BT, int CPL>
+__global__ __launch_bounds__(256) void k_spmm(
+    const int32_t *__restrict__ items, int nitems, const int32_t *__restrict__ rowptr,
+    const uint8_t *__restrict__ row_multi, const int32_t *__restrict__ ent_blk,
+    const uint8_t *__restrict__ ent_idx, const uint8_t *__restrict__ ent_val,
+    const int32_t *__restrict__ blk_seg, const uint16_t *__restrict__ blk_sel,
+    const float *__restrict__ blk_scale, int M, const void *__restrict__ B, int ldb,
+    const float *__restrict__ b_scales, float *__restrict__ C, int ldc, int n_tiles, int nwg) {
+    const int lane = threadIdx.x & 63;
+    const int wpb = blockDim.x / 64;
+    // XCD-aware remap: workgroup g runs on XCD g % 8; give each XCD a
+    // contiguous range of items (the grid is a multiple of 8, so the map
+    // from blockIdx to lwg is a bijection on [0, gridDim.x))
+    const int per = gridDim.x / 8;
+    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const int lwg = xcd * per + slot;
+    if (lwg >= nwg) return;
+    const int tile = lwg % n_tiles;
+    const int wid = (lwg / n_tiles) * wpb + (threadIdx.x >> 6);
+    if (wid >= nitems) return;
+    const int r = items[2 * wid];
+    const int beg = items[2 * wid + 1];
+    const int end = min(beg + ITEM, rowptr[r + 1]);
+    const int col0 = tile * 64 * CPL + lane * CPL;
+    float acc[CPL];
+#pragma unroll
+    for (int i = 0; i < CPL; ++i) acc[i] = 0.f;
+    for (int e = beg; e < end; ++e) {
+        const int b = __builtin_amdgcn_readfirstlane(ent_blk[e]);
+        const uint32_t code = __builtin_amdgcn_readfirstlane((uint32_t)ent_idx[e]);
+        const int seg = __builtin_amdgcn_readfirstlane(blk_seg[b]);
+        const uint32_t sel = __builtin_amdgcn_readfirstlane((uint32_t)blk_sel[b]);
+        const float sc = blk_scale[b];
+        const int c0 = seg * M + ((sel >> (4 * (code & 3u))) & 15u);
+        float x[CPL];
+        load_brow<BT, CPL>(B, ldb, c0, col0, x);
+        float a0 = load_aval<AT>(ent_val, 2 * e) * sc;
+        if constexpr (BT == VNM_DT_FP8) a0 *= b_scales[c0 >> 7];
+#pragma unroll
+        for (int i = 0; i < CPL; ++i) acc[i] = fmaf(a0, x[i], acc[i]);
+        if (code & 0x10u) {
+            const int c1 = seg * M + ((sel >> (4 * ((code >> 2) & 3u))) & 15u);
+            float y[CPL];
+            load_brow<BT, CPL>(B, ldb, c1, col0, y);
+            float a1 = load_aval<AT>(ent_val, 2 * e + 1) * sc;
+            if constexpr (BT == VNM_DT_FP8) a1 *= b_scales[c1 >> 7];
+#pragma unroll
+            for (int i = 0; i < CPL; ++i) acc[i] = fmaf(a1, y[i], acc[i]);
+        }
+    }
+    float *cp = C + (size_t)r * ldc + col0;
+    if (row_multi[r]) {
+#pragma unroll
+        for (int i = 0; i < CPL; ++i) atomicAdd(&cp[i], acc[i]);
+    } else {
+        if constexpr (CPL == 4) *(float4 *)cp = make_float4(acc[0], acc[1], acc[2], acc[3]);
+        else if constexpr (CPL == 2) *(float2 *)cp = make_float2(acc[0], acc[1]);
+        else {
+#pragma unroll
+            for (int i = 0; i < CPL; ++i) cp[i] = acc[i];
+        }
+    }
+}
+
+// Rows that keep nothing are zero; rows spread over several items start at 0.
+__global__ void k_zero_rows(const int32_t *__restrict__ rowptr, const uint8_t *__restrict__ multi,
+                            int R, int n, float *__restrict__ C, int ldc) {
+    const int r = blockIdx.x;
+    if (r >= R) return;
+    if (rowptr[r + 1] > rowptr[r] && !multi[r]) return;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) C[(size_t)r * ldc + c] = 0.f;
+}
+
+// ------------------------------------------------------------ host
+
+static std::vector<uint32_t> subsets4(int M) {
+    std::vector<uint32_t> out;
+    if (M <= 4) {
+        out.push_back((1u << M) - 1);
+        return out;
+    }
+    for (int a = 0; a < M; ++a)
+        for (int b = a + 1; b < M; ++b)
+            for (int c = b + 1; c < M; ++c)
+                for (int d = c + 1; d < M; ++d)
+                    out.push_back((1u << (M - 1 - a)) | (1u << (M - 1 - b)) | (1u << (M - 1 - c)) |
+                                  (1u << (M - 1 - d)));
+    return out;
+}
+
+int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_only) {
+    const int M = g->M;
+    if (V <= 0 || V > 256) return set_error(VNM_E_ARG, "V must be in [1, 256] (got %d)", V);
+    if (M != 4 && M != 8 && M != 16) return set_error(VNM_E_ARG, "M must be 4, 8 or 16");
+    if (dtype < VNM_DT_F32 || dtype > VNM_DT_FP8) return set_error(VNM_E_ARG, "bad dtype %d", dtype);
+    hipStream_t st = g->ctx->stream;
+    const int R = g->nrows;
+    const int nseg = (g->ncols + M - 1) / M;
+    const int nblk = (R + V - 1) / V;
+    const int64_t nnz = g->nnz;
+    const size_t e1 = std::max<int64_t>(nnz, 1);
+    int rc;
+    DBuf<uint64_t> keys, keys2, bkey, bkey2;
+    DBuf<int32_t> idx, idx2, head, excl, rstart, rlen, rid, rid2, bhead, bexcl, blk_first, run_blk,
+        nkeep, rcnt, sel_runs, cnt;
+    DBuf<uint32_t> bits, blk_mask, absmax, dsubs;
+    DBuf<float> kv;
+    DBuf<uint8_t> kidx, flag;
+    DBuf<unsigned long long> dret;
+    if ((rc = keys.alloc(e1)) || (rc = keys2.alloc(e1)) || (rc = idx.alloc(e1)) ||
+        (rc = idx2.alloc(e1)) || (rc = head.alloc(e1)) || (rc = excl.alloc(e1)) ||
+        (rc = dret.alloc(1)) || (rc = cnt.alloc(4)))
+        return rc;
+    const int gb = grid_of(nnz, 256);
+    hipLaunchKernelGGL(k_nz_keys, dim3(gb), dim3(256), 0, st, g->row.p, g->col.p, g->val.p, nnz,
+                       nseg * M, keys.p, idx.p);
+    if ((rc = sort_pairs_u64_i32(keys.p, keys2.p, idx.p, idx2.p, nnz, 64, st))) return rc;
+    // count of nonzero entries = position of the first ~0 key
+    std::vector<uint64_t> hk(1);
+    int64_t nz = nnz;
+    {
+        // binary search on the device array through small copies
+        int64_t lo = 0, hi = nnz;
+        while (lo < hi) {
+            int64_t mid = (lo + hi) / 2;
+            VNM_HIP_TRY(hipMemcpyAsync(hk.data(), keys2.p + mid, 8, hipMemcpyDeviceToHost, st));
+            VNM_HIP_TRY(hipStreamSynchronize(st));
+            if (hk[0] == ~0ull) hi = mid; else lo = mid + 1;
+        }
+        nz = lo;
+    }
+    int U = 0;
+    if (nz > 0) {
+        hipLaunchKernelGGL(k_pk_heads, dim3(grid_of(nz, 256)), dim3(256), 0, st, keys2.p, nz, M, head.p);
+        if ((rc = scan_excl_i32(head.p, excl.p, nz, st))) return rc;
+        int32_t last[2];
+        VNM_HIP_TRY(hipMemcpyAsync(&last[0], excl.p + nz - 1, 4, hipMemcpyDeviceToHost, st));
+        VNM_HIP_TRY(hipMemcpyAsync(&last[1], head.p + nz - 1, 4, hipMemcpyDeviceToHost, st));
+        VNM_HIP_TRY(hipStreamSynchronize(st));
+        U = last[0] + last[1];
+    }
+    const size_t u1 = std::max(U, 1);
+    if ((rc = bits.alloc(u1)) || (rc = rstart.alloc(u1)) || (rc = rlen.alloc(u1)) ||
+        (rc = bkey.alloc(u1)) || (rc = bkey2.alloc(u1)) || (rc = rid.alloc(u1)) ||
+        (rc = rid2.alloc(u1)) || (rc = bhead.alloc(u1)) || (rc = bexcl.alloc(u1)) ||
+        (rc = run_blk.alloc(u1)) || (rc = nkeep.alloc(u1)) || (rc = kv.alloc(2 * u1)) ||
+        (rc = kidx.alloc(u1)) || (rc = flag.alloc(u1)) || (rc = sel_runs.alloc(u1)) ||
+        (rc = rcnt.alloc(R + 1)))
+        return rc;
+    int nb = 0;
+    std::vector<uint32_t> subs = subsets4(M);
+    if ((rc = dsubs.alloc(subs.size()))) return rc;
+    VNM_HIP_TRY(hipMemcpyAsync(dsubs.p, subs.data(), subs.size() * 4, hipMemcpyHostToDevice, st));
+    VNM_HIP_TRY(hipMemsetAsync(dret.p, 0, 8, st));
+    if (U > 0) {
+        VNM_HIP_TRY(hipMemsetAsync(bits.p, 0, (size_t)U * 4, st));
+        VNM_HIP_TRY(hipMemsetAsync(rlen.p, 0, (size_t)U * 4, st));
+        hipLaunchKernelGGL(k_pk_runs, dim3(grid_of(nz, 256)), dim3(256), 0, st, keys2.p, head.p,
+                           excl.p, nz, M, V, nblk, bits.p, rstart.p, rlen.p, bkey.p, rid.p);
+        const int kb = 32 + bits_for((int64_t)nseg * nblk);
+        if ((rc = sort_pairs_u64_i32(bkey.p, bkey2.p, rid.p, rid2.p, U, std::min(64, kb), st))) return rc;
+        hipLaunchKernelGGL(k_blk_heads, dim3(grid_of(U, 256)), dim3(256), 0, st, bkey2.p, U, bhead.p);
+        if ((rc = scan_excl_i32(bhead.p, bexcl.p, U, st))) return rc;
+        int32_t last[2];
+        VNM_HIP_TRY(hipMemcpyAsync(&last[0], bexcl.p + U - 1, 4, hipMemcpyDeviceToHost, st));
+        VNM_HIP_TRY(hipMemcpyAsync(&last[1], bhead.p + U - 1, 4, hipMemcpyDeviceToHost, st));
+        VNM_HIP_TRY(hipStreamSynchronize(st));
+        nb = last[0] + last[1];
+    }
+    vnm_packed *p = new vnm_packed();
+    p->ctx = g->ctx;
+    p->V = V; p->M = M; p->nrows = R; p->ncols = g->ncols; p->dtype = dtype; p->nblocks = nb;
+    const size_t b1 = std::max(nb, 1);
+    if ((rc = blk_first.alloc(b1)) || (rc = blk_mask.alloc(b1)) || (rc = absmax.alloc(b1)) ||
+        (rc = p->blk_seg.alloc(b1)) || (rc = p->blk_sel.alloc(b1)) || (rc = p->blk_scale.alloc(b1))) {
+        delete p;
+        return rc;
+    }
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_blk_first, dim3(grid_of(U, 256)), dim3(256), 0, st, bhead.p, bexcl.p, U,
+                           blk_first.p);
+        hipLaunchKernelGGL(k_blk_select, dim3((nb + 3) / 4), dim3(256), 0, st, bkey2.p, rid2.p,
+                           bexcl.p, bhead.p, U, nb, M, nblk, bits.p, dsubs.p, (int)subs.size(),
+                           blk_first.p, p->blk_seg.p, p->blk_sel.p, blk_mask.p, run_blk.p, dret.p);
+        VNM_HIP_TRY(hipMemsetAsync(absmax.p, 0, (size_t)nb * 4, st));
+        hipLaunchKernelGGL(k_run_keep, dim3(grid_of(U, 256)), dim3(256), 0, st, keys2.p, idx2.p,
+                           g->val.p, U, M, rstart.p, rlen.p, run_blk.p, blk_mask.p, kv.p, kidx.p,
+                           nkeep.p, absmax.p);
+        hipLaunchKernelGGL(k_scales, dim3(grid_of(nb, 256)), dim3(256), 0, st, absmax.p, nb,
+                           dtype == VNM_DT_FP8, p->blk_scale.p);
+    }
+    unsigned long long ret = 0;
+    VNM_HIP_TRY(hipMemcpyAsync(&ret, dret.p, 8, hipMemcpyDeviceToHost, st));
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    p->retained = (int64_t)ret;
+    VNM_LAUNCH_CHECK("pack select");
+    if (retained_only) {
+        *retained_only = p->retained;
+        delete p;
+        return VNM_OK;
+    }
+    // row entries (runs that keep something), row-major
+    VNM_HIP_TRY(hipMemsetAsync(rcnt.p, 0, (size_t)(R + 1) * 4, st));
+    int E = 0;
+    if (U > 0) {
+        hipLaunchKernelGGL(k_row_counts, dim3(grid_of(U, 256)), dim3(256), 0, st, keys2.p, rstart.p,
+                           nkeep.p, U, rcnt.p, flag.p);
+        hipLaunchKernelGGL(k_iota_i, dim3(grid_of(U, 256)), dim3(256), 0, st, rid.p, U);
+        if ((rc = select_flagged_i32(rid.p, flag.p, sel_runs.p, cnt.p, U, st))) { delete p; return rc; }
+        VNM_HIP_TRY(hipMemcpyAsync(&E, cnt.p, 4, hipMemcpyDeviceToHost, st));
+        VNM_HIP_TRY(hipStreamSynchronize(st));
+    }
+    p->nent = E;
+    const int vs = dtype == VNM_DT_F32 ? 4 : dtype == VNM_DT_FP8 ? 1 : 2;
+    if ((rc = p->rowptr.alloc(R + 1)) || (rc = p->ent_blk.alloc(std::max(E, 1))) ||
+        (rc = p->ent_idx.alloc(std::max(E, 1))) || (rc = p->ent_val.alloc((size_t)std::max(E, 1) * 2 * vs)) ||
+        (rc = p->row_multi.alloc(R))) {
+        delete p;
+        return rc;
+    }
+    if ((rc = scan_excl_i32(rcnt.p, p->rowptr.p, R + 1, st))) { delete p; return rc; }
+    if (E > 0)
+        hipLaunchKernelGGL(k_emit, dim3(grid_of(E, 256)), dim3(256), 0, st, sel_runs.p, E, run_blk.p,
+                           kv.p, kidx.p, p->blk_scale.p, dtype, p->ent_blk.p, p->ent_idx.p, p->ent_val.p);
+    // work items
+    DBuf<int32_t> nit, nit_excl;
+    if ((rc = nit.alloc(R + 1)) || (rc = nit_excl.alloc(R + 1))) { delete p; return rc; }
+    VNM_HIP_TRY(hipMemsetAsync(nit.p, 0, (size_t)(R + 1) * 4, st));
+    hipLaunchKernelGGL(k_items_count, dim3(grid_of(R, 256)), dim3(256), 0, st, p->rowptr.p, R, nit.p,
+                       p->row_multi.p);
+    if ((rc = scan_excl_i32(nit.p, nit_excl.p, R + 1, st))) { delete p; return rc; }
+    int32_t nitems = 0;
+    VNM_HIP_TRY(hipMemcpyAsync(&nitems, nit_excl.p + R, 4, hipMemcpyDeviceToHost, st));
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    p->nitems = nitems;
+    if ((rc = p->items.alloc(2 * (size_t)std::max(nitems, 1)))) { delete p; return rc; }
+    hipLaunchKernelGGL(k_items_fill, dim3(grid_of(R, 256)), dim3(256), 0, st, p->rowptr.p, nit_excl.p,
+                       nit.p, R, p->items.p);
+    g->ctx->launches += 16;
+    VNM_LAUNCH_CHECK("pack emit");
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    *out = p;
+    return VNM_OK;
+}
+
+template <int AT, int BT>
+static void launch_spmm(vnm_packed *p, const void *B, int ldb, int n, const float *bs, float *C,
+                        int ldc, hipStream_t st) {
+    const int wpb = 4;
+    if (n % 256 == 0) {
+        const int tiles = n / 256;
+        const int nwg = (p->nitems + wpb - 1) / wpb * tiles;
+        hipLaunchKernelGGL((k_spmm<AT, BT, 4>), dim3((nwg + 7) / 8 * 8), dim3(64 * wpb), 0, st,
+                           p->items.p, p->nitems, p->rowptr.p, p->row_multi.p, p->ent_blk.p,
+                           p->ent_idx.p, p->ent_val.p, p->blk_seg.p, p->blk_sel.p, p->blk_scale.p,
+                           p->M, B, ldb, bs, C, ldc, tiles, nwg);
+    } else if (n % 128 == 0) {
+        const int tiles = n / 128;
+        const int nwg = (p->nitems + wpb - 1) / wpb * tiles;
+        hipLaunchKernelGGL((k_spmm<AT, BT, 2>), dim3((nwg + 7) / 8 * 8), dim3(64 * wpb), 0, st,
+                           p->items.p, p->nitems, p->rowptr.p, p->row_multi.p, p->ent_blk.p,
+                           p->ent_idx.p, p->ent_val.p, p->blk_seg.p, p->blk_sel.p, p->blk_scale.p,
+                           p->M, B, ldb, bs, C, ldc, tiles, nwg);
+    } else {
+        const int tiles = n / 64;
+        const int nwg = (p->nitems + wpb - 1) / wpb * tiles;
+        hipLaunchKernelGGL((k_spmm<AT, BT, 1>), dim3((nwg + 7) / 8 * 8), dim3(64 * wpb), 0, st,
+                           p->items.p, p->nitems, p->rowptr.p, p->row_multi.p, p->ent_blk.p,
+                           p->ent_idx.p, p->ent_val.p, p->blk_seg.p, p->blk_sel.p, p->blk_scale.p,
+                           p->M, B, ldb, bs, C, ldc, tiles, nwg);
+    }
+}
+
+template <int AT>
+static void launch_spmm_b(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs,
+                          float *C, int ldc, hipStream_t st) {
+    if (bt == VNM_DT_F32) launch_spmm<AT, VNM_DT_F32>(p, B, ldb, n, bs, C, ldc, st);
+    else if (bt == VNM_DT_BF16) launch_spmm<AT, VNM_DT_BF16>(p, B, ldb, n, bs, C, ldc, st);
+    else launch_spmm<AT, VNM_DT_FP8>(p, B, ldb, n, bs, C, ldc, st);
+}
+
+int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc) {
+    if (!p || !B || !C || n <= 0 || n % 64 != 0 || ldb < n || ldc < n)
+        return set_error(VNM_E_ARG, "spmm: n must be a positive multiple of 64 with ldb, ldc >= n");
+    if (bt != VNM_DT_F32 && bt != VNM_DT_BF16 && bt != VNM_DT_FP8)
+        return set_error(VNM_E_ARG, "spmm: B dtype must be F32, BF16 or FP8");
+    if (bt == VNM_DT_FP8 && !bs) return set_error(VNM_E_ARG, "spmm: FP8 B needs per-128-row scales");
+    if ((bt == VNM_DT_BF16 && (ldb % 4) != 0) || (bt == VNM_DT_FP8 && (ldb % 4) != 0) || (ldc % 4) != 0)
+        return set_error(VNM_E_ARG, "spmm: leading dimensions must be multiples of 4");
+    hipStream_t st = p->ctx->stream;
+    hipLaunchKernelGGL(k_zero_rows, dim3(std::max(p->nrows, 1)), dim3(256), 0, st, p->rowptr.p,
+                       p->row_multi.p, p->nrows, n, C, ldc);
+    if (p->nitems > 0) {
+        switch (p->dtype) {
+            case VNM_DT_F32: launch_spmm_b<VNM_DT_F32>(p, B, ldb, n, bt, bs, C, ldc, st); break;
+            case VNM_DT_BF16: launch_spmm_b<VNM_DT_BF16>(p, B, ldb, n, bt, bs, C, ldc, st); break;
+            case VNM_DT_F16: launch_spmm_b<VNM_DT_F16>(p, B, ldb, n, bt, bs, C, ldc, st); break;
+            default: launch_spmm_b<VNM_DT_FP8>(p, B, ldb, n, bt, bs, C, ldc, st); break;
+        }
+    }
+    p->ctx->launches += 2;
+    VNM_LAUNCH_CHECK("k_spmm");
+    return VNM_OK;
+}
+
+}  // namespace vnm
+
+using namespace vnm;
+
+extern "C" {
+
+int vnm_retained_nnz(vnm_graph *g, int V, int N, int64_t *out) {
+    if (!g || !out) return set_error(VNM_E_ARG, "null argument");
+    if (N != 2) return set_error(VNM_E_ARG, "the V:N:M format keeps N = 2 per row (got %d)", N);
+    return pack(g, V, VNM_DT_F32, nullptr, out);
+}
+
+int vnm_pack(vnm_graph *g, int V, int dtype, vnm_packed **out) {
+    if (!g || !out) return set_error(VNM_E_ARG, "null argument");
+    return pack(g, V, dtype, out, nullptr);
+}
+
+int vnm_packed_info_get(const vnm_packed *p, vnm_packed_info *info) {
+    if (!p || !info) return set_error(VNM_E_ARG, "null argument");
+    info->V = p->V;
+    info->M = p->M;
+    info->nrows = p->nrows;
+    info->ncols = p->ncols;
+    info->nblocks = p->nblocks;
+    info->dtype = p->dtype;
+    info->retained_nnz = p->retained;
+    const int vs = p->dtype == VNM_DT_F32 ? 4 : p->dtype == VNM_DT_FP8 ? 1 : 2;
+    info->bytes = 4ll * (p->nrows + 1) + (int64_t)p->nent * (4 + 1 + 2 * vs) +
+                  (int64_t)p->nblocks * (4 + 2 + 4) + 8ll * p->nitems + p->nrows;
+    return VNM_OK;
+}
+
+void vnm_packed_destroy(vnm_packed *p) {
+    if (!p) return;
+    if (p->ctx && p->ctx->stream) (void)hipStreamSynchronize(p->ctx->stream);
+    delete p;
+}
+
+int vnm_spmm(vnm_packed *p, const void *B, int ldb, int n, int b_dtype, const float *b_scales,
+             float *C, int ldc) {
+    return spmm(p, B, ldb, n, b_dtype, b_scales, C, ldc);
+}
+
+}  // extern "C"
