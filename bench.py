@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""V:N:M reorder + SpMM benchmark on one MI355X.
+
+Workload (default): symmetric R-MAT adjacency, 65,536 nodes, (a, b, c) =
+(0.57, 0.19, 0.19), 524,288 distinct undirected edges mirrored to
+nnz = 1,048,576, values U(0,1) (seed 2); B = 65,536 x 256 FP32 N(0,1)
+(seed 3).  Setup (untimed, reported): the GPU hybrid reorder with V:N:M =
+128:2:8 (schedule 0, --rounds VENOM/N:M rounds), then packing to V:N:M.
+
+A step is one SpMM C = A_vnm * B (FP32 A, FP32 B, FP32 C) with every input
+resident in HBM.  `value` is its throughput in TFLOP/s of useful work
+(2 * retained_nnz * ncols per step).  Also reported: the GPU reorder time and
+counts, BF16 and FP8 SpMM rates and errors against the FP64 oracle, the
+roofline of the SpMM kernel (HBM bound), and the CPU oracle timed on this
+host (cpu_baseline; single thread).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--rounds R] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "vnm-graph-reordering_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--rounds", type=int, default=10, help="hybrid VENOM/N:M rounds (--maxiter)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain 256 MiB device copy (counter calibration)")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(--gpus {a.gpus} requested); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import vnm
+
+    M, N, V, NCOLS = 8, 2, 128, 256
+    t_gen = time.time()
+    coo = vnm.gen_rmat(16, 524288, 2, M=M)
+    B = vnm.gen_normal(3, (coo.ncols, NCOLS))
+    t_gen = time.time() - t_gen
+
+    ctx = vnm.Context(0)
+    g = vnm.Graph(ctx, coo, M)
+    ctx.timing(True)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    hs = g.hybrid(N, V, a.rounds, 0)
+    torch.cuda.synchronize()
+    reorder_wall_ms = (time.time() - t0) * 1e3
+    greedy_ms, greedy_n = ctx.timer("k_nm_greedy")
+    print(f"[bench] reorder done: {reorder_wall_ms:.1f} ms, nm_rounds={hs.nm_rounds} "
+          f"vnm_rounds={hs.vnm_rounds}", file=sys.stderr, flush=True)
+    retained = g.retained_nnz(V)
+    out_coo = g.download()
+
+    dev = torch.device("cuda:0")
+    Bd = torch.from_numpy(B).to(dev)
+    C = torch.empty((coo.nrows, NCOLS), dtype=torch.float32, device=dev)
+
+    def run(packed, Bt, bdt, steps, scales=None):
+        for _ in range(a.warmup):
+            packed.spmm(Bt.data_ptr(), NCOLS, NCOLS, bdt, C.data_ptr(), NCOLS,
+                        scales.data_ptr() if scales is not None else None)
+        torch.cuda.synchronize()
+        ctx.timing(True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            packed.spmm(Bt.data_ptr(), NCOLS, NCOLS, bdt, C.data_ptr(), NCOLS,
+                        scales.data_ptr() if scales is not None else None)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t) / steps
+        kms, kn = ctx.timer("k_spmm")
+        return wall * 1e3, kms / max(kn, 1)
+
+    # FP32 (the headline configuration)
+    p32 = g.pack(V, vnm.DT_F32)
+    info = p32.info()
+    ms_step, k_ms = run(p32, Bd, vnm.DT_F32, a.steps)
+    C32 = C.cpu().numpy()
+    flops = 2.0 * info.retained_nnz * NCOLS
+    tflops = flops / (ms_step * 1e-3) / 1e12
+    a_bytes = info.bytes
+    alg_bytes = a_bytes + coo.ncols * NCOLS * 4 + coo.nrows * NCOLS * 4
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+
+    # BF16 A and B
+    pbf = g.pack(V, vnm.DT_BF16)
+    Bbf = Bd.to(torch.bfloat16)
+    ms_bf, k_bf = run(pbf, Bbf, vnm.DT_BF16, a.steps)
+    Cbf = C.cpu().numpy()
+    bf_bytes = pbf.info().bytes + coo.ncols * NCOLS * 2 + coo.nrows * NCOLS * 4
+
+    # FP8 e4m3 A (per-block scales) and B (one scale per 128 rows)
+    p8 = g.pack(V, vnm.DT_FP8)
+    Bblk = Bd.view(-1, 128, NCOLS)
+    bsc = (Bblk.abs().amax(dim=(1, 2)) / 448.0).clamp_min(1e-30).float().contiguous()
+    B8 = (Bblk / bsc[:, None, None]).to(torch.float8_e4m3fn).view(-1, NCOLS).contiguous()
+    ms_8, k_8 = run(p8, B8, vnm.DT_FP8, a.steps, bsc)
+    C8 = C.cpu().numpy()
+
+    if a.copy_calib:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+
+    res = {
+        "metric": "V:N:M SpMM TFLOP/s (after GPU V:N:M reorder)",
+        "value": round(tflops, 4),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded R-MAT, N(0,1) B)",
+        "config": {"workload": "rmat65536_deg16_V128_N2_M8_B256_f32", "nodes": coo.nrows,
+                   "nnz": coo.nnz, "V": V, "N": N, "M": M, "ncols": NCOLS,
+                   "hybrid_rounds": a.rounds, "sched": 0, "parallelism": "none (1 GPU)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_spmm", "kernel_ms": round(k_ms, 5),
+                     "alg_bytes": int(alg_bytes), "a_packed_bytes": int(a_bytes)},
+        "reorder": {"gpu_ms": round(reorder_wall_ms, 2), "hybrid_ms": round(hs.milliseconds, 2),
+                    "greedy_kernel_ms": round(greedy_ms, 2), "greedy_launches": greedy_n,
+                    "nm_rounds": hs.nm_rounds, "vnm_rounds": hs.vnm_rounds,
+                    "init_cnt": hs.init_cnt, "final_cnt": hs.final_cnt,
+                    "init_vblocks": hs.init_mb, "final_vblocks": hs.final_mb,
+                    "retained_nnz": retained, "nnz_value_nonzero": int((out_coo.val != 0).sum())},
+        "variants": {"bf16_tflops": round(flops / (ms_bf * 1e-3) / 1e12, 4),
+                     "bf16_kernel_gbs": round(bf_bytes / (k_bf * 1e-3) / 1e9, 1),
+                     "fp8_tflops": round(flops / (ms_8 * 1e-3) / 1e12, 4)},
+        "gen_s": round(t_gen, 2),
+    }
+
+    if not a.no_cpu:
+        import oracle
+        om = oracle.Matrix(out_coo.nrows, out_coo.ncols, out_coo.row, out_coo.col, out_coo.val)
+        t = time.perf_counter()
+        ref = oracle.spmm_vnm_f64(om, V, M, B)
+        cpu_s = time.perf_counter() - t
+        cpu_tflops = flops / cpu_s / 1e12
+
+        def errs(X):
+            d = X.astype(np.float64) - ref
+            return (float(np.linalg.norm(d) / np.linalg.norm(ref)),
+                    float(np.abs(d).max() / np.abs(ref).max()))
+        e32, ebf, e8 = errs(C32), errs(Cbf), errs(C8)
+        res["variants"].update({"f32_rel_err_fro": e32[0], "f32_rel_err_max": e32[1],
+                                "bf16_rel_err_fro": ebf[0], "bf16_rel_err_max": ebf[1],
+                                "fp8_rel_err_fro": e8[0], "fp8_rel_err_max": e8[1]})
+        # bounded CPU sample of the reorder: the first 40 rounds of the first
+        # N:M run of the same workload
+        om0 = oracle.Matrix(coo.nrows, coo.ncols, coo.row, coo.col, coo.val)
+        t = time.perf_counter()
+        st = oracle.nm_reorder_sample(om0, N, M, 0, 10, 40)
+        cpu_round_ms = (time.perf_counter() - t) * 1e3 / max(int(st[4]), 1)
+        res["cpu_baseline"] = {
+            "value": round(cpu_tflops, 6), "unit": "TFLOP/s", "cores": 1, "kind": "port",
+            "sample": "oracle FP64 V:N:M SpMM of the full reordered workload "
+                      f"({cpu_s:.2f} s, single thread); reorder: first 40 greedy rounds "
+                      f"({cpu_round_ms:.1f} ms/round on the CPU vs "
+                      f"{greedy_ms / max(hs.nm_rounds + greedy_n, 1):.3f} ms/round GPU)",
+            "cpu_reorder_ms_per_round": round(cpu_round_ms, 3),
+            "host_cpus": os.cpu_count(),
+        }
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -89,6 +89,8 @@ def lib():
         L.orc_quant_py_roundtrip.restype = ctypes.c_float
         L.orc_ub_events.restype = ctypes.c_long
         L.orc_reload.argtypes = [P(Coo), ctypes.c_int]
+        L.orc_nm_reorder_sample.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, i32p]
         _lib = L
     return _lib
 
@@ -258,6 +260,14 @@ def quant_cpp_write(v):
 
 def quant_py_roundtrip(v):
     return lib().orc_quant_py_roundtrip(v)
+
+
+def nm_reorder_sample(m, N, M, sched, maxiter, round_cap):
+    """First round_cap rounds of an N:M run (CPU timing sample)."""
+    st = np.zeros(5, np.int32)
+    _check(lib().orc_nm_reorder_sample(ctypes.byref(m.c()), N, M, sched, maxiter, round_cap,
+                                       _ptr(st, ctypes.c_int32)), "nm_reorder_sample")
+    return st
 
 
 def reload(m, M):

@@ -91,11 +91,59 @@ int graph_clone(vnm_graph *src, vnm_graph **out) {
     return VNM_OK;
 }
 
+KernelTimer::KernelTimer(vnm_ctx *c, const char *n) : ctx(c), name(n) {
+    if (!ctx->timing) return;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        a = b = nullptr;
+        (void)hipGetLastError();
+        return;
+    }
+    (void)hipEventRecord(a, ctx->stream);
+}
+
+void KernelTimer::stop() {
+    if (!a) return;
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->pending.push_back(vnm_timer_pending{name, a, b});
+    a = b = nullptr;
+}
+
+static void resolve_timers(vnm_ctx *c) {
+    for (auto &p : c->pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            auto &t = c->timers[p.name];
+            t.first += ms;
+            t.second += 1;
+        }
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    c->pending.clear();
+}
+
 }  // namespace vnm
 
 using namespace vnm;
 
 extern "C" {
+
+int vnm_ctx_timing(vnm_ctx *c, int enable) {
+    if (!c) return set_error(VNM_E_ARG, "null context");
+    resolve_timers(c);
+    c->timing = enable != 0;
+    c->timers.clear();
+    return VNM_OK;
+}
+
+int vnm_ctx_timer_read(vnm_ctx *c, const char *kernel, double *total_ms, int64_t *launches) {
+    if (!c || !kernel || !total_ms || !launches) return set_error(VNM_E_ARG, "null argument");
+    resolve_timers(c);
+    auto it = c->timers.find(kernel);
+    *total_ms = it == c->timers.end() ? 0.0 : it->second.first;
+    *launches = it == c->timers.end() ? 0 : it->second.second;
+    return VNM_OK;
+}
 
 int vnm_ctx_create(int device, vnm_ctx **out) {
     if (!out) return set_error(VNM_E_ARG, "null output");
@@ -120,6 +168,7 @@ int vnm_ctx_create(int device, vnm_ctx **out) {
 
 void vnm_ctx_destroy(vnm_ctx *c) {
     if (!c) return;
+    resolve_timers(c);
     if (c->stream) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamDestroy(c->stream);

@@ -5,10 +5,24 @@
 
 #include "vnm_internal.h"
 
+#include <map>
+#include <string>
+#include <vector>
+
+// Kernel timers: events recorded around selected launches on the context's
+// stream (the reference's GpuTimer, util/utility.h:267-301), resolved lazily.
+struct vnm_timer_pending {
+    std::string name;
+    hipEvent_t a, b;
+};
+
 struct vnm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int64_t launches = 0;
+    bool timing = false;
+    std::vector<vnm_timer_pending> pending;
+    std::map<std::string, std::pair<double, int64_t>> timers;
 };
 
 namespace vnm {
@@ -57,5 +71,14 @@ int venom_reorder(vnm_graph *g, int V, int maxiter, vnm_venom_stats *st);
 int reload_sort(vnm_graph *g, bool quantize);
 int graph_clone(vnm_graph *src, vnm_graph **out);
 int ensure_tab(vnm_graph *g, size_t slots);
+
+// Event pair around one launch when the context's timers are on.
+struct KernelTimer {
+    vnm_ctx *ctx;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelTimer(vnm_ctx *c, const char *n);
+    void stop();
+};
 
 }  // namespace vnm

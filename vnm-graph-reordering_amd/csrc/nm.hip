@@ -1017,8 +1017,10 @@ int locate_all(vnm_graph *g, int N, int32_t *d_cnt, int32_t *d_nzr) {
     int grid = locate_grid(g);
     int rc = ensure_tab(g, std::max<size_t>(grid, KMAX + 1));
     if (rc) return rc;
+    KernelTimer kt(g->ctx, "k_locate_all");
     hipLaunchKernelGGL(k_locate_all, dim3(grid), dim3(LOC_T), 0, g->ctx->stream, g->nrows, g->M, N,
                        g->nseg(), g->inv.p, g->cptr.p, g->crow.p, g->tab.p, d_cnt, d_nzr);
+    kt.stop();
     g->ctx->launches++;
     VNM_LAUNCH_CHECK("k_locate_all");
     return VNM_OK;
@@ -1058,7 +1060,9 @@ int nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *stats)
     A.cptr = g->cptr.p; A.crow = g->crow.p; A.perm = g->perm.p; A.inv = g->inv.p;
     A.cnt = g->cnt.p; A.nzr = g->nzr.p; A.tab = g->tab.p; A.out = out;
     A.trace = nullptr; A.trace_cap = 0;
+    KernelTimer kt(g->ctx, "k_nm_greedy");
     hipLaunchKernelGGL(k_nm_greedy, dim3(1), dim3(GT), 0, st, A);
+    kt.stop();
     g->ctx->launches++;
     VNM_LAUNCH_CHECK("k_nm_greedy");
     int32_t h[8];
@@ -1124,8 +1128,10 @@ int vnm_swap_score(vnm_graph *g, int N, int seg1, const int32_t *seg2, int count
         hipLaunchKernelGGL(k_fill_slots, dim3(n + 1), dim3(256), 0, st, g->nrows, M, dseg.p,
                            g->inv.p, g->cptr.p, g->crow.p, g->tab.p, 0);
         int gx = std::max(1, std::min((g->nrows + SC_ROWS - 1) / SC_ROWS, 1024));
+        KernelTimer kt(g->ctx, "k_score_rows");
         hipLaunchKernelGGL(k_score_rows, dim3(gx, n), dim3(SC_ROWS), 0, st, g->nrows, M, N, n,
                            g->tab.p, dsc.p);
+        kt.stop();
         hipLaunchKernelGGL(k_fill_slots, dim3(n + 1), dim3(256), 0, st, g->nrows, M, dseg.p,
                            g->inv.p, g->cptr.p, g->crow.p, g->tab.p, 1);
         g->ctx->launches += 3;

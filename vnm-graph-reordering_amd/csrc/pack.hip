@@ -682,12 +682,14 @@ int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, 
     hipLaunchKernelGGL(k_zero_rows, dim3(std::max(p->nrows, 1)), dim3(256), 0, st, p->rowptr.p,
                        p->row_multi.p, p->nrows, n, C, ldc);
     if (p->nitems > 0) {
+        KernelTimer kt(p->ctx, "k_spmm");
         switch (p->dtype) {
             case VNM_DT_F32: launch_spmm_b<VNM_DT_F32>(p, B, ldb, n, bt, bs, C, ldc, st); break;
             case VNM_DT_BF16: launch_spmm_b<VNM_DT_BF16>(p, B, ldb, n, bt, bs, C, ldc, st); break;
             case VNM_DT_F16: launch_spmm_b<VNM_DT_F16>(p, B, ldb, n, bt, bs, C, ldc, st); break;
             default: launch_spmm_b<VNM_DT_FP8>(p, B, ldb, n, bt, bs, C, ldc, st); break;
         }
+        kt.stop();
     }
     p->ctx->launches += 2;
     VNM_LAUNCH_CHECK("k_spmm");

@@ -27,6 +27,7 @@ EXPORTS = [
     "vnm_last_error", "vnm_mtx_read", "vnm_mtx_write", "vnm_coo_from_tuples", "vnm_coo_free",
     "vnm_gen_er", "vnm_gen_rmat", "vnm_gen_chung_lu", "vnm_gen_normal",
     "vnm_ctx_create", "vnm_ctx_destroy", "vnm_ctx_stream", "vnm_ctx_launches",
+    "vnm_ctx_timing", "vnm_ctx_timer_read",
     "vnm_graph_create", "vnm_graph_destroy", "vnm_graph_download", "vnm_graph_perm",
     "vnm_locate", "vnm_swap_score", "vnm_pq_pop_order", "vnm_nm_reorder",
     "vnm_venom_eval", "vnm_venom_reorder", "vnm_hybrid", "vnm_find_best_sched",
@@ -124,6 +125,8 @@ def lib():
     L.vnm_ctx_stream.restype = vp
     L.vnm_ctx_launches.argtypes = [vp]
     L.vnm_ctx_launches.restype = ctypes.c_int64
+    L.vnm_ctx_timing.argtypes = [vp, ctypes.c_int]
+    L.vnm_ctx_timer_read.argtypes = [vp, ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_int64)]
     L.vnm_graph_create.argtypes = [vp, P(Coo), ctypes.c_int, P(vp)]
     L.vnm_graph_destroy.argtypes = [vp]
     L.vnm_graph_destroy.restype = None
@@ -261,6 +264,15 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def timing(self, enable=True):
+        _check(lib().vnm_ctx_timing(self.h, int(enable)))
+
+    def timer(self, kernel):
+        """(total milliseconds, launches) of a timed kernel."""
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        _check(lib().vnm_ctx_timer_read(self.h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def pq_pop_order(self, scores, M):
         scores = np.ascontiguousarray(scores, np.int32).reshape(-1, 2 * M * M)
