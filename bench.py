@@ -118,10 +118,12 @@ def main():
     C8 = C.cpu().numpy()
 
     if a.copy_calib:
-        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        # plain streaming kernel of known traffic for counter calibration:
+        # reads 96 MiB, writes 96 MiB
+        src = torch.ones(24 * 1024 * 1024, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.add(src, 1.0, out=dst)
         torch.cuda.synchronize()
 
     res = {

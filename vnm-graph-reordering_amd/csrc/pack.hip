@@ -15,15 +15,21 @@
 // laid out row-major (CSR over rows), so the SpMM accumulates each output row
 // in registers and writes it once.
 //
-//   bytes(A) = 4 (R + 1) + E (4 + 1 + 2 s_v) + nb (4 + 2 + 4) + 8 * items
+//   bytes(A) = 4 (R + 1) + E (4 + 4 + 1 + 2 s_v) + nb (4 + 2 + 4)
+// (rowptr; per row entry its row, block, selector byte and two values; per
+// block its segment, selectors and scale).
 //
-// SpMM: C[R x n] = A_vnm B, FP32 accumulation.  Work items are (row, up to
-// 64 entries); a wave takes one item, each lane owns n / 64 consecutive
-// columns and gathers the selected B rows with vector loads.  Items of the
-// same row add into C with FP32 atomics (rows longer than 64 entries only).
-// Item indices are remapped so that consecutive items run on the same XCD
-// (workgroups are dealt round-robin over the 8 XCDs), keeping the B rows of
-// neighbouring (reordered, hence similar) rows in one L2.
+// SpMM: C[R x n] = A_vnm B, FP32 accumulation.  A work item is 64
+// consecutive row entries (rows may start or end inside it); a wave takes one
+// item: lane k first resolves entry k (its two B rows and scaled values),
+// then the wave walks the entries, gathering B rows four entries deep with
+// every lane owning n / 64 consecutive columns, and flushes the row
+// accumulator whenever the row changes.  Rows wholly inside an item are
+// stored once; a row cut by an item boundary adds into C with FP32 atomics
+// (those rows, and rows that keep nothing, are zeroed first from a list built
+// at pack time).  Item indices are remapped so that consecutive items run on
+// the same XCD (workgroups are dealt round-robin over the 8 XCDs), keeping the
+// B rows of neighbouring (reordered, hence similar) rows in one L2.
 #include "graph.h"
 #include "hip_common.h"
 #include "sortlib.h"
@@ -38,13 +44,14 @@
 struct vnm_packed {
     vnm_ctx *ctx = nullptr;
     int32_t V = 0, M = 0, nrows = 0, ncols = 0, dtype = 0;
-    int32_t nblocks = 0, nent = 0, nitems = 0;
+    int32_t nblocks = 0, nent = 0;
     int64_t retained = 0;
-    vnm::DBuf<int32_t> rowptr, ent_blk, blk_seg, items;   // items: (row, begin) pairs
+    int32_t nzero = 0;                                     // rows in zero_rows
+    vnm::DBuf<int32_t> rowptr, ent_row, ent_blk, blk_seg;
     vnm::DBuf<uint8_t> ent_idx, ent_val;                   // values: 2 per entry, dtype-sized
     vnm::DBuf<uint16_t> blk_sel;
     vnm::DBuf<float> blk_scale;
-    vnm::DBuf<uint8_t> row_multi;                          // row spans > 1 item
+    vnm::DBuf<int32_t> zero_rows;                          // empty rows and rows cut by items
 };
 
 namespace vnm {
@@ -288,13 +295,16 @@ __global__ void k_scales(const uint32_t *__restrict__ absmax, int nb, int fp8, f
 }
 
 __global__ void k_emit(const int32_t *__restrict__ sel_runs, int E, const int32_t *__restrict__ run_blk,
+                       const uint64_t *__restrict__ keys, const int32_t *__restrict__ rstart,
                        const float *__restrict__ kv, const uint8_t *__restrict__ kidx,
-                       const float *__restrict__ sc, int dtype, int32_t *__restrict__ ent_blk,
-                       uint8_t *__restrict__ ent_idx, uint8_t *__restrict__ ent_val) {
+                       const float *__restrict__ sc, int dtype, int32_t *__restrict__ ent_row,
+                       int32_t *__restrict__ ent_blk, uint8_t *__restrict__ ent_idx,
+                       uint8_t *__restrict__ ent_val) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     const int u = sel_runs[e];
     const int b = run_blk[u];
+    ent_row[e] = (int32_t)(keys[rstart[u]] >> 32);
     ent_blk[e] = b;
     ent_idx[e] = kidx[u];
     const float v0 = kv[2 * u], v1 = kv[2 * u + 1];
@@ -317,25 +327,21 @@ __global__ void k_emit(const int32_t *__restrict__ sel_runs, int E, const int32_
     }
 }
 
-__global__ void k_items_count(const int32_t *__restrict__ rowptr, int R, int32_t *__restrict__ nit,
-                              uint8_t *__restrict__ multi) {
+// Rows the SpMM must zero before accumulating: rows that keep nothing and
+// rows cut by a 64-entry item boundary.
+__global__ void k_zero_flags(const int32_t *__restrict__ rowptr, int R, int32_t *__restrict__ flag) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
-    const int len = rowptr[r + 1] - rowptr[r];
-    const int n = (len + ITEM - 1) / ITEM;
-    nit[r] = n;
-    multi[r] = n > 1;
+    const int b = rowptr[r], e = rowptr[r + 1];
+    const bool empty = e == b;
+    const bool cut = !empty && (b / ITEM != (e - 1) / ITEM);
+    flag[r] = empty || cut;
 }
 
-__global__ void k_items_fill(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ nit_excl,
-                             const int32_t *__restrict__ nit, int R, int32_t *__restrict__ items) {
+__global__ void k_zero_list(const int32_t *__restrict__ flag, const int32_t *__restrict__ excl, int R,
+                            int32_t *__restrict__ list) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= R) return;
-    const int base = nit_excl[r];
-    for (int k = 0; k < nit[r]; ++k) {
-        items[2 * (base + k)] = r;
-        items[2 * (base + k) + 1] = rowptr[r] + k * ITEM;
-    }
+    if (r < R && flag[r]) list[excl[r]] = r;
 }
 
 // ------------------------------------------------------------ SpMM
@@ -389,58 +395,11 @@ __device__ __forceinline__ void load_brow(const void *B, int ldb, int c, int col
     }
 }
 
-template <int AT, int BT, int CPL>
-__global__ __launch_bounds__(256) void k_spmm(
-    const int32_t *__restrict__ items, int nitems, const int32_t *__restrict__ rowptr,
-    const uint8_t *__restrict__ row_multi, const int32_t *__restrict__ ent_blk,
-    const uint8_t *__restrict__ ent_idx, const uint8_t *__restrict__ ent_val,
-    const int32_t *__restrict__ blk_seg, const uint16_t *__restrict__ blk_sel,
-    const float *__restrict__ blk_scale, int M, const void *__restrict__ B, int ldb,
-    const float *__restrict__ b_scales, float *__restrict__ C, int ldc, int n_tiles, int nwg) {
-    const int lane = threadIdx.x & 63;
-    const int wpb = blockDim.x / 64;
-    // XCD-aware remap: workgroup g runs on XCD g % 8; give each XCD a
-    // contiguous range of items (the grid is a multiple of 8, so the map
-    // from blockIdx to lwg is a bijection on [0, gridDim.x))
-    const int per = gridDim.x / 8;
-    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-    const int lwg = xcd * per + slot;
-    if (lwg >= nwg) return;
-    const int tile = lwg % n_tiles;
-    const int wid = (lwg / n_tiles) * wpb + (threadIdx.x >> 6);
-    if (wid >= nitems) return;
-    const int r = items[2 * wid];
-    const int beg = items[2 * wid + 1];
-    const int end = min(beg + ITEM, rowptr[r + 1]);
-    const int col0 = tile * 64 * CPL + lane * CPL;
-    float acc[CPL];
-#pragma unroll
-    for (int i = 0; i < CPL; ++i) acc[i] = 0.f;
-    for (int e = beg; e < end; ++e) {
-        const int b = __builtin_amdgcn_readfirstlane(ent_blk[e]);
-        const uint32_t code = __builtin_amdgcn_readfirstlane((uint32_t)ent_idx[e]);
-        const int seg = __builtin_amdgcn_readfirstlane(blk_seg[b]);
-        const uint32_t sel = __builtin_amdgcn_readfirstlane((uint32_t)blk_sel[b]);
-        const float sc = blk_scale[b];
-        const int c0 = seg * M + ((sel >> (4 * (code & 3u))) & 15u);
-        float x[CPL];
-        load_brow<BT, CPL>(B, ldb, c0, col0, x);
-        float a0 = load_aval<AT>(ent_val, 2 * e) * sc;
-        if constexpr (BT == VNM_DT_FP8) a0 *= b_scales[c0 >> 7];
-#pragma unroll
-        for (int i = 0; i < CPL; ++i) acc[i] = fmaf(a0, x[i], acc[i]);
-        if (code & 0x10u) {
-            const int c1 = seg * M + ((sel >> (4 * ((code >> 2) & 3u))) & 15u);
-            float y[CPL];
-            load_brow<BT, CPL>(B, ldb, c1, col0, y);
-            float a1 = load_aval<AT>(ent_val, 2 * e + 1) * sc;
-            if constexpr (BT == VNM_DT_FP8) a1 *= b_scales[c1 >> 7];
-#pragma unroll
-            for (int i = 0; i < CPL; ++i) acc[i] = fmaf(a1, y[i], acc[i]);
-        }
-    }
-    float *cp = C + (size_t)r * ldc + col0;
-    if (row_multi[r]) {
+template <int CPL>
+__device__ __forceinline__ void flush_row(float *__restrict__ C, int ldc, int row, int col0,
+                                          const float (&acc)[CPL], bool atomic) {
+    float *cp = C + (size_t)row * ldc + col0;
+    if (atomic) {
 #pragma unroll
         for (int i = 0; i < CPL; ++i) atomicAdd(&cp[i], acc[i]);
     } else {
@@ -453,13 +412,114 @@ __global__ __launch_bounds__(256) void k_spmm(
     }
 }
 
-// Rows that keep nothing are zero; rows spread over several items start at 0.
-__global__ void k_zero_rows(const int32_t *__restrict__ rowptr, const uint8_t *__restrict__ multi,
-                            int R, int n, float *__restrict__ C, int ldc) {
-    const int r = blockIdx.x;
-    if (r >= R) return;
-    if (rowptr[r + 1] > rowptr[r] && !multi[r]) return;
-    for (int c = threadIdx.x; c < n; c += blockDim.x) C[(size_t)r * ldc + c] = 0.f;
+template <int AT, int BT, int CPL>
+__global__ __launch_bounds__(256) void k_spmm(
+    int E, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ ent_row,
+    const int32_t *__restrict__ ent_blk, const uint8_t *__restrict__ ent_idx,
+    const uint8_t *__restrict__ ent_val, const int32_t *__restrict__ blk_seg,
+    const uint16_t *__restrict__ blk_sel, const float *__restrict__ blk_scale, int M,
+    const void *__restrict__ B, int ldb, const float *__restrict__ b_scales,
+    float *__restrict__ C, int ldc, int n_tiles, int nwg) {
+    const int lane = threadIdx.x & 63;
+    const int wpb = blockDim.x / 64;
+    // XCD-aware remap: workgroup g runs on XCD g % 8; give each XCD a
+    // contiguous range of items (the grid is a multiple of 8, so the map
+    // from blockIdx to lwg is a bijection on [0, gridDim.x))
+    const int per = gridDim.x / 8;
+    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    const int lwg = xcd * per + slot;
+    if (lwg >= nwg) return;
+    const int tile = lwg % n_tiles;
+    const int item = (lwg / n_tiles) * wpb + (threadIdx.x >> 6);
+    const int beg = item * ITEM;
+    if (beg >= E) return;
+    const int cnt = min(ITEM, E - beg);
+    const int col0 = tile * 64 * CPL + lane * CPL;
+    // Lane k resolves entry beg + k once: its row, both B rows and values.
+    int rr = 0, c0 = 0, c1 = -1;
+    float a0 = 0.f, a1 = 0.f;
+    if (lane < cnt) {
+        const int e = beg + lane;
+        rr = ent_row[e];
+        const int b = ent_blk[e];
+        const uint32_t code = ent_idx[e];
+        const int base = blk_seg[b] * M;
+        const uint32_t sel = blk_sel[b];
+        const float sc = blk_scale[b];
+        c0 = base + ((sel >> (4 * (code & 3u))) & 15u);
+        a0 = load_aval<AT>(ent_val, 2 * e) * sc;
+        if (code & 0x10u) {
+            c1 = base + ((sel >> (4 * ((code >> 2) & 3u))) & 15u);
+            a1 = load_aval<AT>(ent_val, 2 * e + 1) * sc;
+        }
+        if constexpr (BT == VNM_DT_FP8) {
+            a0 *= b_scales[c0 >> 7];
+            if (c1 >= 0) a1 *= b_scales[c1 >> 7];
+        }
+    }
+    const int first_row = __builtin_amdgcn_readlane(rr, 0);
+    const int last_row = __builtin_amdgcn_readlane(rr, (cnt - 1) & 63);
+    float acc[CPL];
+#pragma unroll
+    for (int i = 0; i < CPL; ++i) acc[i] = 0.f;
+    int cur = first_row;
+    // Gather B rows four entries at a time (up to eight rows in flight).
+    constexpr int U = 4;
+    for (int k = 0; k < cnt; k += U) {
+        float x[U][2][CPL];
+        float w[U][2];
+        int rowk[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k + u;  // wave-uniform
+            const bool ok = kk < cnt;
+            const int ck0 = __builtin_amdgcn_readlane(c0, kk & 63);
+            const int ck1 = __builtin_amdgcn_readlane(c1, kk & 63);
+            // slots past the item's end repeat its last row, so they never
+            // look like a row change
+            rowk[u] = ok ? __builtin_amdgcn_readlane(rr, kk & 63) : last_row;
+            w[u][0] = ok ? __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a0), kk & 63)) : 0.f;
+            w[u][1] = (ok && ck1 >= 0) ? __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a1), kk & 63)) : 0.f;
+            if (ok) load_brow<BT, CPL>(B, ldb, ck0, col0, x[u][0]);
+            else {
+#pragma unroll
+                for (int i = 0; i < CPL; ++i) x[u][0][i] = 0.f;
+            }
+            if (ok && ck1 >= 0) load_brow<BT, CPL>(B, ldb, ck1, col0, x[u][1]);
+            else {
+#pragma unroll
+                for (int i = 0; i < CPL; ++i) x[u][1][i] = 0.f;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (rowk[u] != cur) {  // wave-uniform: the row changed
+                const bool cut = (cur == first_row && rowptr[cur] < beg) ||
+                                 (cur == last_row && rowptr[cur + 1] > beg + cnt);
+                flush_row<CPL>(C, ldc, cur, col0, acc, cut);
+#pragma unroll
+                for (int i = 0; i < CPL; ++i) acc[i] = 0.f;
+                cur = rowk[u];
+            }
+#pragma unroll
+            for (int i = 0; i < CPL; ++i)
+                acc[i] = fmaf(w[u][1], x[u][1][i], fmaf(w[u][0], x[u][0][i], acc[i]));
+        }
+    }
+    const bool cut = (cur == first_row && rowptr[cur] < beg) ||
+                     (cur == last_row && rowptr[cur + 1] > beg + cnt);
+    flush_row<CPL>(C, ldc, cur, col0, acc, cut);
+}
+
+// Zero the rows of the pack-time list (rows that keep nothing, rows cut by
+// an item boundary): one wave per row, float4 stores.
+__global__ void k_zero_rows(const int32_t *__restrict__ list, int nz, int n, float *__restrict__ C,
+                            int ldc) {
+    const int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (w >= nz) return;
+    const int lane = threadIdx.x & 63;
+    float *cp = C + (size_t)list[w] * ldc;
+    for (int c = lane * 4; c < n; c += 256) *(float4 *)(cp + c) = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // ------------------------------------------------------------ host
@@ -604,29 +664,29 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
     p->nent = E;
     const int vs = dtype == VNM_DT_F32 ? 4 : dtype == VNM_DT_FP8 ? 1 : 2;
     if ((rc = p->rowptr.alloc(R + 1)) || (rc = p->ent_blk.alloc(std::max(E, 1))) ||
-        (rc = p->ent_idx.alloc(std::max(E, 1))) || (rc = p->ent_val.alloc((size_t)std::max(E, 1) * 2 * vs)) ||
-        (rc = p->row_multi.alloc(R))) {
+        (rc = p->ent_row.alloc(std::max(E, 1))) || (rc = p->ent_idx.alloc(std::max(E, 1))) ||
+        (rc = p->ent_val.alloc((size_t)std::max(E, 1) * 2 * vs))) {
         delete p;
         return rc;
     }
     if ((rc = scan_excl_i32(rcnt.p, p->rowptr.p, R + 1, st))) { delete p; return rc; }
     if (E > 0)
         hipLaunchKernelGGL(k_emit, dim3(grid_of(E, 256)), dim3(256), 0, st, sel_runs.p, E, run_blk.p,
-                           kv.p, kidx.p, p->blk_scale.p, dtype, p->ent_blk.p, p->ent_idx.p, p->ent_val.p);
-    // work items
-    DBuf<int32_t> nit, nit_excl;
-    if ((rc = nit.alloc(R + 1)) || (rc = nit_excl.alloc(R + 1))) { delete p; return rc; }
-    VNM_HIP_TRY(hipMemsetAsync(nit.p, 0, (size_t)(R + 1) * 4, st));
-    hipLaunchKernelGGL(k_items_count, dim3(grid_of(R, 256)), dim3(256), 0, st, p->rowptr.p, R, nit.p,
-                       p->row_multi.p);
-    if ((rc = scan_excl_i32(nit.p, nit_excl.p, R + 1, st))) { delete p; return rc; }
-    int32_t nitems = 0;
-    VNM_HIP_TRY(hipMemcpyAsync(&nitems, nit_excl.p + R, 4, hipMemcpyDeviceToHost, st));
+                           keys2.p, rstart.p, kv.p, kidx.p, p->blk_scale.p, dtype, p->ent_row.p,
+                           p->ent_blk.p, p->ent_idx.p, p->ent_val.p);
+    // rows to zero before accumulating
+    DBuf<int32_t> zflag, zexcl;
+    if ((rc = zflag.alloc(R + 1)) || (rc = zexcl.alloc(R + 1))) { delete p; return rc; }
+    VNM_HIP_TRY(hipMemsetAsync(zflag.p, 0, (size_t)(R + 1) * 4, st));
+    hipLaunchKernelGGL(k_zero_flags, dim3(grid_of(R, 256)), dim3(256), 0, st, p->rowptr.p, R, zflag.p);
+    if ((rc = scan_excl_i32(zflag.p, zexcl.p, R + 1, st))) { delete p; return rc; }
+    int32_t nzero = 0;
+    VNM_HIP_TRY(hipMemcpyAsync(&nzero, zexcl.p + R, 4, hipMemcpyDeviceToHost, st));
     VNM_HIP_TRY(hipStreamSynchronize(st));
-    p->nitems = nitems;
-    if ((rc = p->items.alloc(2 * (size_t)std::max(nitems, 1)))) { delete p; return rc; }
-    hipLaunchKernelGGL(k_items_fill, dim3(grid_of(R, 256)), dim3(256), 0, st, p->rowptr.p, nit_excl.p,
-                       nit.p, R, p->items.p);
+    p->nzero = nzero;
+    if ((rc = p->zero_rows.alloc(std::max(nzero, 1)))) { delete p; return rc; }
+    hipLaunchKernelGGL(k_zero_list, dim3(grid_of(R, 256)), dim3(256), 0, st, zflag.p, zexcl.p, R,
+                       p->zero_rows.p);
     g->ctx->launches += 16;
     VNM_LAUNCH_CHECK("pack emit");
     VNM_HIP_TRY(hipStreamSynchronize(st));
@@ -634,32 +694,25 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
     return VNM_OK;
 }
 
+template <int AT, int BT, int CPL>
+static void launch_spmm_cpl(vnm_packed *p, const void *B, int ldb, int n, const float *bs, float *C,
+                            int ldc, hipStream_t st) {
+    const int wpb = 4;
+    const int tiles = n / (64 * CPL);
+    const int items = (p->nent + ITEM - 1) / ITEM;
+    const int nwg = (items + wpb - 1) / wpb * tiles;
+    hipLaunchKernelGGL((k_spmm<AT, BT, CPL>), dim3((nwg + 7) / 8 * 8), dim3(64 * wpb), 0, st,
+                       p->nent, p->rowptr.p, p->ent_row.p, p->ent_blk.p, p->ent_idx.p, p->ent_val.p,
+                       p->blk_seg.p, p->blk_sel.p, p->blk_scale.p, p->M, B, ldb, bs, C, ldc, tiles,
+                       nwg);
+}
+
 template <int AT, int BT>
 static void launch_spmm(vnm_packed *p, const void *B, int ldb, int n, const float *bs, float *C,
                         int ldc, hipStream_t st) {
-    const int wpb = 4;
-    if (n % 256 == 0) {
-        const int tiles = n / 256;
-        const int nwg = (p->nitems + wpb - 1) / wpb * tiles;
-        hipLaunchKernelGGL((k_spmm<AT, BT, 4>), dim3((nwg + 7) / 8 * 8), dim3(64 * wpb), 0, st,
-                           p->items.p, p->nitems, p->rowptr.p, p->row_multi.p, p->ent_blk.p,
-                           p->ent_idx.p, p->ent_val.p, p->blk_seg.p, p->blk_sel.p, p->blk_scale.p,
-                           p->M, B, ldb, bs, C, ldc, tiles, nwg);
-    } else if (n % 128 == 0) {
-        const int tiles = n / 128;
-        const int nwg = (p->nitems + wpb - 1) / wpb * tiles;
-        hipLaunchKernelGGL((k_spmm<AT, BT, 2>), dim3((nwg + 7) / 8 * 8), dim3(64 * wpb), 0, st,
-                           p->items.p, p->nitems, p->rowptr.p, p->row_multi.p, p->ent_blk.p,
-                           p->ent_idx.p, p->ent_val.p, p->blk_seg.p, p->blk_sel.p, p->blk_scale.p,
-                           p->M, B, ldb, bs, C, ldc, tiles, nwg);
-    } else {
-        const int tiles = n / 64;
-        const int nwg = (p->nitems + wpb - 1) / wpb * tiles;
-        hipLaunchKernelGGL((k_spmm<AT, BT, 1>), dim3((nwg + 7) / 8 * 8), dim3(64 * wpb), 0, st,
-                           p->items.p, p->nitems, p->rowptr.p, p->row_multi.p, p->ent_blk.p,
-                           p->ent_idx.p, p->ent_val.p, p->blk_seg.p, p->blk_sel.p, p->blk_scale.p,
-                           p->M, B, ldb, bs, C, ldc, tiles, nwg);
-    }
+    if (n % 256 == 0) launch_spmm_cpl<AT, BT, 4>(p, B, ldb, n, bs, C, ldc, st);
+    else if (n % 128 == 0) launch_spmm_cpl<AT, BT, 2>(p, B, ldb, n, bs, C, ldc, st);
+    else launch_spmm_cpl<AT, BT, 1>(p, B, ldb, n, bs, C, ldc, st);
 }
 
 template <int AT>
@@ -679,9 +732,10 @@ int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, 
     if ((bt == VNM_DT_BF16 && (ldb % 4) != 0) || (bt == VNM_DT_FP8 && (ldb % 4) != 0) || (ldc % 4) != 0)
         return set_error(VNM_E_ARG, "spmm: leading dimensions must be multiples of 4");
     hipStream_t st = p->ctx->stream;
-    hipLaunchKernelGGL(k_zero_rows, dim3(std::max(p->nrows, 1)), dim3(256), 0, st, p->rowptr.p,
-                       p->row_multi.p, p->nrows, n, C, ldc);
-    if (p->nitems > 0) {
+    if (p->nzero > 0)
+        hipLaunchKernelGGL(k_zero_rows, dim3((p->nzero + 3) / 4), dim3(256), 0, st, p->zero_rows.p,
+                           p->nzero, n, C, ldc);
+    if (p->nent > 0) {
         KernelTimer kt(p->ctx, "k_spmm");
         switch (p->dtype) {
             case VNM_DT_F32: launch_spmm_b<VNM_DT_F32>(p, B, ldb, n, bt, bs, C, ldc, st); break;
@@ -723,8 +777,8 @@ int vnm_packed_info_get(const vnm_packed *p, vnm_packed_info *info) {
     info->dtype = p->dtype;
     info->retained_nnz = p->retained;
     const int vs = p->dtype == VNM_DT_F32 ? 4 : p->dtype == VNM_DT_FP8 ? 1 : 2;
-    info->bytes = 4ll * (p->nrows + 1) + (int64_t)p->nent * (4 + 1 + 2 * vs) +
-                  (int64_t)p->nblocks * (4 + 2 + 4) + 8ll * p->nitems + p->nrows;
+    info->bytes = 4ll * (p->nrows + 1) + (int64_t)p->nent * (4 + 4 + 1 + 2 * vs) +
+                  (int64_t)p->nblocks * (4 + 2 + 4);
     return VNM_OK;
 }
 
