@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--rounds", type=int, default=10, help="hybrid VENOM/N:M rounds (--maxiter)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calib", action="store_true",
-                    help="also run a plain 256 MiB device copy (counter calibration)")
+                    help="also run a streaming 96 MiB add (counter calibration)")
     return ap.parse_args()
 
 
@@ -126,6 +126,12 @@ def main():
             torch.add(src, 1.0, out=dst)
         torch.cuda.synchronize()
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_spmm_rmat65536_V128_traffic.json")
+    if os.path.exists(tpath):
+        t = json.load(open(tpath))["k_spmm_f32"]
+        traffic = int(t["read_bytes"] + t["write_bytes"])  # rocprofv3 FETCH/WRITE_SIZE, calibrated
+
     res = {
         "metric": "V:N:M SpMM TFLOP/s (after GPU V:N:M reorder)",
         "value": round(tflops, 4),
@@ -143,7 +149,7 @@ def main():
                    "nnz": coo.nnz, "V": V, "N": N, "M": M, "ncols": NCOLS,
                    "hybrid_rounds": a.rounds, "sched": 0, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_spmm", "kernel_ms": round(k_ms, 5),
                      "alg_bytes": int(alg_bytes), "a_packed_bytes": int(a_bytes)},
         "reorder": {"gpu_ms": round(reorder_wall_ms, 2), "hybrid_ms": round(hs.milliseconds, 2),
