@@ -50,50 +50,54 @@ __device__ __forceinline__ int64_t swap_key(int lg, int rg) {
 // strictly smaller keys), so rank[a] <= rank[b] exactly when
 // key[a] <= key[b], i.e. when Comparator(a, b) holds.
 struct HeapRef {
-    uint8_t *h;           // heap slots (element indices)
-    const uint8_t *rank;  // ranks by element index
+    uint8_t *h;           // heap slots (element indices), slot s at h[s * stride]
+    const uint8_t *rank;  // rank of element e at rank[e * stride]
     int n;                // current size
+    int stride;           // interleave of several heaps (conflict-free LDS)
 };
 
+__device__ __forceinline__ int hget(const HeapRef &H, int slot) { return H.h[slot * H.stride]; }
+__device__ __forceinline__ void hset(HeapRef &H, int slot, int v) { H.h[slot * H.stride] = (uint8_t)v; }
+
 __device__ __forceinline__ bool heap_less(const HeapRef &H, int a, int b) {
-    return H.rank[a] <= H.rank[b];
+    return H.rank[a * H.stride] <= H.rank[b * H.stride];
 }
 
 __device__ __forceinline__ void heap_sift_up(HeapRef &H, int hole, int top, int value) {
     int parent = (hole - 1) / 2;
-    while (hole > top && heap_less(H, H.h[parent], value)) {
-        H.h[hole] = H.h[parent];
+    while (hole > top && heap_less(H, hget(H, parent), value)) {
+        hset(H, hole, hget(H, parent));
         hole = parent;
         parent = (hole - 1) / 2;
     }
-    H.h[hole] = (uint8_t)value;
+    hset(H, hole, value);
 }
 
 __device__ __forceinline__ void heap_push(HeapRef &H, int value) {
-    H.h[H.n] = (uint8_t)value;
+    hset(H, H.n, value);
     H.n += 1;
     heap_sift_up(H, H.n - 1, 0, value);
 }
 
 // Returns the popped element (the former top).
 __device__ __forceinline__ int heap_pop(HeapRef &H) {
-    int top_el = H.h[0];
+    int top_el = hget(H, 0);
     int n = H.n;
     if (n > 1) {
         int last = n - 1;
-        int value = H.h[last];
-        H.h[last] = H.h[0];
+        int value = hget(H, last);
+        hset(H, last, top_el);
         // __adjust_heap(first, 0, len = last, value)
         int len = last, hole = 0, second = 0;
         while (second < (len - 1) / 2) {
             second = 2 * (second + 1);
-            if (heap_less(H, H.h[second], H.h[second - 1])) second--;
-            H.h[hole] = H.h[second];
+            if (heap_less(H, hget(H, second), hget(H, second - 1))) second--;
+            hset(H, hole, hget(H, second));
             hole = second;
         }
         if ((len & 1) == 0 && second == (len - 2) / 2) {
             second = 2 * (second + 1);
-            H.h[hole] = H.h[second - 1];
+            hset(H, hole, hget(H, second - 1));
             hole = second - 1;
         }
         heap_sift_up(H, hole, 0, value);

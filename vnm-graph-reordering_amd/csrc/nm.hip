@@ -32,6 +32,8 @@
 #include "hip_common.h"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 namespace vnm {
@@ -317,19 +319,20 @@ __global__ void k_pq_pop_order(const int32_t *__restrict__ scores, int count, in
                                int32_t *__restrict__ order) {
     extern __shared__ unsigned char smem[];
     const int MM = M * M;
+    // slot-major layout: heap slot s of thread t at [s * blockDim.x + t]
     uint8_t *ranks = smem;
     uint8_t *heaps = smem + (size_t)blockDim.x * MM;
     int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= count) return;
     const int32_t *sc = scores + (size_t)q * 2 * MM;
-    uint8_t *rank = ranks + (size_t)threadIdx.x * MM;
+    uint8_t *rank = ranks + threadIdx.x;
     for (int i = 0; i < MM; ++i) {
         int64_t ki = swap_key(sc[2 * i], sc[2 * i + 1]);
         int r = 0;
         for (int j = 0; j < MM; ++j) r += swap_key(sc[2 * j], sc[2 * j + 1]) < ki;
-        rank[i] = (uint8_t)r;
+        rank[i * blockDim.x] = (uint8_t)r;
     }
-    HeapRef H{heaps + (size_t)threadIdx.x * MM, rank, 0};
+    HeapRef H{heaps + threadIdx.x, rank, 0, (int)blockDim.x};
     for (int i = 0; i < MM; ++i) heap_push(H, i);
     for (int k = 0; k < MM; ++k) order[(size_t)q * MM + k] = heap_pop(H);
 }
@@ -359,7 +362,7 @@ constexpr int KMAX = 9;          // primary + up to M targets (M <= 8)
 constexpr int MMAX = 8;
 constexpr int NPAIR_T = KMAX * (KMAX - 1) / 2 + 1;  // unordered pairs + one self pair
 constexpr int NQ = (KMAX - 1) * (KMAX - 1);         // ordered pairs the greedy can visit
-constexpr int POPP = 12;                            // pops precomputed per ordered pair
+constexpr int POPP = 16;                            // pops precomputed per ordered pair
 
 struct NMArgs {
     int R, M, N, nseg, sched, maxiter;
@@ -385,9 +388,11 @@ struct GreedyShared {
     //   Br[b1*M+b2] = sum over rows in both of x_b1 * (y_b2 ? D : C)
     int both[NPAIR_T][2][MMAX * MMAX];
     int newcnt[KMAX], newnzr[KMAX];
-    uint8_t rank[NQ][MMAX * MMAX];
-    uint8_t heap[NQ][MMAX * MMAX];
-    uint8_t popped[NQ][POPP];
+    // slot-major (element / slot / pop index first, list second), so the
+    // lanes of a wave building different lists touch consecutive bytes
+    uint8_t rank[MMAX * MMAX][NQ];
+    uint8_t heap[MMAX * MMAX][NQ];
+    uint8_t popped[POPP][NQ];
     int heapn[NQ];
     uint8_t vis[KMAX * MMAX];
     int swv1[64], swv2[64], swlg[64], swrg[64], swp[64], swt[64], swpc[64], swtc[64];
@@ -396,6 +401,7 @@ struct GreedyShared {
     uint64_t wbest[NW];
     uint64_t topk[KMAX + 1];
     int64_t wkeys[NW][64];
+    uint64_t phase_clk[8];
     int64_t trace_len;
     int64_t work;
 };
@@ -658,10 +664,10 @@ __device__ void greedy_recount_phase(const NMArgs &A, GreedyShared &S) {
 __device__ int first_free(GreedyShared &S, int q, int a, int b, int M) {
     const int npre = min(POPP, M * M);
     for (int k = 0; k < npre; ++k) {
-        int e = S.popped[q][k];
+        int e = S.popped[k][q];
         if (!S.vis[a * M + e / M] && !S.vis[b * M + e % M]) return e;
     }
-    HeapRef H{S.heap[q], S.rank[q], S.heapn[q]};
+    HeapRef H{&S.heap[0][q], &S.rank[0][q], S.heapn[q], NQ};
     while (H.n > 0) {
         int e = heap_pop(H);
         if (!S.vis[a * M + e / M] && !S.vis[b * M + e % M]) { S.heapn[q] = H.n; return e; }
@@ -693,7 +699,7 @@ __device__ void greedy_apply(GreedyShared &S, int M, bool rev, bool drain) {
         // one invalid segment: swap the top pair with the emptiest segment, no
         // visited filter (reorder/reorder.h:1837-1860)
         const int b = S.self_pair ? 0 : 1;
-        const int e = S.popped[0][0];
+        const int e = S.popped[0][0];  // first pop of list 0
         int lg, rg;
         pair_gain(S, K, M, 0, b, e, lg, rg);
         add_swap(0, b, e, lg, rg, 0, 0);
@@ -745,8 +751,20 @@ __device__ void greedy_apply(GreedyShared &S, int M, bool rev, bool drain) {
     }
 }
 
+// Phase clock (wall_clock64 ticks, 100 MHz on gfx950) accumulated per phase
+// for the diagnostic counters returned in out[8..15].
+#define PHASE_MARK(i)                                                     \
+    do {                                                                  \
+        if (tid == 0) {                                                   \
+            uint64_t _t = __builtin_amdgcn_s_memrealtime();               \
+            S.phase_clk[i] += _t - t_last;                                \
+            t_last = _t;                                                  \
+        }                                                                 \
+    } while (0)
+
 __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
     __shared__ GreedyShared S;
+    uint64_t t_last = __builtin_amdgcn_s_memrealtime();
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
     const int M = A.M, R = A.R, nseg = A.nseg, MM = M * M;
@@ -757,7 +775,10 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
     const int min_size = plain ? 2 : 1;
 
     // initial list size and count (reorder/reorder.h:1808-1810)
-    if (tid == 0) { S.L = 0; S.totcnt = 0; S.work = 0; }
+    if (tid == 0) {
+        S.L = 0; S.totcnt = 0; S.work = 0;
+        for (int i = 0; i < 8; ++i) S.phase_clk[i] = 0;
+    }
     __syncthreads();
     {
         int lsum = 0, csum = 0;
@@ -848,8 +869,10 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
             __syncthreads();
         }
 
+        PHASE_MARK(0);
         // ---- phase C: gains of every pair of the round's segments
         greedy_score_phase(A, S);
+        PHASE_MARK(1);
 
         // ---- phase D: priority queues: dense ranks (one wave per list), then
         // one thread per list pushes all pairs and pops the first POPP
@@ -871,23 +894,25 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
                 int rnk = 0;
                 if (lane < MM)
                     for (int j = 0; j < MM; ++j) rnk += S.wkeys[w][j] < key;
-                if (lane < MM) S.rank[q][lane] = (uint8_t)rnk;
+                if (lane < MM) S.rank[lane][q] = (uint8_t)rnk;
                 __builtin_amdgcn_wave_barrier();
             }
             __syncthreads();
             if (tid < nq) {
-                HeapRef H{S.heap[tid], S.rank[tid], 0};
+                HeapRef H{&S.heap[0][tid], &S.rank[0][tid], 0, NQ};
                 for (int e = 0; e < MM; ++e) heap_push(H, e);
                 const int npre = min(POPP, MM);
-                for (int k = 0; k < npre; ++k) S.popped[tid][k] = (uint8_t)heap_pop(H);
+                for (int k = 0; k < npre; ++k) S.popped[k][tid] = (uint8_t)heap_pop(H);
                 S.heapn[tid] = H.n;
             }
             __syncthreads();
         }
 
+        PHASE_MARK(2);
         // ---- phase E: the sequential greedy (one thread)
         if (tid == 0) greedy_apply(S, M, rev, drain);
         __syncthreads();
+        PHASE_MARK(3);
         if (S.status != 0) break;
 
         // ---- phase F: commit the swaps to perm/inv (reorder(), :392-418)
@@ -914,8 +939,10 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
         if (tid == 0) S.trace_len += S.nsw;
         __syncthreads();
 
+        PHASE_MARK(4);
         // ---- phase G: recount the touched segments (locate_invalid_segment)
         greedy_recount_phase(A, S);
+        PHASE_MARK(5);
         if (tid == 0) {
             for (int j = 0; j < S.K; ++j) {
                 int s = S.sel[j];
@@ -939,6 +966,7 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
         A.out[5] = S.status;
         A.out[6] = (int32_t)min(S.work, (int64_t)0x7FFFFFFF);
         A.out[7] = (int32_t)S.trace_len;
+        for (int i = 0; i < 6; ++i) A.out[8 + i] = (int32_t)min(S.phase_clk[i] / 100, (uint64_t)0x7FFFFFFF);
     }
 }
 
@@ -1065,9 +1093,12 @@ int nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *stats)
     kt.stop();
     g->ctx->launches++;
     VNM_LAUNCH_CHECK("k_nm_greedy");
-    int32_t h[8];
+    int32_t h[16];
     VNM_HIP_TRY(hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, st));
     VNM_HIP_TRY(hipStreamSynchronize(st));
+    if (getenv("VNM_DEBUG"))
+        fprintf(stderr, "[vnm] greedy rounds=%d entries=%d phase_us A=%d C=%d D=%d E=%d F=%d G=%d\n",
+                h[4], h[6], h[8], h[9], h[10], h[11], h[12], h[13]);
     if (h[5] == VNM_E_UB_PATH)
         return set_error(VNM_E_UB_PATH, "greedy reached top() of an empty swap queue "
                                         "(undefined in the reference)");
