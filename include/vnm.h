@@ -137,6 +137,15 @@ typedef struct {
  * GPU; the stats are the reference's return vector. */
 int vnm_nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *st);
 
+/* vnm_nm_reorder with the reference's swap decisions recorded (what the
+ * verbose swap print of run_onepair shows, reorder/reorder.h:1913-1918, as
+ * data): 9 ints
+ * per swap {round, primary, target, v1, v2, lgain, rgain, primary_cnt,
+ * target_cnt}, at most cap records kept, *count = swaps made.  round_cap > 0
+ * stops after that many rounds (training traces for the proposer). */
+int vnm_nm_trace(vnm_graph *g, int N, int sched, int maxiter, int round_cap, int32_t *rec,
+                 int64_t cap, int64_t *count, vnm_nm_stats *st);
+
 /* Replaces `python3 reordering.py --evalonly --m M --v V` (reordering.py:125-198). */
 int vnm_venom_eval(vnm_graph *g, int V, int32_t *inv, int32_t *inv_strict);
 
@@ -203,6 +212,63 @@ void vnm_packed_destroy(vnm_packed *p);
  * scale per 128 rows in b_scales), C FP32, FP32 accumulation. */
 int vnm_spmm(vnm_packed *p, const void *B, int ldb, int n, int b_dtype,
              const float *b_scales, float *C, int ldc);
+
+/* ---------------------------------------------------------------- learned proposer */
+
+/* A graph network that ranks candidate column pairs for the N:M reorder so
+ * that only the top ones are scored exactly.  An addition of this library:
+ * the reference has no learned component; the exact gain it is trained on
+ * and judged by is GetSegSwapScore's (reorder/reorder.h:75-131).
+ * Network: 16 features per node, three neighbour-mean layers of width 64
+ * (BF16 activations and weights, FP32 accumulation), a 34-output head; pair
+ * score s(u, v) = Y0[u] + Y1[v] + sum_j Y[2+j][u] * Y[18+j][v].  22784 FP32
+ * parameters (layout: W0 [32][64], b0, W1 [128][64], b1, W2 [128][64], b2,
+ * Wh [64][64], bh). */
+#define VNM_LP_NPARAMS 22784
+typedef struct vnm_lp_model vnm_lp_model;
+
+/* params == NULL: seeded He-normal initialisation. */
+int  vnm_lp_model_create(vnm_ctx *ctx, const float *params, uint64_t seed, vnm_lp_model **out);
+void vnm_lp_model_destroy(vnm_lp_model *m);
+int  vnm_lp_model_get(const vnm_lp_model *m, float *params);
+
+/* Input features of the current matrix (nrows x 16, FP32 host array). */
+int vnm_lp_features(vnm_graph *g, int N, float *X);
+/* Head outputs (nrows x 64, FP32 host array); bf16 = 0 runs the FP32 path. */
+int vnm_lp_forward(vnm_lp_model *m, vnm_graph *g, int N, int bf16, float *head);
+/* Mean squared error of s(u_i, v_i) against target[i] (pairs = 2*count
+ * current ids) and, when grad != NULL, its gradient (VNM_LP_NPARAMS). */
+int vnm_lp_loss_grad(vnm_lp_model *m, vnm_graph *g, int N, int bf16, const int32_t *pairs,
+                     const float *target, int count, float *loss, float *grad);
+/* One Adam step (BF16 forward, FP32 backward) on one batch of pairs. */
+int vnm_lp_train_step(vnm_lp_model *m, vnm_graph *g, int N, const int32_t *pairs,
+                      const float *target, int count, float lr, float *loss);
+
+/* Exact gain matrices of segment pairs (segp[k], segq[k]), k < count, from
+ * the entries of the two segments; same values and layout as vnm_swap_score
+ * (2*M*M ints per pair).  segp[k] != segq[k]. */
+int vnm_pair_score(vnm_graph *g, int N, const int32_t *segp, const int32_t *segq, int count,
+                   int32_t *scores);
+
+#define VNM_LP_LEARNED    0  /* network proposals; exhaustive window when none pays */
+#define VNM_LP_RANDOM     1  /* seeded random proposals (ablation)                 */
+#define VNM_LP_EXHAUSTIVE 2  /* every window pair scored exactly, count heuristic  */
+
+typedef struct {
+    int32_t init_cnt, final_cnt;     /* rows with more than N ones, summed   */
+    int32_t init_size, final_size;   /* segments holding such rows           */
+    int32_t iters, swaps, fallbacks, gain;
+    int64_t exact_pairs;             /* segment pairs scored exactly         */
+    float   ms;
+} vnm_lp_stats;
+
+/* Learned N:M reorder: per iteration the top `topk` donor segments each
+ * propose one pair from a window of `window` acceptor segments; the pairs
+ * are scored exactly and applied greedily by exact gain, segment-disjoint,
+ * positive gains only, so final_cnt <= init_cnt.  Relabels the graph like
+ * vnm_nm_reorder.  m may be NULL for modes 1 and 2. */
+int vnm_lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window,
+                   int maxiter, uint64_t seed, vnm_lp_stats *st);
 
 #ifdef __cplusplus
 }

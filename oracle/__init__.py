@@ -91,6 +91,10 @@ def lib():
         L.orc_reload.argtypes = [P(Coo), ctypes.c_int]
         L.orc_nm_reorder_sample.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, i32p]
+        f64p = P(ctypes.c_double)
+        L.orc_lp_features.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, f64p]
+        L.orc_lp_loss_grad.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, f64p, i32p, f64p,
+                                       ctypes.c_int, f64p, f64p, f64p]
         _lib = L
     return _lib
 
@@ -282,3 +286,32 @@ def reload(m, M):
 
 def ub_events():
     return lib().orc_ub_events()
+
+
+LP_NPARAMS = 22784
+LP_F = 16
+LP_HEAD = 64
+
+
+def lp_features(m, N, M):
+    """Proposer input features (nrows x 16, FP64) of the matrix as labelled."""
+    X = np.zeros((m.nrows, LP_F), np.float64)
+    _check(lib().orc_lp_features(ctypes.byref(m.c()), N, M, _ptr(X, ctypes.c_double)),
+           "lp_features")
+    return X
+
+
+def lp_loss_grad(m, N, M, params, pairs, target, want_grad=True):
+    """(loss, grad[22784] or None, head[nrows x 64]) of the proposer in FP64."""
+    P = np.ascontiguousarray(params, np.float64)
+    pr = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+    tg = np.ascontiguousarray(target, np.float64)
+    loss = ctypes.c_double(0)
+    grad = np.zeros(LP_NPARAMS, np.float64) if want_grad else None
+    head = np.zeros((m.nrows, LP_HEAD), np.float64)
+    _check(lib().orc_lp_loss_grad(ctypes.byref(m.c()), N, M, _ptr(P, ctypes.c_double),
+                                  _ptr(pr, ctypes.c_int32), _ptr(tg, ctypes.c_double), len(tg),
+                                  ctypes.byref(loss),
+                                  _ptr(grad, ctypes.c_double) if want_grad else None,
+                                  _ptr(head, ctypes.c_double)), "lp_loss_grad")
+    return loss.value, grad, head

@@ -120,6 +120,15 @@ int orc_spmm_vnm_f64(const orc_coo *coo, int V, int M, const float *B,
 float orc_quant_cpp_write(float v);   /* "%.2f" then fscanf %f */
 float orc_quant_py_roundtrip(float v);/* "%.2f" -> float64 -> "%.1e" -> %f */
 
+/* ---- Learned pair proposer (lp_oracle.cpp; an addition of this project,
+ * no reference counterpart).  Features X = nrows x 16 (FP64); the loss of the
+ * pair head against targets and its gradient (22784 parameters, FP64). */
+#define ORC_LP_NPARAMS 22784
+int orc_lp_features(const orc_coo *coo, int N, int M, double *X);
+int orc_lp_loss_grad(const orc_coo *coo, int N, int M, const double *params,
+                     const int32_t *pairs, const double *target, int count,
+                     double *loss, double *grad, double *head);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,210 @@
+"""GPU parity of the learned pair proposer and its exact scorer.
+
+  * the entry-based pair scorer (vnm_pair_score) equals the R-row ballot
+    scorer (vnm_swap_score) and the oracle's GetSegSwapScore restatement,
+    bit for bit;
+  * input features, head outputs, loss and gradients of the network equal the
+    FP64 oracle (oracle/lp_oracle.cpp): within 1e-5 relative for the FP32
+    path, and within the BF16 bounds written below for the MFMA path;
+  * the swap trace with a round cap equals the oracle's trace and output;
+  * the learned reorder only applies positive exact gains: its reported
+    counts equal an oracle recount of the output, the relabel is a bijection
+    and two runs give identical matrices.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import vnm
+
+pytestmark = pytest.mark.gpu
+N, M = 2, 8
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = vnm.Context(0)
+    yield c
+    c.close()
+
+
+def _omat(m):
+    # copies: the oracle relabels in place
+    return oracle.Matrix(m.nrows, m.ncols, m.row.copy(), m.col.copy(), m.val.copy())
+
+
+def _rand(rng, R, dens, M, sym=True):
+    mask = rng.random((R, R)) < dens
+    if sym:
+        mask |= mask.T
+    r, c = np.nonzero(mask)
+    v = rng.uniform(0.1, 1, r.shape[0]).astype(np.float32)
+    return vnm.coo_from_tuples(R, R, r, c, v, M)
+
+
+def test_pair_score_matches_swap_score_and_oracle(ctx):
+    rng = np.random.default_rng(3)
+    cases = [(vnm.gen_rmat(12, 30000, 7), 8, 2), (vnm.gen_er(2048, 41943, 1), 8, 2),
+             (_rand(rng, 96, 0.2, 8), 8, 3), (vnm.gen_rmat(11, 12000, 8, M=4), 4, 1)]
+    for m, MM, NN in cases:
+        g = vnm.Graph(ctx, m, MM)
+        ns = g.nseg
+        p = rng.integers(0, ns, 40).astype(np.int32)
+        q = (p + rng.integers(1, ns, 40)) % ns
+        got = g.pair_score(NN, p, q)
+        om = _omat(m)
+        for k in range(40):
+            ref = g.swap_score(NN, int(p[k]), [int(q[k])])[0]
+            assert np.array_equal(got[k], ref), (k, p[k], q[k])
+            if k < 8:
+                assert np.array_equal(got[k], oracle.swap_score(om, NN, MM, int(p[k]), int(q[k])))
+
+
+def test_pair_score_rejects_self_pair(ctx):
+    g = vnm.Graph(ctx, vnm.gen_er(256, 2000, 2), 8)
+    with pytest.raises(vnm.VnmError):
+        g.pair_score(2, [3], [3])
+
+
+@pytest.mark.parametrize("NN", [1, 2, 3])
+def test_features_match_oracle(ctx, NN):
+    for m in (vnm.gen_rmat(11, 9000, 4), _rand(np.random.default_rng(NN), 64, 0.15, 8, sym=False)):
+        X = vnm.Graph(ctx, m, M).lp_features(NN)
+        ref = oracle.lp_features(_omat(m), NN, M)
+        np.testing.assert_allclose(X, ref, rtol=1e-5, atol=1e-5)
+
+
+def _model_and_params(ctx, seed=5):
+    lm = vnm.LPModel(ctx, seed=seed)
+    P = lm.params().astype(np.float64)
+    # give the biases and the unused head columns nonzero values through a
+    # second model built from explicit parameters
+    rng = np.random.default_rng(seed)
+    P2 = P.copy()
+    for off in (2048, 10304, 18560):
+        P2[off:off + 64] = rng.normal(0, 0.1, 64)
+    P2[22720:22720 + 34] = rng.normal(0, 0.1, 34)
+    return vnm.LPModel(ctx, params=P2.astype(np.float32)), P2.astype(np.float32).astype(np.float64)
+
+
+def _pairs(rng, R, n):
+    return rng.integers(0, R, (n, 2)).astype(np.int32), rng.normal(0, 1, n)
+
+
+def test_forward_fp32_matches_oracle(ctx):
+    m = vnm.gen_rmat(11, 9000, 9)
+    g = vnm.Graph(ctx, m, M)
+    lm, P = _model_and_params(ctx)
+    Y = lm.forward(g, N, bf16=False)
+    _, _, ref = oracle.lp_loss_grad(_omat(m), N, M, P, np.zeros((0, 2), np.int32), np.zeros(0),
+                                    want_grad=False)
+    err = np.abs(Y - ref).max() / np.abs(ref).max()
+    assert err < 1e-5, err
+
+
+def test_forward_bf16_matches_oracle(ctx):
+    # BF16 activations and weights, FP32 accumulation: relative error of the
+    # head at most 3e-2 (Frobenius) against FP64
+    m = vnm.gen_rmat(12, 30000, 10)
+    g = vnm.Graph(ctx, m, M)
+    lm, P = _model_and_params(ctx, 6)
+    Y = lm.forward(g, N, bf16=True)
+    _, _, ref = oracle.lp_loss_grad(_omat(m), N, M, P, np.zeros((0, 2), np.int32), np.zeros(0),
+                                    want_grad=False)
+    err = np.linalg.norm(Y[:, :34] - ref[:, :34]) / np.linalg.norm(ref[:, :34])
+    assert err < 3e-2, err
+
+
+GROUPS = [("W0", 0, 2048), ("b0", 2048, 2112), ("W1", 2112, 10304), ("b1", 10304, 10368),
+          ("W2", 10368, 18560), ("b2", 18560, 18624), ("Wh", 18624, 22720), ("bh", 22720, 22784)]
+
+
+def test_gradient_fp32_matches_oracle(ctx):
+    rng = np.random.default_rng(12)
+    m = vnm.gen_rmat(11, 9000, 11)
+    g = vnm.Graph(ctx, m, M)
+    lm, P = _model_and_params(ctx, 7)
+    pairs, tg = _pairs(rng, m.nrows, 3000)
+    loss, grad = lm.loss_grad(g, N, pairs, tg, bf16=False)
+    rl, rg, _ = oracle.lp_loss_grad(_omat(m), N, M, P, pairs, tg)
+    assert abs(loss - rl) <= 1e-5 * abs(rl)
+    for name, a, b in GROUPS:
+        e = np.linalg.norm(grad[a:b] - rg[a:b]) / max(np.linalg.norm(rg[a:b]), 1e-30)
+        assert e < 1e-4, (name, e)
+
+
+def test_gradient_bf16_matches_oracle(ctx):
+    # BF16 forward, FP32 backward: per parameter group the gradient is within
+    # 5e-2 relative (Frobenius) of FP64 and points the same way (cos > 0.99)
+    rng = np.random.default_rng(13)
+    m = vnm.gen_rmat(12, 30000, 12)
+    g = vnm.Graph(ctx, m, M)
+    lm, P = _model_and_params(ctx, 8)
+    pairs, tg = _pairs(rng, m.nrows, 4000)
+    loss, grad = lm.loss_grad(g, N, pairs, tg, bf16=True)
+    rl, rg, _ = oracle.lp_loss_grad(_omat(m), N, M, P, pairs, tg)
+    assert abs(loss - rl) <= 3e-2 * abs(rl)
+    for name, a, b in GROUPS:
+        x, y = grad[a:b].astype(np.float64), rg[a:b]
+        e = np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-30)
+        cos = x @ y / max(np.linalg.norm(x) * np.linalg.norm(y), 1e-30)
+        assert e < 5e-2 and cos > 0.99, (name, e, cos)
+
+
+def test_train_step_lowers_loss(ctx):
+    rng = np.random.default_rng(14)
+    m = vnm.gen_rmat(11, 9000, 13)
+    g = vnm.Graph(ctx, m, M)
+    lm = vnm.LPModel(ctx, seed=3)
+    pairs, _ = _pairs(rng, m.nrows, 2048)
+    tg = rng.normal(0, 1, 2048).astype(np.float32)
+    l0 = lm.train_step(g, N, pairs, tg, 3e-3)
+    for _ in range(30):
+        l1 = lm.train_step(g, N, pairs, tg, 3e-3)
+    assert l1 < 0.8 * l0, (l0, l1)
+
+
+def test_trace_round_cap_matches_oracle(ctx):
+    m = vnm.gen_er(2048, 41943, 1)
+    for sched, cap in ((0, 1), (1, 2), (2, 3)):
+        g = vnm.Graph(ctx, m, M)
+        st, tr = g.nm_trace(N, sched, 10, round_cap=cap)
+        om = _omat(m)
+        ost = oracle.nm_reorder_sample(om, N, M, sched, 10, cap)
+        assert st[4] == ost[4] == cap
+        assert st == list(ost)
+        full = oracle.nm_reorder(_omat(m), N, M, sched, 10, trace_cap=100000)[1]
+        ref = full[full[:, 0] <= cap]
+        assert np.array_equal(tr, ref)
+        out = g.download()
+        assert np.array_equal(out.row, om.row) and np.array_equal(out.col, om.col)
+
+
+def _recount(coo):
+    cnt, _ = oracle.locate(_omat(coo), N, M)
+    return int(cnt.sum()), int((cnt > 0).sum())
+
+
+@pytest.mark.parametrize("mode", [vnm.LP_LEARNED, vnm.LP_RANDOM, vnm.LP_EXHAUSTIVE])
+def test_lp_reorder_invariants(ctx, mode):
+    m = vnm.gen_rmat(12, 30000, 21)
+    lm = vnm.LPModel(ctx, seed=4)
+    outs = []
+    for _ in range(2):
+        g = vnm.Graph(ctx, m, M)
+        st = g.lp_reorder(lm if mode == vnm.LP_LEARNED else None, N, mode=mode, topk=256, window=8,
+                          maxiter=6, seed=9)
+        out = g.download()
+        perm = g.perm()
+        assert np.array_equal(np.sort(perm), np.arange(m.nrows))
+        assert (st.init_cnt, st.init_size) == _recount(m)
+        assert (st.final_cnt, st.final_size) == _recount(out)
+        assert st.final_cnt == st.init_cnt - st.gain
+        assert st.final_cnt <= st.init_cnt
+        # the output is the input relabelled by perm
+        assert np.array_equal(out.row, perm[m.row]) and np.array_equal(out.col, perm[m.col])
+        outs.append((out.row.copy(), st.as_dict()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    d0, d1 = dict(outs[0][1]), dict(outs[1][1])
+    d0.pop("ms"), d1.pop("ms")
+    assert d0 == d1

@@ -14,36 +14,6 @@
 
 namespace vnm {
 
-template <typename T>
-int DBuf<T>::alloc(size_t count) {
-    release();
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
-    if (e != hipSuccess) {
-        p = nullptr;
-        (void)hipGetLastError();
-        return set_error(e == hipErrorOutOfMemory ? VNM_E_OOM : VNM_E_HIP,
-                         "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
-    }
-    n = count;
-    return VNM_OK;
-}
-
-template <typename T>
-void DBuf<T>::release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-}
-
-template struct DBuf<int32_t>;
-template struct DBuf<uint32_t>;
-template struct DBuf<float>;
-template struct DBuf<int64_t>;
-template struct DBuf<uint64_t>;
-template struct DBuf<uint8_t>;
-template struct DBuf<uint16_t>;
-template struct DBuf<unsigned long long>;
 
 __global__ void k_iota(int32_t *p, int32_t n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -32,8 +32,27 @@ template <typename T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
-    int alloc(size_t count);
-    void release();
+    int alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return set_error(e == hipErrorOutOfMemory ? VNM_E_OOM : VNM_E_HIP,
+                             "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+        }
+        n = count;
+        return VNM_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
 };
 
@@ -56,7 +75,15 @@ struct vnm_graph {
     vnm::DBuf<int32_t> perm, inv, cnt, nzr, ws;
     vnm::DBuf<uint32_t> tab;              // per-slot row bitmaps (bytes in words)
     size_t tab_slots = 0;
+    // bumped whenever the COO is relabelled or re-sorted; caches built from
+    // the COO (the proposer's adjacency) are valid for one epoch
+    int64_t epoch = 0;
+    void *lp_cache = nullptr;
+    void (*lp_cache_free)(void *) = nullptr;
     int32_t nseg() const { return (nrows + M - 1) / M; }
+    ~vnm_graph() {
+        if (lp_cache_free) lp_cache_free(lp_cache);
+    }
 };
 
 namespace vnm {
@@ -66,6 +93,8 @@ int build_csc(vnm_graph *g);
 int locate_all(vnm_graph *g, int N, int32_t *d_cnt, int32_t *d_nzr);
 int relabel(vnm_graph *g, const int32_t *d_perm);
 int nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *st);
+int nm_reorder_ex(vnm_graph *g, int N, int sched, int maxiter, int round_cap, int32_t *d_trace,
+                  int64_t trace_cap, int64_t *trace_len, vnm_nm_stats *stats);
 int venom_eval(vnm_graph *g, int V, int32_t *inv, int32_t *inv_strict);
 int venom_reorder(vnm_graph *g, int V, int maxiter, vnm_venom_stats *st);
 int reload_sort(vnm_graph *g, bool quantize);

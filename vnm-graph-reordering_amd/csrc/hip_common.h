@@ -2,6 +2,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cstring>
 #include <stdint.h>
 
 #include "vnm_internal.h"
@@ -104,6 +106,22 @@ __device__ __forceinline__ int heap_pop(HeapRef &H) {
     }
     H.n = n - 1;
     return top_el;
+}
+
+// BF16 storage codec: round to nearest even, NaN kept quiet.
+__host__ __device__ inline uint16_t f32_to_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float bf16_to_f32(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
 }
 
 __device__ __forceinline__ int popc32(uint32_t x) { return __builtin_popcount(x); }

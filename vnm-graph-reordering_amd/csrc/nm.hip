@@ -30,6 +30,7 @@
 //   k_relabel     : the Reorder kernel (reorder/reorder.h:23-35), applied once
 #include "graph.h"
 #include "hip_common.h"
+#include "tables.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -124,77 +125,6 @@ __global__ __launch_bounds__(1024) void k_scan_totals(int32_t *tot, int ntiles) 
 __global__ void k_scan_add(int32_t *p, int64_t n, const int32_t *tile_tot) {
     int64_t i = (int64_t)blockIdx.x * SCAN_TILE + threadIdx.x;
     if (blockIdx.x > 0 && i < n) p[i] += tile_tot[blockIdx.x - 1];
-}
-
-// ------------------------------------------------------------ tables
-
-// Column list of slot j, bit b: run-start column inv[seg*M + b].
-struct ColSpan { int start, len; };
-
-// Fill per-column spans for K segments (K*M columns) and their prefix sums.
-// Must be called by all threads; returns the total entry count.
-__device__ int table_columns(const int *sel, int K, int M, const int32_t *inv,
-                             const int32_t *cptr, ColSpan *cols, int *pre) {
-    const int nc = K * M;
-    for (int t = threadIdx.x; t < nc; t += blockDim.x) {
-        int j = t / M, b = t - j * M;
-        int c0 = ld_l2(&inv[sel[j] * M + b]);
-        int s = cptr[c0];
-        cols[t].start = s;
-        cols[t].len = cptr[c0 + 1] - s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < nc; ++t) {
-            pre[t] = acc;
-            acc += cols[t].len;
-        }
-        pre[nc] = acc;
-    }
-    __syncthreads();
-    return pre[nc];
-}
-
-// Locate the column of entry e (0 <= e < total) by binary search on pre.
-__device__ __forceinline__ int entry_col(const int *pre, int nc, int e) {
-    int lo = 0, hi = nc - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= e) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ uint32_t *slot_word(uint32_t *tab, int j, int R, int r) {
-    return tab + (size_t)j * (size_t)(R >> 2) + (r >> 2);
-}
-
-__device__ __forceinline__ uint32_t slot_byte(uint32_t *tab, int j, int R, int r) {
-    return (ld_l2(slot_word(tab, j, R, r)) >> ((r & 3) * 8)) & 0xFFu;
-}
-
-__device__ void table_scatter(uint32_t *tab, const int *slot_of_col, int M, int R,
-                              const ColSpan *cols, const int *pre, int nc, int total,
-                              const int32_t *crow) {
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        int t = entry_col(pre, nc, e);
-        int j = slot_of_col ? slot_of_col[t] : t / M;
-        int b = t % M;
-        int r = crow[cols[t].start + (e - pre[t])];
-        atomicOr(slot_word(tab, j, R, r), (1u << (M - 1 - b)) << ((r & 3) * 8));
-    }
-}
-
-__device__ void table_clear(uint32_t *tab, const int *slot_of_col, int M, int R,
-                            const ColSpan *cols, const int *pre, int nc, int total,
-                            const int32_t *crow) {
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        int t = entry_col(pre, nc, e);
-        int j = slot_of_col ? slot_of_col[t] : t / M;
-        int r = crow[cols[t].start + (e - pre[t])];
-        st_l2(slot_word(tab, j, R, r), 0u);
-    }
 }
 
 // ------------------------------------------------------------ locate (all segments)
@@ -372,6 +302,7 @@ struct NMArgs {
     int32_t *out;        // stats[5], status, work, nswaps
     int32_t *trace;      // optional: 9 ints per swap
     int64_t trace_cap;
+    int round_cap;       // stop after this many rounds (training traces; INT_MAX = none)
 };
 
 struct GreedyShared {
@@ -802,7 +733,8 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
     __syncthreads();
 
     while (true) {
-        if (tid == 0) S.cont = (S.round <= S.maxiter && S.L >= min_size && S.status == 0);
+        if (tid == 0)
+            S.cont = (S.round <= S.maxiter && S.round <= A.round_cap && S.L >= min_size && S.status == 0);
         __syncthreads();
         if (!S.cont) break;
 
@@ -1056,6 +988,7 @@ int locate_all(vnm_graph *g, int N, int32_t *d_cnt, int32_t *d_nzr) {
 
 int relabel(vnm_graph *g, const int32_t *d_perm) {
     hipStream_t st = g->ctx->stream;
+    g->epoch++;
     hipLaunchKernelGGL(k_relabel, dim3(grid_for(g->nnz, 256)), dim3(256), 0, st, g->row.p,
                        g->col.p, g->nnz, d_perm, g->nrows);
     hipLaunchKernelGGL(k_compose, dim3(grid_for(g->nrows, 256)), dim3(256), 0, st,
@@ -1074,7 +1007,8 @@ static int check_nm_args(vnm_graph *g, int N) {
     return VNM_OK;
 }
 
-int nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *stats) {
+int nm_reorder_ex(vnm_graph *g, int N, int sched, int maxiter, int round_cap, int32_t *d_trace,
+                  int64_t trace_cap, int64_t *trace_len, vnm_nm_stats *stats) {
     int rc = check_nm_args(g, N);
     if (rc) return rc;
     if (sched < 0 || (sched >= 4 && sched <= 7))
@@ -1087,7 +1021,8 @@ int nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *stats)
     A.R = g->nrows; A.M = g->M; A.N = N; A.nseg = g->nseg(); A.sched = sched; A.maxiter = maxiter;
     A.cptr = g->cptr.p; A.crow = g->crow.p; A.perm = g->perm.p; A.inv = g->inv.p;
     A.cnt = g->cnt.p; A.nzr = g->nzr.p; A.tab = g->tab.p; A.out = out;
-    A.trace = nullptr; A.trace_cap = 0;
+    A.trace = d_trace; A.trace_cap = d_trace ? trace_cap : 0;
+    A.round_cap = round_cap > 0 ? round_cap : 0x7FFFFFFF;
     KernelTimer kt(g->ctx, "k_nm_greedy");
     hipLaunchKernelGGL(k_nm_greedy, dim3(1), dim3(GT), 0, st, A);
     kt.stop();
@@ -1103,12 +1038,17 @@ int nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *stats)
         return set_error(VNM_E_UB_PATH, "greedy reached top() of an empty swap queue "
                                         "(undefined in the reference)");
     if ((rc = relabel(g, g->perm.p))) return rc;
+    if (trace_len) *trace_len = h[7];
     stats->init_size = h[0];
     stats->final_size = h[1];
     stats->init_cnt = h[2];
     stats->final_cnt = h[3];
     stats->rounds = h[4];
     return VNM_OK;
+}
+
+int nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *stats) {
+    return nm_reorder_ex(g, N, sched, maxiter, 0, nullptr, 0, nullptr, stats);
 }
 
 }  // namespace vnm
@@ -1197,6 +1137,24 @@ int vnm_pq_pop_order(vnm_ctx *ctx, const int32_t *scores, int count, int M, int3
 int vnm_nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *st) {
     if (!g || !st) return set_error(VNM_E_ARG, "bad arguments");
     return nm_reorder(g, N, sched, maxiter, st);
+}
+
+int vnm_nm_trace(vnm_graph *g, int N, int sched, int maxiter, int round_cap, int32_t *rec,
+                 int64_t cap, int64_t *count, vnm_nm_stats *st) {
+    if (!g || !st || cap < 0 || (cap > 0 && !rec)) return set_error(VNM_E_ARG, "bad arguments");
+    if (cap > ((int64_t)1 << 31) / 9) return set_error(VNM_E_ARG, "trace capacity too large");
+    DBuf<int32_t> d;
+    int rc;
+    if ((rc = d.alloc((size_t)std::max<int64_t>(cap, 1) * 9))) return rc;
+    int64_t n = 0;
+    if ((rc = nm_reorder_ex(g, N, sched, maxiter, round_cap, d.p, cap, &n, st))) return rc;
+    int64_t keep = std::min(n, cap);
+    if (keep > 0) {
+        VNM_HIP_TRY(hipMemcpyAsync(rec, d.p, (size_t)keep * 9 * 4, hipMemcpyDeviceToHost, g->ctx->stream));
+        VNM_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+    }
+    if (count) *count = n;
+    return VNM_OK;
 }
 
 }  // extern "C"

@@ -71,21 +71,6 @@ static int bits_for(int64_t x) {
 
 // ------------------------------------------------------------ value codecs
 
-__host__ __device__ inline uint16_t f32_to_bf16(float f) {
-    uint32_t u;
-    memcpy(&u, &f, 4);
-    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return (uint16_t)((u >> 16) | 0x40);
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-    return (uint16_t)(u >> 16);
-}
-
-__host__ __device__ inline float bf16_to_f32(uint16_t h) {
-    uint32_t u = (uint32_t)h << 16;
-    float f;
-    memcpy(&f, &u, 4);
-    return f;
-}
-
 // OCP FP8 E4M3 (bias 7, no infinities, 0x7F = NaN, max 448): round to
 // nearest even, saturating to +-448.
 __host__ __device__ inline uint8_t f32_to_e4m3(float f) {

@@ -383,6 +383,7 @@ static void quantize_values(std::vector<float> &vals) {
 int reload_sort(vnm_graph *g, bool quantize) {
     hipStream_t st = g->ctx->stream;
     const int64_t nnz = g->nnz;
+    g->epoch++;
     // mmwrite(shape=(n, n)) with n the padded column count (reordering.py:180)
     const int M = g->M;
     const int n = ((g->ncols + M - 1) / M) * M;
@@ -448,6 +449,7 @@ static int check_venom_args(vnm_graph *g, int V) {
 }
 
 int venom_reorder(vnm_graph *g, int V, int maxiter, vnm_venom_stats *st) {
+    g->epoch++;
     VenomWork w;
     int rc = w.init(g, V);
     if (rc) return rc;

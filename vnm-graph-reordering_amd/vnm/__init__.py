@@ -13,6 +13,7 @@ no CPU fallback.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -32,7 +33,13 @@ EXPORTS = [
     "vnm_locate", "vnm_swap_score", "vnm_pq_pop_order", "vnm_nm_reorder",
     "vnm_venom_eval", "vnm_venom_reorder", "vnm_hybrid", "vnm_find_best_sched",
     "vnm_retained_nnz", "vnm_pack", "vnm_packed_info_get", "vnm_packed_destroy", "vnm_spmm",
+    "vnm_nm_trace", "vnm_lp_model_create", "vnm_lp_model_destroy", "vnm_lp_model_get",
+    "vnm_lp_features", "vnm_lp_forward", "vnm_lp_loss_grad", "vnm_lp_train_step",
+    "vnm_pair_score", "vnm_lp_reorder",
 ]
+
+LP_NPARAMS = 22784
+LP_LEARNED, LP_RANDOM, LP_EXHAUSTIVE = 0, 1, 2
 
 
 class VnmError(RuntimeError):
@@ -83,6 +90,15 @@ class HybridStats(ctypes.Structure):
                           f(t.init_cnt), f(t.final_cnt), f(t.improve_rate), str(t.init_mb),
                           str(t.final_mb), str(t.init_vnm), str(t.final_vnm), str(t.nm_rounds),
                           str(t.vnm_rounds), str(t.total_rounds), f(t.milliseconds)])
+
+
+class LPStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("init_cnt", "final_cnt", "init_size", "final_size", "iters", "swaps",
+                 "fallbacks", "gain")] + [("exact_pairs", ctypes.c_int64), ("ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class PackedInfo(ctypes.Structure):
@@ -147,6 +163,22 @@ def lib():
     L.vnm_packed_destroy.argtypes = [vp]
     L.vnm_packed_destroy.restype = None
     L.vnm_spmm.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int]
+    f32p = P(ctypes.c_float)
+    L.vnm_nm_trace.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p,
+                               ctypes.c_int64, P(ctypes.c_int64), P(NMStats)]
+    L.vnm_lp_model_create.argtypes = [vp, f32p, ctypes.c_uint64, P(vp)]
+    L.vnm_lp_model_destroy.argtypes = [vp]
+    L.vnm_lp_model_destroy.restype = None
+    L.vnm_lp_model_get.argtypes = [vp, f32p]
+    L.vnm_lp_features.argtypes = [vp, ctypes.c_int, f32p]
+    L.vnm_lp_forward.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, f32p]
+    L.vnm_lp_loss_grad.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, i32p, f32p, ctypes.c_int,
+                                   f32p, f32p]
+    L.vnm_lp_train_step.argtypes = [vp, vp, ctypes.c_int, i32p, f32p, ctypes.c_int, ctypes.c_float,
+                                    f32p]
+    L.vnm_pair_score.argtypes = [vp, ctypes.c_int, i32p, i32p, ctypes.c_int, i32p]
+    L.vnm_lp_reorder.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_uint64, P(LPStats)]
     _lib = L
     return L
 
@@ -241,10 +273,28 @@ def gen_normal(seed, shape):
 # ------------------------------------------------------------------ device
 
 class Context:
+    """A device context.  close() waits for the graphs, packed matrices and
+    models created on it: the library handle is destroyed with the last."""
+
     def __init__(self, device=0):
+        self.h = None
+        self._children = weakref.WeakSet()
+        self._closing = False
         h = ctypes.c_void_p()
         _check(lib().vnm_ctx_create(device, ctypes.byref(h)))
         self.h = h
+
+    def _adopt(self, obj):
+        self._children.add(obj)
+
+    def _release(self, obj):
+        self._children.discard(obj)
+        self._maybe_destroy()
+
+    def _maybe_destroy(self):
+        if self.h and self._closing and len(self._children) == 0:
+            lib().vnm_ctx_destroy(self.h)
+            self.h = None
 
     @property
     def stream(self):
@@ -255,13 +305,14 @@ class Context:
         return lib().vnm_ctx_launches(self.h)
 
     def close(self):
-        if self.h:
-            lib().vnm_ctx_destroy(self.h)
-            self.h = None
+        self._closing = True
+        self._maybe_destroy()
 
     def __del__(self):
         try:
-            self.close()
+            self._closing = True
+            self._children = weakref.WeakSet()
+            self._maybe_destroy()
         except Exception:
             pass
 
@@ -291,12 +342,14 @@ class Graph:
         h = ctypes.c_void_p()
         _check(lib().vnm_graph_create(ctx.h, ctypes.byref(coo.c()), M, ctypes.byref(h)))
         self.h = h
+        ctx._adopt(self)
         self.nrows, self.ncols = coo.nrows, coo.ncols
 
     def close(self):
         if self.h:
             lib().vnm_graph_destroy(self.h)
             self.h = None
+            self.ctx._release(self)
 
     def __del__(self):
         try:
@@ -336,6 +389,38 @@ class Graph:
         _check(lib().vnm_nm_reorder(self.h, N, sched, maxiter, ctypes.byref(st)))
         return st.as_list()
 
+    def nm_trace(self, N, sched, maxiter=10, round_cap=0, cap=1 << 20):
+        """nm_reorder with a swap trace (9 ints per swap: round, primary,
+        target, v1, v2, lgain, rgain, primary_cnt, target_cnt), optionally
+        stopped after round_cap rounds.  Returns (stats, trace)."""
+        st = NMStats()
+        rec = np.zeros((max(cap, 1), 9), np.int32)
+        n = ctypes.c_int64()
+        _check(lib().vnm_nm_trace(self.h, N, sched, maxiter, round_cap, _p(rec, ctypes.c_int32), cap,
+                                  ctypes.byref(n), ctypes.byref(st)))
+        return st.as_list(), rec[:min(n.value, cap)].copy()
+
+    def pair_score(self, N, segp, segq):
+        """Exact gain matrices of segment pairs from their entries (same
+        layout as swap_score: one row of 2*M*M ints per pair)."""
+        segp = np.ascontiguousarray(np.atleast_1d(segp), np.int32)
+        segq = np.ascontiguousarray(np.atleast_1d(segq), np.int32)
+        out = np.zeros((segp.shape[0], 2 * self.M * self.M), np.int32)
+        _check(lib().vnm_pair_score(self.h, N, _p(segp, ctypes.c_int32), _p(segq, ctypes.c_int32),
+                                    segp.shape[0], _p(out, ctypes.c_int32)))
+        return out
+
+    def lp_features(self, N):
+        X = np.zeros((self.nrows, 16), np.float32)
+        _check(lib().vnm_lp_features(self.h, N, _p(X, ctypes.c_float)))
+        return X
+
+    def lp_reorder(self, model, N, mode=LP_LEARNED, topk=4096, window=16, maxiter=20, seed=1):
+        st = LPStats()
+        _check(lib().vnm_lp_reorder(self.h, model.h if model is not None else None, N, mode, topk,
+                                    window, maxiter, seed, ctypes.byref(st)))
+        return st
+
     def venom_eval(self, V):
         a, b = ctypes.c_int32(), ctypes.c_int32()
         _check(lib().vnm_venom_eval(self.h, V, ctypes.byref(a), ctypes.byref(b)))
@@ -365,12 +450,77 @@ class Graph:
         return Packed(self, V, dtype)
 
 
+class LPModel:
+    """The learned pair proposer (parameters on the device, Adam state)."""
+
+    def __init__(self, ctx, params=None, seed=1):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        pp = None
+        if params is not None:
+            self._init = np.ascontiguousarray(params, np.float32)
+            assert self._init.shape == (LP_NPARAMS,)
+            pp = _p(self._init, ctypes.c_float)
+        _check(lib().vnm_lp_model_create(ctx.h, pp, seed, ctypes.byref(h)))
+        self.h = h
+        ctx._adopt(self)
+
+    @classmethod
+    def load(cls, ctx, path):
+        return cls(ctx, np.fromfile(path, dtype="<f4"))
+
+    def save(self, path):
+        self.params().astype("<f4").tofile(path)
+
+    def params(self):
+        out = np.zeros(LP_NPARAMS, np.float32)
+        _check(lib().vnm_lp_model_get(self.h, _p(out, ctypes.c_float)))
+        return out
+
+    def forward(self, graph, N, bf16=True):
+        Y = np.zeros((graph.nrows, 64), np.float32)
+        _check(lib().vnm_lp_forward(self.h, graph.h, N, int(bf16), _p(Y, ctypes.c_float)))
+        return Y
+
+    def loss_grad(self, graph, N, pairs, target, bf16=True, want_grad=True):
+        pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+        target = np.ascontiguousarray(target, np.float32)
+        loss = ctypes.c_float()
+        grad = np.zeros(LP_NPARAMS, np.float32) if want_grad else None
+        _check(lib().vnm_lp_loss_grad(self.h, graph.h, N, int(bf16), _p(pairs, ctypes.c_int32),
+                                      _p(target, ctypes.c_float), target.shape[0], ctypes.byref(loss),
+                                      _p(grad, ctypes.c_float) if want_grad else None))
+        return loss.value, grad
+
+    def train_step(self, graph, N, pairs, target, lr):
+        pairs = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+        target = np.ascontiguousarray(target, np.float32)
+        loss = ctypes.c_float()
+        _check(lib().vnm_lp_train_step(self.h, graph.h, N, _p(pairs, ctypes.c_int32),
+                                       _p(target, ctypes.c_float), target.shape[0], lr,
+                                       ctypes.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self.h:
+            lib().vnm_lp_model_destroy(self.h)
+            self.h = None
+            self.ctx._release(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Packed:
     def __init__(self, graph, V, dtype):
         h = ctypes.c_void_p()
         _check(lib().vnm_pack(graph.h, V, dtype, ctypes.byref(h)))
         self.h = h
         self.graph = graph
+        graph.ctx._adopt(self)
 
     def info(self):
         i = PackedInfo()
@@ -384,6 +534,7 @@ class Packed:
         if self.h:
             lib().vnm_packed_destroy(self.h)
             self.h = None
+            self.graph.ctx._release(self)
 
     def __del__(self):
         try:
