@@ -14,7 +14,15 @@ counts, BF16 and FP8 SpMM rates and errors against the FP64 oracle, the
 roofline of the SpMM kernel (HBM bound), and the CPU oracle timed on this
 host (cpu_baseline; single thread).
 
+Also reported (section learned_vs_exact): the reference's exact N:M greedy
+(GetSegSwapScore + SwapPairList, schedules 0 and 1) against the learned
+proposer reorder on the same input -- GPU time, invalid rows left, retained
+V:N:M nnz, and the time the learned reorder needs to reach the exact one's
+retained nnz.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--rounds R] [--no-cpu]
+    python bench.py --workload rmat262144   # the learned-vs-exact comparison on
+                                            # R-MAT 262,144 nodes, degree 32 (one line)
 """
 import argparse
 import json
@@ -38,7 +46,77 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a streaming 96 MiB add (counter calibration)")
+    ap.add_argument("--workload", default="rmat65536", choices=["rmat65536", "rmat262144"])
     return ap.parse_args()
+
+
+WEIGHTS = os.path.join(ROOT, "vnm-graph-reordering_amd", "weights", "lp_rmat_deg16.f32")
+
+
+def learned_vs_exact(vnm, ctx, coo, N, M, V, exact_scheds, topk=4096, window=16, maxiter=40):
+    """Exact greedy (per schedule) vs the learned proposer on copies of coo."""
+    import torch
+    out = {"exact": {}, "learned": {}}
+    for s in exact_scheds:
+        g = vnm.Graph(ctx, coo, M)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = g.nm_reorder(N, s, 10)
+        ms = (time.perf_counter() - t) * 1e3
+        out["exact"][f"sched{s}"] = {"ms": round(ms, 2), "init_cnt": st[2], "final_cnt": st[3],
+                                     "rounds": st[4], "retained_nnz": g.retained_nnz(V)}
+        g.close()
+    best = max(out["exact"].values(), key=lambda e: (e["retained_nnz"], -e["ms"]))
+    model = vnm.LPModel.load(ctx, WEIGHTS)
+    g = vnm.Graph(ctx, coo, M)  # warm-up (adjacency build, kernel load)
+    g.lp_reorder(model, N, topk=topk, window=window, maxiter=1)
+    g.close()
+    g = vnm.Graph(ctx, coo, M)
+    st = g.lp_reorder(model, N, topk=topk, window=window, maxiter=maxiter)
+    full = {"ms": round(st.ms, 2), "init_cnt": st.init_cnt, "final_cnt": st.final_cnt,
+            "iters": st.iters, "swaps": st.swaps, "exact_pairs": st.exact_pairs,
+            "fallbacks": st.fallbacks, "retained_nnz": g.retained_nnz(V)}
+    g.close()
+    out["learned"] = full
+    # time to reach the best exact schedule's retained nnz: the shortest
+    # learned run (iteration cap k) whose output retains at least as much
+    t_eq, k_eq = None, None
+    for k in range(1, st.iters + 1):
+        g = vnm.Graph(ctx, coo, M)
+        sk = g.lp_reorder(model, N, topk=topk, window=window, maxiter=k)
+        r = g.retained_nnz(V)
+        g.close()
+        if r >= best["retained_nnz"]:
+            t_eq, k_eq = sk.ms, k
+            break
+    out["learned_ms_to_exact_retained"] = round(t_eq, 2) if t_eq is not None else None
+    out["learned_iters_to_exact_retained"] = k_eq
+    out["best_exact"] = [k for k, v in out["exact"].items() if v is best][0]
+    out["speedup_vs_best_exact"] = round(best["ms"] / t_eq, 2) if t_eq else None
+    out["learned_retained_over_best_exact"] = round(full["retained_nnz"] / best["retained_nnz"], 5)
+    return out
+
+
+def bench_learned_262k(a):
+    import torch
+    import vnm
+    N, M, V = 2, 8, 128
+    coo = vnm.gen_rmat(18, 4194304, 6, M=M)
+    ctx = vnm.Context(0)
+    torch.cuda.synchronize()
+    lv = learned_vs_exact(vnm, ctx, coo, N, M, V, exact_scheds=(0, 1, 2, 3))
+    res = {
+        "metric": "reorder time to equal retained nnz (learned vs exact)",
+        "value": lv["speedup_vs_best_exact"], "unit": "x (exact GPU time / learned GPU time)",
+        "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": lv["learned_ms_to_exact_retained"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32 (gains), bf16 (network)",
+        "data": "synthetic (seeded R-MAT)",
+        "config": {"workload": "rmat262144_deg32_V128_N2_M8", "nodes": coo.nrows, "nnz": coo.nnz,
+                   "V": V, "N": N, "M": M, "topk": 4096, "window": 16, "parallelism": "none (1 GPU)"},
+        "learned_vs_exact": lv,
+    }
+    print(json.dumps(res), flush=True)
 
 
 def main():
@@ -47,6 +125,9 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
               f"(--gpus {a.gpus} requested); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "rmat262144":
+        bench_learned_262k(a)
+        return
     import numpy as np
     import torch
     import vnm
@@ -163,6 +244,9 @@ def main():
                      "fp8_tflops": round(flops / (ms_8 * 1e-3) / 1e12, 4)},
         "gen_s": round(t_gen, 2),
     }
+
+    res["learned_vs_exact"] = learned_vs_exact(vnm, ctx, coo, N, M, V, exact_scheds=(0, 1))
+    print(f"[bench] learned vs exact done", file=sys.stderr, flush=True)
 
     if not a.no_cpu:
         import oracle

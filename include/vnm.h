@@ -102,6 +102,8 @@ void vnm_graph_destroy(vnm_graph *g);
 /* Current COO, in the order the reference writes it (mtx/Mtx.h:429-431):
  * the order at the last (re)load, ids remapped.  Arrays library-allocated. */
 int  vnm_graph_download(vnm_graph *g, vnm_coo *out);
+/* A device copy of g (current COO and composed relabel) on the same context. */
+int  vnm_graph_clone(vnm_graph *g, vnm_graph **out);
 /* Map from the ids at vnm_graph_create to current ids (int32[nrows]). */
 int  vnm_graph_perm(vnm_graph *g, int32_t *perm);
 
@@ -269,6 +271,12 @@ typedef struct {
  * vnm_nm_reorder.  m may be NULL for modes 1 and 2. */
 int vnm_lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window,
                    int maxiter, uint64_t seed, vnm_lp_stats *st);
+
+/* hybrid_reordering (main.cu:195-337) with vnm_lp_reorder (mode, topk,
+ * window, maxiter) as its N:M step; VENOM steps and stats as vnm_hybrid
+ * (nm_rounds counts proposer iterations). */
+int vnm_hybrid_learned(vnm_graph *g, const vnm_lp_model *m, int N, int V, int rounds, int mode,
+                       int topk, int window, int maxiter, vnm_hybrid_stats *st);
 
 #ifdef __cplusplus
 }

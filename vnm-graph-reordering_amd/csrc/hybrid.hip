@@ -8,10 +8,13 @@
 
 #include <chrono>
 #include <cstring>
+#include <functional>
 
 namespace vnm {
 
-int hybrid(vnm_graph *g, int N, int V, int rounds, int sched, vnm_hybrid_stats *st) {
+using NMStep = std::function<int(vnm_graph *, vnm_nm_stats *)>;
+
+static int hybrid_with(vnm_graph *g, int V, int rounds, const NMStep &nm_step, vnm_hybrid_stats *st) {
     memset(st, 0, sizeof *st);
     if (g->nnz == 0) return VNM_OK;  // main.cu:204 prints an empty line
     st->nrows = (float)g->nrows;
@@ -27,7 +30,7 @@ int hybrid(vnm_graph *g, int N, int V, int rounds, int sched, vnm_hybrid_stats *
     VNM_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
     auto t0 = std::chrono::steady_clock::now();
     vnm_nm_stats first, last;
-    if ((rc = nm_reorder(g, N, sched, 10, &first))) return rc;
+    if ((rc = nm_step(g, &first))) return rc;
     last = first;
     int vnm_round = 0, nm_round = 0;
     for (int i = 0; i < rounds; ++i) {
@@ -35,7 +38,7 @@ int hybrid(vnm_graph *g, int N, int V, int rounds, int sched, vnm_hybrid_stats *
         if ((rc = venom_reorder(g, V, 10, &vs))) return rc;  // reordering.py --m M --v V
         vnm_round += vs.rounds;
         if ((rc = reload_sort(g, true))) return rc;         // new Mtx("temp.mtx")
-        if ((rc = nm_reorder(g, N, sched, 10, &last))) return rc;
+        if ((rc = nm_step(g, &last))) return rc;
         nm_round += last.rounds;
     }
     VNM_HIP_TRY(hipStreamSynchronize(g->ctx->stream));
@@ -58,6 +61,31 @@ int hybrid(vnm_graph *g, int N, int V, int rounds, int sched, vnm_hybrid_stats *
     st->total_rounds = nm_round + vnm_round;
     st->milliseconds = std::chrono::duration<float, std::milli>(t1 - t0).count();
     return VNM_OK;
+}
+
+int hybrid(vnm_graph *g, int N, int V, int rounds, int sched, vnm_hybrid_stats *st) {
+    return hybrid_with(g, V, rounds,
+                       [=](vnm_graph *h, vnm_nm_stats *s) { return nm_reorder(h, N, sched, 10, s); }, st);
+}
+
+// The hybrid loop with the learned N:M reorder as its N:M step (an addition
+// of this library; the VENOM steps and the stats are the reference's).
+int hybrid_learned(vnm_graph *g, const vnm_lp_model *m, int N, int V, int rounds, int mode, int topk,
+                   int window, int maxiter, vnm_hybrid_stats *st) {
+    return hybrid_with(
+        g, V, rounds,
+        [=](vnm_graph *h, vnm_nm_stats *s) {
+            vnm_lp_stats ls;
+            int rc = lp_reorder(h, m, N, mode, topk, window, maxiter, 1, &ls);
+            if (rc) return rc;
+            s->init_size = ls.init_size;
+            s->final_size = ls.final_size;
+            s->init_cnt = ls.init_cnt;
+            s->final_cnt = ls.final_cnt;
+            s->rounds = ls.iters;
+            return VNM_OK;
+        },
+        st);
 }
 
 int find_best_sched(vnm_graph *g, int N, int nsched, int32_t *best) {
@@ -96,6 +124,19 @@ int vnm_hybrid(vnm_graph *g, int N, int V, int rounds, int sched, vnm_hybrid_sta
     if (V <= 0) return set_error(VNM_E_ARG, "V must be positive (got %d)", V);
     if (rounds < 0) return set_error(VNM_E_ARG, "rounds must be >= 0");
     return hybrid(g, N, V, rounds, sched, st);
+}
+
+int vnm_hybrid_learned(vnm_graph *g, const vnm_lp_model *m, int N, int V, int rounds, int mode, int topk,
+                       int window, int maxiter, vnm_hybrid_stats *st) {
+    if (!g || !st) return set_error(VNM_E_ARG, "null argument");
+    if (V <= 0) return set_error(VNM_E_ARG, "V must be positive (got %d)", V);
+    if (rounds < 0) return set_error(VNM_E_ARG, "rounds must be >= 0");
+    return hybrid_learned(g, m, N, V, rounds, mode, topk, window, maxiter, st);
+}
+
+int vnm_graph_clone(vnm_graph *g, vnm_graph **out) {
+    if (!g || !out) return set_error(VNM_E_ARG, "null argument");
+    return graph_clone(g, out);
 }
 
 int vnm_find_best_sched(vnm_graph *g, int N, int nsched, int32_t *best) {

@@ -1251,7 +1251,7 @@ static int lp_score_cands(vnm_graph *g, LPCache *c, int N, Cand *cand, int4 *bes
     return VNM_OK;
 }
 
-static int lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window, int maxiter,
+int lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window, int maxiter,
                       uint64_t seed, vnm_lp_stats *stats) {
     if (mode < 0 || mode > 2) return set_error(VNM_E_ARG, "mode must be 0 (learned), 1 (random) or 2 (exhaustive)");
     if (mode == 0 && !m) return set_error(VNM_E_ARG, "learned mode needs a model");

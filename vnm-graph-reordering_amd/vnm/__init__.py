@@ -35,7 +35,7 @@ EXPORTS = [
     "vnm_retained_nnz", "vnm_pack", "vnm_packed_info_get", "vnm_packed_destroy", "vnm_spmm",
     "vnm_nm_trace", "vnm_lp_model_create", "vnm_lp_model_destroy", "vnm_lp_model_get",
     "vnm_lp_features", "vnm_lp_forward", "vnm_lp_loss_grad", "vnm_lp_train_step",
-    "vnm_pair_score", "vnm_lp_reorder",
+    "vnm_pair_score", "vnm_lp_reorder", "vnm_hybrid_learned", "vnm_graph_clone",
 ]
 
 LP_NPARAMS = 22784
@@ -179,6 +179,9 @@ def lib():
     L.vnm_pair_score.argtypes = [vp, ctypes.c_int, i32p, i32p, ctypes.c_int, i32p]
     L.vnm_lp_reorder.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_uint64, P(LPStats)]
+    L.vnm_hybrid_learned.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, P(HybridStats)]
+    L.vnm_graph_clone.argtypes = [vp, P(vp)]
     _lib = L
     return L
 
@@ -351,6 +354,15 @@ class Graph:
             self.h = None
             self.ctx._release(self)
 
+    def clone(self):
+        """A device copy (current COO and composed relabel)."""
+        h = ctypes.c_void_p()
+        _check(lib().vnm_graph_clone(self.h, ctypes.byref(h)))
+        g = Graph.__new__(Graph)
+        g.ctx, g.M, g.h, g.nrows, g.ncols = self.ctx, self.M, h, self.nrows, self.ncols
+        self.ctx._adopt(g)
+        return g
+
     def __del__(self):
         try:
             self.close()
@@ -434,6 +446,12 @@ class Graph:
     def hybrid(self, N, V, rounds, sched):
         st = HybridStats()
         _check(lib().vnm_hybrid(self.h, N, V, rounds, sched, ctypes.byref(st)))
+        return st
+
+    def hybrid_learned(self, model, N, V, rounds, mode=LP_LEARNED, topk=4096, window=16, maxiter=20):
+        st = HybridStats()
+        _check(lib().vnm_hybrid_learned(self.h, model.h if model is not None else None, N, V, rounds,
+                                        mode, topk, window, maxiter, ctypes.byref(st)))
         return st
 
     def find_best_sched(self, N, nsched=4):
