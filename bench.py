@@ -23,6 +23,8 @@ retained nnz.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--rounds R] [--no-cpu]
     python bench.py --workload rmat262144   # the learned-vs-exact comparison on
                                             # R-MAT 262,144 nodes, degree 32 (one line)
+    python bench.py --workload rmat131072   # precision workload: R-MAT 131,072 nodes,
+                                            # degree 24, 64:2:4, BF16 / FP8 SpMM, n = 128
 """
 import argparse
 import json
@@ -46,7 +48,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a streaming 96 MiB add (counter calibration)")
-    ap.add_argument("--workload", default="rmat65536", choices=["rmat65536", "rmat262144"])
+    ap.add_argument("--workload", default="rmat65536",
+                    choices=["rmat65536", "rmat262144", "rmat131072"])
     return ap.parse_args()
 
 
@@ -97,6 +100,104 @@ def learned_vs_exact(vnm, ctx, coo, N, M, V, exact_scheds, topk=4096, window=16,
     return out
 
 
+def timed_spmm(ctx, packed, Bt, bdt, C, n, steps, warmup, scales=None):
+    """(wall ms per call, k_spmm ms per launch) over `steps` calls."""
+    import torch
+    sp = scales.data_ptr() if scales is not None else None
+    for _ in range(warmup):
+        packed.spmm(Bt.data_ptr(), n, n, bdt, C.data_ptr(), n, sp)
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        packed.spmm(Bt.data_ptr(), n, n, bdt, C.data_ptr(), n, sp)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t) / steps
+    kms, kn = ctx.timer("k_spmm")
+    return wall * 1e3, kms / max(kn, 1)
+
+
+def fp8_rows(Bd, n):
+    """OCP e4m3 copy of B with one FP32 scale per 128 rows (absmax / 448)."""
+    import torch
+    Bb = Bd.view(-1, 128, n)
+    bsc = (Bb.abs().amax(dim=(1, 2)) / 448.0).clamp_min(1e-30).float().contiguous()
+    return (Bb / bsc[:, None, None]).to(torch.float8_e4m3fn).view(-1, n).contiguous(), bsc
+
+
+def bench_precision_131k(a):
+    """R-MAT 131,072 nodes, degree 24 (nnz 3,145,728, seed 4), V:N:M = 64:2:4;
+    reorder with the learned proposer (exact gains judge every swap), then
+    C = A_vnm B with B 131,072 x 128 N(0,1) (seed 5): A and B in BF16 (the
+    headline) and in FP8-e4m3 with per-block / per-128-row scales, FP32
+    accumulate and output; errors against the FP64 oracle of the same pruned
+    matrix."""
+    import numpy as np
+    import torch
+    import vnm
+    N, M, V, NC = 2, 4, 64, 128
+    coo = vnm.gen_rmat(17, 1572864, 4, M=M)
+    B = vnm.gen_normal(5, (coo.ncols, NC))
+    ctx = vnm.Context(0)
+    model = vnm.LPModel.load(ctx, WEIGHTS)
+    g = vnm.Graph(ctx, coo, M)
+    retained0 = g.retained_nnz(V)
+    torch.cuda.synchronize()
+    st = g.lp_reorder(model, N, topk=4096, window=16, maxiter=40)
+    out_coo = g.download()
+    dev = torch.device("cuda:0")
+    Bd = torch.from_numpy(B).to(dev)
+    C = torch.empty((coo.nrows, NC), dtype=torch.float32, device=dev)
+    res_dt = {}
+    outs = {}
+    for name, dt in (("bf16", vnm.DT_BF16), ("fp8", vnm.DT_FP8), ("f32", vnm.DT_F32)):
+        p = g.pack(V, dt)
+        if dt == vnm.DT_FP8:
+            Bt, sc = fp8_rows(Bd, NC)
+            sb = 1
+        elif dt == vnm.DT_BF16:
+            Bt, sc, sb = Bd.to(torch.bfloat16), None, 2
+        else:
+            Bt, sc, sb = Bd, None, 4
+        wall, kms = timed_spmm(ctx, p, Bt, dt, C, NC, a.steps, a.warmup, sc)
+        info = p.info()
+        alg = info.bytes + coo.ncols * NC * sb + coo.nrows * NC * 4
+        res_dt[name] = {"ms_per_step": round(wall, 5), "kernel_ms": round(kms, 5),
+                        "tflops": round(2.0 * info.retained_nnz * NC / (wall * 1e-3) / 1e12, 4),
+                        "alg_bytes": int(alg), "a_packed_bytes": int(info.bytes),
+                        "achieved_gbs": round(alg / (kms * 1e-3) / 1e9, 1),
+                        "retained_nnz": int(info.retained_nnz)}
+        outs[name] = C.cpu().numpy()
+    head = res_dt["bf16"]
+    res = {
+        "metric": "V:N:M SpMM TFLOP/s (after GPU V:N:M reorder)",
+        "value": head["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (seeded R-MAT, N(0,1) B)",
+        "config": {"workload": "rmat131072_deg24_V64_N2_M4_B128_bf16", "nodes": coo.nrows,
+                   "nnz": coo.nnz, "V": V, "N": N, "M": M, "ncols": NC, "parallelism": "none (1 GPU)"},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_spmm", "kernel_ms": head["kernel_ms"],
+                     "alg_bytes": head["alg_bytes"], "a_packed_bytes": head["a_packed_bytes"]},
+        "reorder": {"method": "learned proposer + exact gains", "gpu_ms": round(st.ms, 2),
+                    "init_cnt": st.init_cnt, "final_cnt": st.final_cnt,
+                    "retained_nnz_before": retained0, "retained_nnz": head["retained_nnz"]},
+        "variants": res_dt,
+    }
+    if not a.no_cpu:
+        import oracle
+        om = oracle.Matrix(out_coo.nrows, out_coo.ncols, out_coo.row, out_coo.col, out_coo.val)
+        ref = oracle.spmm_vnm_f64(om, V, M, B)
+        for name, X in outs.items():
+            d = X.astype(np.float64) - ref
+            res["variants"][name]["rel_err_fro"] = float(np.linalg.norm(d) / np.linalg.norm(ref))
+            res["variants"][name]["rel_err_max"] = float(np.abs(d).max() / np.abs(ref).max())
+    print(json.dumps(res), flush=True)
+
+
 def bench_learned_262k(a):
     import torch
     import vnm
@@ -127,6 +228,9 @@ def main():
         sys.exit(2)
     if a.workload == "rmat262144":
         bench_learned_262k(a)
+        return
+    if a.workload == "rmat131072":
+        bench_precision_131k(a)
         return
     import numpy as np
     import torch
