@@ -215,6 +215,17 @@ void vnm_packed_destroy(vnm_packed *p);
 int vnm_spmm(vnm_packed *p, const void *B, int ldb, int n, int b_dtype,
              const float *b_scales, float *C, int ldc);
 
+/* Launch shape of vnm_spmm for (b_dtype, n): each lane gathers
+ * cols_per_lane consecutive values of a B row per entry (2 to 16 bytes) and
+ * the columns split into tiles of 64 * cols_per_lane; when the tile count
+ * divides 8, each tile runs on its own XCDs.  Not in the reference (its
+ * kernels have fixed shapes).  By default the first vnm_spmm call of a shape
+ * times every usable value and keeps the fastest; set_shape with a value in
+ * {1, 2, 4, 8, 16} fixes it, 0 clears it; get_shape gives 0 when the shape
+ * has not been chosen yet. */
+int vnm_spmm_set_shape(vnm_packed *p, int b_dtype, int n, int cols_per_lane);
+int vnm_spmm_get_shape(const vnm_packed *p, int b_dtype, int n, int32_t *cols_per_lane);
+
 /* ---------------------------------------------------------------- learned proposer */
 
 /* A graph network that ranks candidate column pairs for the N:M reorder so

@@ -77,9 +77,11 @@ def _e4m3_rows(B):
     return q, scales
 
 
-def _run(ctx, m, M, V, n, adt, bdt, seed=5):
+def _run(ctx, m, M, V, n, adt, bdt, seed=5, vec=None):
     g = vnm.Graph(ctx, m, M)
     p = g.pack(V, adt)
+    if vec is not None:
+        p.set_shape(bdt, n, vec)
     info = p.info()
     B = vnm.gen_normal(seed, (m.ncols, n))
     dev = torch.device("cuda:0")
@@ -137,3 +139,56 @@ def test_spmm_empty_rows_written(ctx):
     C, ref, _ = _run(ctx, m, 8, 8, 64, vnm.DT_F32, vnm.DT_F32)
     assert np.isfinite(C).all()
     assert np.allclose(C, ref, rtol=1e-6, atol=1e-6)
+
+
+TOL = {vnm.DT_F32: 1e-5, vnm.DT_BF16: 1e-2, vnm.DT_FP8: 1e-1}
+SB = {vnm.DT_F32: 4, vnm.DT_BF16: 2, vnm.DT_FP8: 1}
+
+
+def _vecs(dt, n):
+    return [v for v in (1, 2, 4, 8, 16) if 2 <= v * SB[dt] <= 16 and n % (64 * v) == 0]
+
+
+@pytest.mark.parametrize("dt,n", [(vnm.DT_F32, 256), (vnm.DT_F32, 128), (vnm.DT_BF16, 256),
+                                  (vnm.DT_BF16, 128), (vnm.DT_FP8, 128), (vnm.DT_FP8, 1024)])
+def test_spmm_every_shape(ctx, dt, n):
+    # every launch shape the tuner may pick gives the oracle's product
+    m = vnm.gen_rmat(12, 30000, 8)  # hub rows span several work items
+    for v in _vecs(dt, n):
+        C, ref, _ = _run(ctx, m, 8, 64, n, dt, dt, vec=v)
+        fro, mx = _errs(C, ref)
+        assert fro < TOL[dt] and mx < TOL[dt], (v, fro, mx)
+
+
+def test_spmm_tuner_picks_a_usable_shape(ctx):
+    m = vnm.gen_rmat(11, 10000, 9)
+    g = vnm.Graph(ctx, m, 8)
+    p = g.pack(32, vnm.DT_BF16)
+    assert p.get_shape(vnm.DT_BF16, 128) == 0
+    B = torch.zeros((m.ncols, 128), dtype=torch.bfloat16, device="cuda")
+    C = torch.empty((m.nrows, 128), dtype=torch.float32, device="cuda")
+    p.spmm(B.data_ptr(), 128, 128, vnm.DT_BF16, C.data_ptr(), 128)
+    assert p.get_shape(vnm.DT_BF16, 128) in _vecs(vnm.DT_BF16, 128)
+    with pytest.raises(vnm.VnmError):
+        p.set_shape(vnm.DT_F32, 64, 2)  # 64 lanes x 2 columns do not tile 64 columns
+
+
+def test_fp8_decode_every_code(ctx):
+    # C = I * B with FP8 B holding every finite e4m3 code: the hardware
+    # conversion must give the OCP e4m3 values torch decodes
+    R, n = 512, 1024
+    idx = np.arange(R, dtype=np.int32)
+    m = vnm.coo_from_tuples(R, R, idx, idx, np.ones(R, np.float32), 8)
+    g = vnm.Graph(ctx, m, 8)
+    p = g.pack(1, vnm.DT_F32)  # V = 1: every row keeps its one entry
+    assert p.info().retained_nnz == R
+    codes = np.array([c for c in range(256) if (c & 0x7F) != 0x7F], np.uint8)
+    Bq = np.resize(codes, R * n).reshape(R, n)
+    Bd = torch.from_numpy(Bq).to("cuda").view(torch.float8_e4m3fn)
+    sc = torch.ones((R + 127) // 128, dtype=torch.float32, device="cuda")
+    C = torch.empty((R, n), dtype=torch.float32, device="cuda")
+    ref = Bd.to(torch.float32)
+    for v in _vecs(vnm.DT_FP8, n):
+        p.set_shape(vnm.DT_FP8, n, v)
+        p.spmm(Bd.data_ptr(), n, n, vnm.DT_FP8, C.data_ptr(), n, sc.data_ptr())
+        assert torch.equal(C, ref), v

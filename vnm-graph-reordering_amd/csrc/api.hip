@@ -62,7 +62,7 @@ int graph_clone(vnm_graph *src, vnm_graph **out) {
 }
 
 KernelTimer::KernelTimer(vnm_ctx *c, const char *n) : ctx(c), name(n) {
-    if (!ctx->timing) return;
+    if (!ctx || !ctx->timing) return;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
         a = b = nullptr;
         (void)hipGetLastError();

@@ -19,17 +19,23 @@
 // (rowptr; per row entry its row, block, selector byte and two values; per
 // block its segment, selectors and scale).
 //
-// SpMM: C[R x n] = A_vnm B, FP32 accumulation.  A work item is 64
+// SpMM: C[R x n] = A_vnm B, FP32 accumulation.  Row entries also carry the
+// absolute column of their first value (ent_col) and the in-segment offset of
+// the second (low nibble of ent_idx, 0x10 = present), so the SpMM reads no
+// block data (except the block scale of FP8 A).  A work item is 64
 // consecutive row entries (rows may start or end inside it); a wave takes one
-// item: lane k first resolves entry k (its two B rows and scaled values),
-// then the wave walks the entries, gathering B rows four entries deep with
-// every lane owning n / 64 consecutive columns, and flushes the row
-// accumulator whenever the row changes.  Rows wholly inside an item are
-// stored once; a row cut by an item boundary adds into C with FP32 atomics
-// (those rows, and rows that keep nothing, are zeroed first from a list built
-// at pack time).  Item indices are remapped so that consecutive items run on
-// the same XCD (workgroups are dealt round-robin over the 8 XCDs), keeping the
-// B rows of neighbouring (reordered, hence similar) rows in one L2.
+// item for one column tile of 64 * VEC columns: lane k first resolves entry k
+// (its two B rows and scaled values), then the wave walks the entries with the
+// entry data broadcast by scalar reads, every lane gathering VEC consecutive
+// values of each B row (2 to 16 bytes) four entries deep and owning VEC
+// columns of the output row, which is flushed when the row changes.  Rows
+// wholly inside an item are stored once; a row cut by an item boundary adds
+// into C with FP32 atomics (those rows, and rows that keep nothing, are zeroed
+// first from a list built on first use).  When the tile count T divides 8,
+// tile t runs on its own 8/T XCDs, so each XCD's L2 serves one slice of B.
+// VEC is chosen per (B dtype, n) by timing the usable values on first use
+// (vnm_spmm_set_shape overrides).  FP8 B is decoded by the gfx950 e4m3
+// conversion instruction.
 #include "graph.h"
 #include "hip_common.h"
 #include "sortlib.h"
@@ -39,19 +45,28 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <vector>
+
+namespace vnm {
+struct ZeroList {
+    int n = 0;
+    DBuf<int32_t> rows;  // rows that keep nothing or are cut by a group boundary
+};
+}  // namespace vnm
 
 struct vnm_packed {
     vnm_ctx *ctx = nullptr;
     int32_t V = 0, M = 0, nrows = 0, ncols = 0, dtype = 0;
     int32_t nblocks = 0, nent = 0;
     int64_t retained = 0;
-    int32_t nzero = 0;                                     // rows in zero_rows
-    vnm::DBuf<int32_t> rowptr, ent_row, ent_blk, blk_seg;
+    vnm::DBuf<int32_t> rowptr, ent_row, ent_col, ent_blk, blk_seg;
     vnm::DBuf<uint8_t> ent_idx, ent_val;                   // values: 2 per entry, dtype-sized
     vnm::DBuf<uint16_t> blk_sel;
     vnm::DBuf<float> blk_scale;
-    vnm::DBuf<int32_t> zero_rows;                          // empty rows and rows cut by items
+    std::map<int, std::unique_ptr<vnm::ZeroList>> zero;    // by entries per item
+    std::map<int, int> shapes;                             // (b dtype << 24 | n) -> values per lane
 };
 
 namespace vnm {
@@ -282,7 +297,9 @@ __global__ void k_scales(const uint32_t *__restrict__ absmax, int nb, int fp8, f
 __global__ void k_emit(const int32_t *__restrict__ sel_runs, int E, const int32_t *__restrict__ run_blk,
                        const uint64_t *__restrict__ keys, const int32_t *__restrict__ rstart,
                        const float *__restrict__ kv, const uint8_t *__restrict__ kidx,
-                       const float *__restrict__ sc, int dtype, int32_t *__restrict__ ent_row,
+                       const float *__restrict__ sc, const int32_t *__restrict__ blk_seg,
+                       const uint16_t *__restrict__ blk_sel, int M, int dtype,
+                       int32_t *__restrict__ ent_row, int32_t *__restrict__ ent_col,
                        int32_t *__restrict__ ent_blk, uint8_t *__restrict__ ent_idx,
                        uint8_t *__restrict__ ent_val) {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -290,8 +307,11 @@ __global__ void k_emit(const int32_t *__restrict__ sel_runs, int E, const int32_
     const int u = sel_runs[e];
     const int b = run_blk[u];
     ent_row[e] = (int32_t)(keys[rstart[u]] >> 32);
-    ent_blk[e] = b;
-    ent_idx[e] = kidx[u];
+    // selector indices -> columns: first value absolute, second as offset
+    const uint32_t code = kidx[u], sel = blk_sel[b];
+    ent_col[e] = blk_seg[b] * M + (int)((sel >> (4 * (code & 3u))) & 15u);
+    ent_idx[e] = (code & 0x10u) ? (uint8_t)(0x10u | ((sel >> (4 * ((code >> 2) & 3u))) & 15u)) : (uint8_t)0;
+    if (dtype == VNM_DT_FP8) ent_blk[e] = b;
     const float v0 = kv[2 * u], v1 = kv[2 * u + 1];
     if (dtype == VNM_DT_F32) {
         float *p = (float *)ent_val;
@@ -313,13 +333,13 @@ __global__ void k_emit(const int32_t *__restrict__ sel_runs, int E, const int32_
 }
 
 // Rows the SpMM must zero before accumulating: rows that keep nothing and
-// rows cut by a 64-entry item boundary.
-__global__ void k_zero_flags(const int32_t *__restrict__ rowptr, int R, int32_t *__restrict__ flag) {
+// rows cut by a boundary of gsz-entry groups.
+__global__ void k_zero_flags(const int32_t *__restrict__ rowptr, int R, int gsz, int32_t *__restrict__ flag) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const int b = rowptr[r], e = rowptr[r + 1];
     const bool empty = e == b;
-    const bool cut = !empty && (b / ITEM != (e - 1) / ITEM);
+    const bool cut = !empty && (b / gsz != (e - 1) / gsz);
     flag[r] = empty || cut;
 }
 
@@ -339,102 +359,133 @@ __device__ __forceinline__ float load_aval(const uint8_t *p, int i) {
     else return e4m3_to_f32(p[i]);
 }
 
-// CPL consecutive B values of row c starting at column col0, as floats.
-template <int BT, int CPL>
-__device__ __forceinline__ void load_brow(const void *B, int ldb, int c, int col0, float (&x)[CPL]) {
+template <int BT>
+constexpr int bsize() { return BT == VNM_DT_F32 ? 4 : BT == VNM_DT_BF16 ? 2 : 1; }
+
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+// VEC consecutive B values (VEC * sizeof(value) bytes, 2..16) of one row.
+template <int BT, int VEC>
+struct BRaw {
+    static constexpr int BYTES = VEC * bsize<BT>();
+    static constexpr int NW = BYTES < 4 ? 1 : BYTES / 4;
+    uint32_t w[NW];
+};
+
+template <int BT, int VEC>
+__device__ __forceinline__ BRaw<BT, VEC> load_raw(const uint8_t *p) {
+    BRaw<BT, VEC> r;
+    constexpr int BY = BRaw<BT, VEC>::BYTES;
+    if constexpr (BY == 16) {
+        const uint4 t = *(const uint4 *)p;
+        r.w[0] = t.x; r.w[1] = t.y; r.w[2] = t.z; r.w[3] = t.w;
+    } else if constexpr (BY == 8) {
+        const uint2 t = *(const uint2 *)p;
+        r.w[0] = t.x; r.w[1] = t.y;
+    } else if constexpr (BY == 4) {
+        r.w[0] = *(const uint32_t *)p;
+    } else {
+        r.w[0] = *(const uint16_t *)p;
+    }
+    return r;
+}
+
+template <int BT, int VEC>
+__device__ __forceinline__ BRaw<BT, VEC> zero_raw() {
+    BRaw<BT, VEC> r;
+#pragma unroll
+    for (int i = 0; i < BRaw<BT, VEC>::NW; ++i) r.w[i] = 0;
+    return r;
+}
+
+// acc += w * (decoded raw)
+template <int BT, int VEC>
+__device__ __forceinline__ void fma_raw(float (&acc)[VEC], float w, const BRaw<BT, VEC> &raw) {
     if constexpr (BT == VNM_DT_F32) {
-        const float *p = (const float *)B + (size_t)c * ldb + col0;
-        if constexpr (CPL == 4) {
-            float4 t = *(const float4 *)p;
-            x[0] = t.x; x[1] = t.y; x[2] = t.z; x[3] = t.w;
-        } else if constexpr (CPL == 2) {
-            float2 t = *(const float2 *)p;
-            x[0] = t.x; x[1] = t.y;
-        } else {
 #pragma unroll
-            for (int i = 0; i < CPL; ++i) x[i] = p[i];
-        }
+        for (int i = 0; i < VEC; ++i) acc[i] = fmaf(w, __uint_as_float(raw.w[i]), acc[i]);
     } else if constexpr (BT == VNM_DT_BF16) {
-        const uint16_t *p = (const uint16_t *)B + (size_t)c * ldb + col0;
-        if constexpr (CPL == 4) {
-            uint2 t = *(const uint2 *)p;
-            x[0] = __uint_as_float(t.x << 16); x[1] = __uint_as_float(t.x & 0xFFFF0000u);
-            x[2] = __uint_as_float(t.y << 16); x[3] = __uint_as_float(t.y & 0xFFFF0000u);
-        } else if constexpr (CPL == 2) {
-            uint32_t t = *(const uint32_t *)p;
-            x[0] = __uint_as_float(t << 16); x[1] = __uint_as_float(t & 0xFFFF0000u);
-        } else {
 #pragma unroll
-            for (int i = 0; i < CPL; ++i) x[i] = bf16_to_f32(p[i]);
+        for (int i = 0; i < VEC; ++i) {
+            const uint32_t x = raw.w[i / 2];
+            acc[i] = fmaf(w, __uint_as_float((i & 1) ? (x & 0xFFFF0000u) : (x << 16)), acc[i]);
         }
     } else {
-        const uint8_t *p = (const uint8_t *)B + (size_t)c * ldb + col0;
-        if constexpr (CPL == 4) {
-            uint32_t t = *(const uint32_t *)p;
+        // OCP e4m3 through the gfx950 conversion, two values per call
 #pragma unroll
-            for (int i = 0; i < 4; ++i) x[i] = e4m3_to_f32((uint8_t)(t >> (8 * i)));
-        } else {
-#pragma unroll
-            for (int i = 0; i < CPL; ++i) x[i] = e4m3_to_f32(p[i]);
+        for (int i = 0; i < VEC; i += 4) {
+            const uint32_t x = raw.w[i / 4];
+            const f32x2_t lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)x, false);
+            acc[i] = fmaf(w, lo.x, acc[i]);
+            acc[i + 1] = fmaf(w, lo.y, acc[i + 1]);
+            if constexpr (VEC >= 4) {
+                const f32x2_t hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)x, true);
+                acc[i + 2] = fmaf(w, hi.x, acc[i + 2]);
+                acc[i + 3] = fmaf(w, hi.y, acc[i + 3]);
+            }
         }
     }
 }
 
-template <int CPL>
-__device__ __forceinline__ void flush_row(float *__restrict__ C, int ldc, int row, int col0,
-                                          const float (&acc)[CPL], bool atomic) {
+template <int NV>
+__device__ __forceinline__ void flush_vec(float *__restrict__ C, int ldc, int row, int col0,
+                                          const float (&acc)[NV], bool atomic) {
     float *cp = C + (size_t)row * ldc + col0;
     if (atomic) {
 #pragma unroll
-        for (int i = 0; i < CPL; ++i) atomicAdd(&cp[i], acc[i]);
-    } else {
-        if constexpr (CPL == 4) *(float4 *)cp = make_float4(acc[0], acc[1], acc[2], acc[3]);
-        else if constexpr (CPL == 2) *(float2 *)cp = make_float2(acc[0], acc[1]);
-        else {
+        for (int i = 0; i < NV; ++i) atomicAdd(&cp[i], acc[i]);
+    } else if constexpr (NV % 4 == 0) {
 #pragma unroll
-            for (int i = 0; i < CPL; ++i) cp[i] = acc[i];
-        }
+        for (int i = 0; i < NV; i += 4) *(float4 *)(cp + i) = make_float4(acc[i], acc[i + 1], acc[i + 2], acc[i + 3]);
+    } else if constexpr (NV == 2) {
+        *(float2 *)cp = make_float2(acc[0], acc[1]);
+    } else {
+        cp[0] = acc[0];
     }
 }
 
-template <int AT, int BT, int CPL>
+// C = A_vnm B for one column tile of 64 * VEC columns: a wave takes a work
+// item of 64 row entries, lane k resolves entry k (row, both B rows, scaled
+// values), then the wave walks the entries with the entry data broadcast
+// through scalar reads, each lane gathering VEC values of the B rows four
+// entries deep and owning VEC columns of the output row.
+template <int AT, int BT, int VEC>
 __global__ __launch_bounds__(256) void k_spmm(
     int E, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ ent_row,
-    const int32_t *__restrict__ ent_blk, const uint8_t *__restrict__ ent_idx,
-    const uint8_t *__restrict__ ent_val, const int32_t *__restrict__ blk_seg,
-    const uint16_t *__restrict__ blk_sel, const float *__restrict__ blk_scale, int M,
-    const void *__restrict__ B, int ldb, const float *__restrict__ b_scales,
-    float *__restrict__ C, int ldc, int n_tiles, int nwg) {
-    const int lane = threadIdx.x & 63;
-    const int wpb = blockDim.x / 64;
-    // XCD-aware remap: workgroup g runs on XCD g % 8; give each XCD a
-    // contiguous range of items (the grid is a multiple of 8, so the map
-    // from blockIdx to lwg is a bijection on [0, gridDim.x))
-    const int per = gridDim.x / 8;
+    const int32_t *__restrict__ ent_col, const uint8_t *__restrict__ ent_idx,
+    const uint8_t *__restrict__ ent_val, const int32_t *__restrict__ ent_blk,
+    const float *__restrict__ blk_scale, int M, const uint8_t *__restrict__ B, int ldb,
+    const float *__restrict__ b_scales, float *__restrict__ C, int ldc, int T, int items, int per_g) {
+    constexpr int SB = bsize<BT>(), U = 4;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // workgroups are dealt round-robin over the 8 XCDs
     const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-    const int lwg = xcd * per + slot;
-    if (lwg >= nwg) return;
-    const int tile = lwg % n_tiles;
-    const int item = (lwg / n_tiles) * wpb + (threadIdx.x >> 6);
+    int tile, iwg;
+    if (8 % T == 0) {  // tile t on XCDs [t*8/T, (t+1)*8/T)
+        const int gs = 8 / T;
+        tile = xcd / gs;
+        iwg = (xcd % gs) * per_g + slot;
+    } else {
+        const int lwg = xcd * per_g + slot;
+        tile = lwg % T;
+        iwg = lwg / T;
+    }
+    const int item = iwg * 4 + wv;
+    if (item >= items) return;  // wave-uniform
     const int beg = item * ITEM;
-    if (beg >= E) return;
     const int cnt = min(ITEM, E - beg);
-    const int col0 = tile * 64 * CPL + lane * CPL;
-    // Lane k resolves entry beg + k once: its row, both B rows and values.
     int rr = 0, c0 = 0, c1 = -1;
     float a0 = 0.f, a1 = 0.f;
     if (lane < cnt) {
         const int e = beg + lane;
         rr = ent_row[e];
-        const int b = ent_blk[e];
+        c0 = ent_col[e];
         const uint32_t code = ent_idx[e];
-        const int base = blk_seg[b] * M;
-        const uint32_t sel = blk_sel[b];
-        const float sc = blk_scale[b];
-        c0 = base + ((sel >> (4 * (code & 3u))) & 15u);
+        float sc = 1.f;
+        if constexpr (AT == VNM_DT_FP8) sc = blk_scale[ent_blk[e]];
         a0 = load_aval<AT>(ent_val, 2 * e) * sc;
         if (code & 0x10u) {
-            c1 = base + ((sel >> (4 * ((code >> 2) & 3u))) & 15u);
+            c1 = (c0 & ~(M - 1)) | (int)(code & 15u);
             a1 = load_aval<AT>(ent_val, 2 * e + 1) * sc;
         }
         if constexpr (BT == VNM_DT_FP8) {
@@ -444,14 +495,14 @@ __global__ __launch_bounds__(256) void k_spmm(
     }
     const int first_row = __builtin_amdgcn_readlane(rr, 0);
     const int last_row = __builtin_amdgcn_readlane(rr, (cnt - 1) & 63);
-    float acc[CPL];
+    const int col0 = tile * (64 * VEC) + lane * VEC;
+    const uint8_t *bcol = B + (size_t)col0 * SB;
+    float acc[VEC];
 #pragma unroll
-    for (int i = 0; i < CPL; ++i) acc[i] = 0.f;
+    for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
     int cur = first_row;
-    // Gather B rows four entries at a time (up to eight rows in flight).
-    constexpr int U = 4;
     for (int k = 0; k < cnt; k += U) {
-        float x[U][2][CPL];
+        BRaw<BT, VEC> raw[U][2];
         float w[U][2];
         int rowk[U];
 #pragma unroll
@@ -463,41 +514,31 @@ __global__ __launch_bounds__(256) void k_spmm(
             // slots past the item's end repeat its last row, so they never
             // look like a row change
             rowk[u] = ok ? __builtin_amdgcn_readlane(rr, kk & 63) : last_row;
-            w[u][0] = ok ? __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a0), kk & 63)) : 0.f;
-            w[u][1] = (ok && ck1 >= 0) ? __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, a1), kk & 63)) : 0.f;
-            if (ok) load_brow<BT, CPL>(B, ldb, ck0, col0, x[u][0]);
-            else {
-#pragma unroll
-                for (int i = 0; i < CPL; ++i) x[u][0][i] = 0.f;
-            }
-            if (ok && ck1 >= 0) load_brow<BT, CPL>(B, ldb, ck1, col0, x[u][1]);
-            else {
-#pragma unroll
-                for (int i = 0; i < CPL; ++i) x[u][1][i] = 0.f;
-            }
+            w[u][0] = ok ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a0), kk & 63)) : 0.f;
+            w[u][1] = (ok && ck1 >= 0) ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a1), kk & 63)) : 0.f;
+            raw[u][0] = ok ? load_raw<BT, VEC>(bcol + (size_t)ck0 * ldb * SB) : zero_raw<BT, VEC>();
+            raw[u][1] = (ok && ck1 >= 0) ? load_raw<BT, VEC>(bcol + (size_t)ck1 * ldb * SB) : zero_raw<BT, VEC>();
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (rowk[u] != cur) {  // wave-uniform: the row changed
                 const bool cut = (cur == first_row && rowptr[cur] < beg) ||
                                  (cur == last_row && rowptr[cur + 1] > beg + cnt);
-                flush_row<CPL>(C, ldc, cur, col0, acc, cut);
+                flush_vec<VEC>(C, ldc, cur, col0, acc, cut);
 #pragma unroll
-                for (int i = 0; i < CPL; ++i) acc[i] = 0.f;
+                for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
                 cur = rowk[u];
             }
-#pragma unroll
-            for (int i = 0; i < CPL; ++i)
-                acc[i] = fmaf(w[u][1], x[u][1][i], fmaf(w[u][0], x[u][0][i], acc[i]));
+            fma_raw<BT, VEC>(acc, w[u][0], raw[u][0]);
+            fma_raw<BT, VEC>(acc, w[u][1], raw[u][1]);
         }
     }
-    const bool cut = (cur == first_row && rowptr[cur] < beg) ||
-                     (cur == last_row && rowptr[cur + 1] > beg + cnt);
-    flush_row<CPL>(C, ldc, cur, col0, acc, cut);
+    const bool cut = (cur == first_row && rowptr[cur] < beg) || (cur == last_row && rowptr[cur + 1] > beg + cnt);
+    flush_vec<VEC>(C, ldc, cur, col0, acc, cut);
 }
 
-// Zero the rows of the pack-time list (rows that keep nothing, rows cut by
-// an item boundary): one wave per row, float4 stores.
+// Zero the rows of a zero list (rows that keep nothing, rows cut by a group
+// boundary): one wave per row, float4 stores.
 __global__ void k_zero_rows(const int32_t *__restrict__ list, int nz, int n, float *__restrict__ C,
                             int ldc) {
     const int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
@@ -648,7 +689,8 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
     }
     p->nent = E;
     const int vs = dtype == VNM_DT_F32 ? 4 : dtype == VNM_DT_FP8 ? 1 : 2;
-    if ((rc = p->rowptr.alloc(R + 1)) || (rc = p->ent_blk.alloc(std::max(E, 1))) ||
+    if ((rc = p->rowptr.alloc(R + 1)) || (rc = p->ent_col.alloc(std::max(E, 1))) ||
+        (rc = p->ent_blk.alloc(dtype == VNM_DT_FP8 ? std::max(E, 1) : 1)) ||
         (rc = p->ent_row.alloc(std::max(E, 1))) || (rc = p->ent_idx.alloc(std::max(E, 1))) ||
         (rc = p->ent_val.alloc((size_t)std::max(E, 1) * 2 * vs))) {
         delete p;
@@ -657,21 +699,8 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
     if ((rc = scan_excl_i32(rcnt.p, p->rowptr.p, R + 1, st))) { delete p; return rc; }
     if (E > 0)
         hipLaunchKernelGGL(k_emit, dim3(grid_of(E, 256)), dim3(256), 0, st, sel_runs.p, E, run_blk.p,
-                           keys2.p, rstart.p, kv.p, kidx.p, p->blk_scale.p, dtype, p->ent_row.p,
-                           p->ent_blk.p, p->ent_idx.p, p->ent_val.p);
-    // rows to zero before accumulating
-    DBuf<int32_t> zflag, zexcl;
-    if ((rc = zflag.alloc(R + 1)) || (rc = zexcl.alloc(R + 1))) { delete p; return rc; }
-    VNM_HIP_TRY(hipMemsetAsync(zflag.p, 0, (size_t)(R + 1) * 4, st));
-    hipLaunchKernelGGL(k_zero_flags, dim3(grid_of(R, 256)), dim3(256), 0, st, p->rowptr.p, R, zflag.p);
-    if ((rc = scan_excl_i32(zflag.p, zexcl.p, R + 1, st))) { delete p; return rc; }
-    int32_t nzero = 0;
-    VNM_HIP_TRY(hipMemcpyAsync(&nzero, zexcl.p + R, 4, hipMemcpyDeviceToHost, st));
-    VNM_HIP_TRY(hipStreamSynchronize(st));
-    p->nzero = nzero;
-    if ((rc = p->zero_rows.alloc(std::max(nzero, 1)))) { delete p; return rc; }
-    hipLaunchKernelGGL(k_zero_list, dim3(grid_of(R, 256)), dim3(256), 0, st, zflag.p, zexcl.p, R,
-                       p->zero_rows.p);
+                           keys2.p, rstart.p, kv.p, kidx.p, p->blk_scale.p, p->blk_seg.p, p->blk_sel.p,
+                           M, dtype, p->ent_row.p, p->ent_col.p, p->ent_blk.p, p->ent_idx.p, p->ent_val.p);
     g->ctx->launches += 16;
     VNM_LAUNCH_CHECK("pack emit");
     VNM_HIP_TRY(hipStreamSynchronize(st));
@@ -679,33 +708,139 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
     return VNM_OK;
 }
 
-template <int AT, int BT, int CPL>
-static void launch_spmm_cpl(vnm_packed *p, const void *B, int ldb, int n, const float *bs, float *C,
-                            int ldc, hipStream_t st) {
-    const int wpb = 4;
-    const int tiles = n / (64 * CPL);
-    const int items = (p->nent + ITEM - 1) / ITEM;
-    const int nwg = (items + wpb - 1) / wpb * tiles;
-    hipLaunchKernelGGL((k_spmm<AT, BT, CPL>), dim3((nwg + 7) / 8 * 8), dim3(64 * wpb), 0, st,
-                       p->nent, p->rowptr.p, p->ent_row.p, p->ent_blk.p, p->ent_idx.p, p->ent_val.p,
-                       p->blk_seg.p, p->blk_sel.p, p->blk_scale.p, p->M, B, ldb, bs, C, ldc, tiles,
-                       nwg);
+// Rows to zero for groups of gsz entries, built on first use.
+static int zero_list(vnm_packed *p, int gsz, ZeroList **out) {
+    auto it = p->zero.find(gsz);
+    if (it != p->zero.end()) {
+        *out = it->second.get();
+        return VNM_OK;
+    }
+    hipStream_t st = p->ctx->stream;
+    const int R = p->nrows;
+    auto z = std::make_unique<ZeroList>();
+    DBuf<int32_t> zflag, zexcl;
+    int rc;
+    if ((rc = zflag.alloc(R + 1)) || (rc = zexcl.alloc(R + 1))) return rc;
+    VNM_HIP_TRY(hipMemsetAsync(zflag.p, 0, (size_t)(R + 1) * 4, st));
+    hipLaunchKernelGGL(k_zero_flags, dim3(grid_of(R, 256)), dim3(256), 0, st, p->rowptr.p, R, gsz, zflag.p);
+    if ((rc = scan_excl_i32(zflag.p, zexcl.p, R + 1, st))) return rc;
+    VNM_HIP_TRY(hipMemcpyAsync(&z->n, zexcl.p + R, 4, hipMemcpyDeviceToHost, st));
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = z->rows.alloc(std::max(z->n, 1)))) return rc;
+    hipLaunchKernelGGL(k_zero_list, dim3(grid_of(R, 256)), dim3(256), 0, st, zflag.p, zexcl.p, R, z->rows.p);
+    p->ctx->launches += 2;
+    VNM_LAUNCH_CHECK("k_zero_list");
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    *out = z.get();
+    p->zero[gsz] = std::move(z);
+    return VNM_OK;
 }
 
-template <int AT, int BT>
-static void launch_spmm(vnm_packed *p, const void *B, int ldb, int n, const float *bs, float *C,
-                        int ldc, hipStream_t st) {
-    if (n % 256 == 0) launch_spmm_cpl<AT, BT, 4>(p, B, ldb, n, bs, C, ldc, st);
-    else if (n % 128 == 0) launch_spmm_cpl<AT, BT, 2>(p, B, ldb, n, bs, C, ldc, st);
-    else launch_spmm_cpl<AT, BT, 1>(p, B, ldb, n, bs, C, ldc, st);
+static int bsize_of(int bt) { return bt == VNM_DT_F32 ? 4 : bt == VNM_DT_BF16 ? 2 : 1; }
+
+// Values per lane usable for (bt, n): 2 to 16 bytes per lane load, and the
+// 64 * vec column tile divides n.
+static bool vec_ok(int bt, int n, int vec) {
+    if (vec != 1 && vec != 2 && vec != 4 && vec != 8 && vec != 16) return false;
+    const int bytes = vec * bsize_of(bt);
+    return bytes >= 2 && bytes <= 16 && n % (64 * vec) == 0;
+}
+
+template <int AT, int BT, int VEC>
+static void launch_v(vnm_packed *p, const void *B, int ldb, int n, const float *bs, float *C, int ldc,
+                     hipStream_t st) {
+    const int T = n / (64 * VEC);
+    const int items = (p->nent + ITEM - 1) / ITEM;
+    const int iwg = (items + 3) / 4;
+    int per_g;
+    if (8 % T == 0) {
+        const int gs = 8 / T;
+        per_g = (iwg + gs - 1) / gs;
+    } else {
+        per_g = (iwg * T + 7) / 8;
+    }
+    hipLaunchKernelGGL((k_spmm<AT, BT, VEC>), dim3(8 * per_g), dim3(256), 0, st, p->nent, p->rowptr.p,
+                       p->ent_row.p, p->ent_col.p, p->ent_idx.p, p->ent_val.p, p->ent_blk.p,
+                       p->blk_scale.p, p->M, (const uint8_t *)B, ldb, bs, C, ldc, T, items, per_g);
 }
 
 template <int AT>
-static void launch_spmm_b(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs,
-                          float *C, int ldc, hipStream_t st) {
-    if (bt == VNM_DT_F32) launch_spmm<AT, VNM_DT_F32>(p, B, ldb, n, bs, C, ldc, st);
-    else if (bt == VNM_DT_BF16) launch_spmm<AT, VNM_DT_BF16>(p, B, ldb, n, bs, C, ldc, st);
-    else launch_spmm<AT, VNM_DT_FP8>(p, B, ldb, n, bs, C, ldc, st);
+static void launch_a(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, const float *bs, float *C,
+                     int ldc, hipStream_t st) {
+    if (bt == VNM_DT_F32) {
+        if (vec == 4) launch_v<AT, VNM_DT_F32, 4>(p, B, ldb, n, bs, C, ldc, st);
+        else if (vec == 2) launch_v<AT, VNM_DT_F32, 2>(p, B, ldb, n, bs, C, ldc, st);
+        else launch_v<AT, VNM_DT_F32, 1>(p, B, ldb, n, bs, C, ldc, st);
+    } else if (bt == VNM_DT_BF16) {
+        if (vec == 8) launch_v<AT, VNM_DT_BF16, 8>(p, B, ldb, n, bs, C, ldc, st);
+        else if (vec == 4) launch_v<AT, VNM_DT_BF16, 4>(p, B, ldb, n, bs, C, ldc, st);
+        else if (vec == 2) launch_v<AT, VNM_DT_BF16, 2>(p, B, ldb, n, bs, C, ldc, st);
+        else launch_v<AT, VNM_DT_BF16, 1>(p, B, ldb, n, bs, C, ldc, st);
+    } else {
+        if (vec == 16) launch_v<AT, VNM_DT_FP8, 16>(p, B, ldb, n, bs, C, ldc, st);
+        else if (vec == 8) launch_v<AT, VNM_DT_FP8, 8>(p, B, ldb, n, bs, C, ldc, st);
+        else if (vec == 4) launch_v<AT, VNM_DT_FP8, 4>(p, B, ldb, n, bs, C, ldc, st);
+        else launch_v<AT, VNM_DT_FP8, 2>(p, B, ldb, n, bs, C, ldc, st);
+    }
+}
+
+// One SpMM with `vec` values per lane: zero the listed rows, then the gather kernel.
+static int spmm_v(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, const float *bs, float *C,
+                  int ldc, bool timed) {
+    hipStream_t st = p->ctx->stream;
+    ZeroList *z;
+    int rc = zero_list(p, ITEM, &z);
+    if (rc) return rc;
+    if (z->n > 0)
+        hipLaunchKernelGGL(k_zero_rows, dim3((z->n + 3) / 4), dim3(256), 0, st, z->rows.p, z->n, n, C, ldc);
+    if (p->nent > 0) {
+        KernelTimer kt(timed ? p->ctx : nullptr, "k_spmm");
+        switch (p->dtype) {
+            case VNM_DT_F32: launch_a<VNM_DT_F32>(p, vec, B, ldb, n, bt, bs, C, ldc, st); break;
+            case VNM_DT_BF16: launch_a<VNM_DT_BF16>(p, vec, B, ldb, n, bt, bs, C, ldc, st); break;
+            case VNM_DT_F16: launch_a<VNM_DT_F16>(p, vec, B, ldb, n, bt, bs, C, ldc, st); break;
+            default: launch_a<VNM_DT_FP8>(p, vec, B, ldb, n, bt, bs, C, ldc, st); break;
+        }
+        kt.stop();
+    }
+    p->ctx->launches += 2;
+    VNM_LAUNCH_CHECK("k_spmm");
+    return VNM_OK;
+}
+
+static int shape_key(int bt, int n) { return (bt << 24) | n; }
+
+// Autotuner: time every usable values-per-lane setting on this matrix and
+// shape (one warm-up and three timed calls each, HIP events on the context
+// stream) and keep the fastest; the choice stays with the packed matrix.
+static int tune_vec(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc,
+                    int *best_v) {
+    hipStream_t st = p->ctx->stream;
+    hipEvent_t e0, e1;
+    VNM_HIP_TRY(hipEventCreate(&e0));
+    VNM_HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bv = 0, rc = VNM_OK;
+    for (int v = 16; v >= 1 && rc == VNM_OK; v /= 2) {
+        if (!vec_ok(bt, n, v)) continue;
+        if ((rc = spmm_v(p, v, B, ldb, n, bt, bs, C, ldc, false))) break;
+        if (hipEventRecord(e0, st) != hipSuccess) { rc = set_error(VNM_E_HIP, "event record failed"); break; }
+        for (int i = 0; i < 3 && rc == VNM_OK; ++i) rc = spmm_v(p, v, B, ldb, n, bt, bs, C, ldc, false);
+        if (rc) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = set_error(VNM_E_HIP, "event timing failed");
+            break;
+        }
+        if (getenv("VNM_DEBUG")) fprintf(stderr, "[vnm] spmm tune bt=%d n=%d vec=%d %.4f ms\n", bt, n, v, ms / 3);
+        if (ms < best) { best = ms; bv = v; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *best_v = bv;
+    return VNM_OK;
 }
 
 int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc) {
@@ -714,25 +849,20 @@ int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, 
     if (bt != VNM_DT_F32 && bt != VNM_DT_BF16 && bt != VNM_DT_FP8)
         return set_error(VNM_E_ARG, "spmm: B dtype must be F32, BF16 or FP8");
     if (bt == VNM_DT_FP8 && !bs) return set_error(VNM_E_ARG, "spmm: FP8 B needs per-128-row scales");
-    if ((bt == VNM_DT_BF16 && (ldb % 4) != 0) || (bt == VNM_DT_FP8 && (ldb % 4) != 0) || (ldc % 4) != 0)
-        return set_error(VNM_E_ARG, "spmm: leading dimensions must be multiples of 4");
-    hipStream_t st = p->ctx->stream;
-    if (p->nzero > 0)
-        hipLaunchKernelGGL(k_zero_rows, dim3((p->nzero + 3) / 4), dim3(256), 0, st, p->zero_rows.p,
-                           p->nzero, n, C, ldc);
-    if (p->nent > 0) {
-        KernelTimer kt(p->ctx, "k_spmm");
-        switch (p->dtype) {
-            case VNM_DT_F32: launch_spmm_b<VNM_DT_F32>(p, B, ldb, n, bt, bs, C, ldc, st); break;
-            case VNM_DT_BF16: launch_spmm_b<VNM_DT_BF16>(p, B, ldb, n, bt, bs, C, ldc, st); break;
-            case VNM_DT_F16: launch_spmm_b<VNM_DT_F16>(p, B, ldb, n, bt, bs, C, ldc, st); break;
-            default: launch_spmm_b<VNM_DT_FP8>(p, B, ldb, n, bt, bs, C, ldc, st); break;
-        }
-        kt.stop();
+    // vector loads of up to 16 bytes of B, stores of up to 16 bytes of C
+    if ((ldb * bsize_of(bt)) % 16 != 0 || (ldc % 4) != 0 || ((uintptr_t)B & 15) || ((uintptr_t)C & 15))
+        return set_error(VNM_E_ARG, "spmm: B and C must be 16-byte aligned with rows of whole 16-byte "
+                                    "units (ldb * sizeof(B value) and 4 * ldc multiples of 16)");
+    int v = 0;
+    auto it = p->shapes.find(shape_key(bt, n));
+    if (it != p->shapes.end()) {
+        v = it->second;
+    } else {
+        int rc = tune_vec(p, B, ldb, n, bt, bs, C, ldc, &v);
+        if (rc) return rc;
+        p->shapes[shape_key(bt, n)] = v;
     }
-    p->ctx->launches += 2;
-    VNM_LAUNCH_CHECK("k_spmm");
-    return VNM_OK;
+    return spmm_v(p, v, B, ldb, n, bt, bs, C, ldc, true);
 }
 
 }  // namespace vnm
@@ -762,7 +892,8 @@ int vnm_packed_info_get(const vnm_packed *p, vnm_packed_info *info) {
     info->dtype = p->dtype;
     info->retained_nnz = p->retained;
     const int vs = p->dtype == VNM_DT_F32 ? 4 : p->dtype == VNM_DT_FP8 ? 1 : 2;
-    info->bytes = 4ll * (p->nrows + 1) + (int64_t)p->nent * (4 + 4 + 1 + 2 * vs) +
+    const int blk = p->dtype == VNM_DT_FP8 ? 4 : 0;  // per-entry block index for the FP8 scale
+    info->bytes = 4ll * (p->nrows + 1) + (int64_t)p->nent * (4 + 4 + 1 + 2 * vs + blk) +
                   (int64_t)p->nblocks * (4 + 2 + 4);
     return VNM_OK;
 }
@@ -776,6 +907,26 @@ void vnm_packed_destroy(vnm_packed *p) {
 int vnm_spmm(vnm_packed *p, const void *B, int ldb, int n, int b_dtype, const float *b_scales,
              float *C, int ldc) {
     return spmm(p, B, ldb, n, b_dtype, b_scales, C, ldc);
+}
+
+int vnm_spmm_set_shape(vnm_packed *p, int b_dtype, int n, int cols_per_lane) {
+    if (!p) return set_error(VNM_E_ARG, "null argument");
+    if (cols_per_lane == 0) {
+        p->shapes.erase(shape_key(b_dtype, n));
+        return VNM_OK;
+    }
+    if (!vec_ok(b_dtype, n, cols_per_lane))
+        return set_error(VNM_E_ARG, "spmm: %d columns per lane do not fit n = %d for B dtype %d",
+                         cols_per_lane, n, b_dtype);
+    p->shapes[shape_key(b_dtype, n)] = cols_per_lane;
+    return VNM_OK;
+}
+
+int vnm_spmm_get_shape(const vnm_packed *p, int b_dtype, int n, int32_t *cols_per_lane) {
+    if (!p || !cols_per_lane) return set_error(VNM_E_ARG, "null argument");
+    auto it = p->shapes.find(shape_key(b_dtype, n));
+    *cols_per_lane = it == p->shapes.end() ? 0 : it->second;
+    return VNM_OK;
 }
 
 }  // extern "C"

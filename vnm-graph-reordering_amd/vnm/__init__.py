@@ -36,6 +36,7 @@ EXPORTS = [
     "vnm_nm_trace", "vnm_lp_model_create", "vnm_lp_model_destroy", "vnm_lp_model_get",
     "vnm_lp_features", "vnm_lp_forward", "vnm_lp_loss_grad", "vnm_lp_train_step",
     "vnm_pair_score", "vnm_lp_reorder", "vnm_hybrid_learned", "vnm_graph_clone",
+    "vnm_spmm_set_shape", "vnm_spmm_get_shape",
 ]
 
 LP_NPARAMS = 22784
@@ -182,6 +183,8 @@ def lib():
     L.vnm_hybrid_learned.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, P(HybridStats)]
     L.vnm_graph_clone.argtypes = [vp, P(vp)]
+    L.vnm_spmm_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.vnm_spmm_get_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p]
     _lib = L
     return L
 
@@ -547,6 +550,15 @@ class Packed:
 
     def spmm(self, B_ptr, ldb, n, b_dtype, C_ptr, ldc, b_scales_ptr=None):
         _check(lib().vnm_spmm(self.h, B_ptr, ldb, n, b_dtype, b_scales_ptr, C_ptr, ldc))
+
+    def set_shape(self, b_dtype, n, cols_per_lane):
+        """Fix the SpMM launch shape for (b_dtype, n); 0 re-enables tuning."""
+        _check(lib().vnm_spmm_set_shape(self.h, b_dtype, n, cols_per_lane))
+
+    def get_shape(self, b_dtype, n):
+        g = ctypes.c_int32()
+        _check(lib().vnm_spmm_get_shape(self.h, b_dtype, n, ctypes.byref(g)))
+        return g.value
 
     def close(self):
         if self.h:
