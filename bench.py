@@ -312,10 +312,12 @@ def main():
         torch.cuda.synchronize()
 
     traffic = None
+    vec32 = p32.get_shape(vnm.DT_F32, NCOLS)  # the launch shape the tuner chose
     tpath = os.path.join(ROOT, "profiles", "k_spmm_rmat65536_V128_traffic.json")
     if os.path.exists(tpath):
-        t = json.load(open(tpath))["k_spmm_f32"]
-        traffic = int(t["read_bytes"] + t["write_bytes"])  # rocprofv3 FETCH/WRITE_SIZE, calibrated
+        t = json.load(open(tpath)).get("k_spmm_f32", {}).get(f"cols_per_lane_{vec32}")
+        if t:  # rocprofv3 FETCH_SIZE / WRITE_SIZE per dispatch, calibrated
+            traffic = int(t["read_bytes"] + t["write_bytes"])
 
     res = {
         "metric": "V:N:M SpMM TFLOP/s (after GPU V:N:M reorder)",
@@ -335,7 +337,7 @@ def main():
                    "hybrid_rounds": a.rounds, "sched": 0, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_spmm", "kernel_ms": round(k_ms, 5),
+                     "kernel": f"k_spmm<f32, f32, {vec32}>", "kernel_ms": round(k_ms, 5),
                      "alg_bytes": int(alg_bytes), "a_packed_bytes": int(a_bytes)},
         "reorder": {"gpu_ms": round(reorder_wall_ms, 2), "hybrid_ms": round(hs.milliseconds, 2),
                     "greedy_kernel_ms": round(greedy_ms, 2), "greedy_launches": greedy_n,
