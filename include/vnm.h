@@ -226,6 +226,30 @@ int vnm_spmm(vnm_packed *p, const void *B, int ldb, int n, int b_dtype,
 int vnm_spmm_set_shape(vnm_packed *p, int b_dtype, int n, int cols_per_lane);
 int vnm_spmm_get_shape(const vnm_packed *p, int b_dtype, int n, int32_t *cols_per_lane);
 
+/* GCN aggregation layer (the consumer of gnn/single/pyg_gcn.py:41-74,
+ * aggr='add'): Y = A_vnm (X W) + bias, X: ncols x K BF16 (leading dimension
+ * ldx), W: K x N BF16 row-major, bias: N floats or NULL, Y: nrows x N FP32.
+ * X W runs on MFMA (FP32 accumulation, BF16 result kept with p), then the
+ * SpMM with the bias added in its row writes.  K % 16 == 0, N % 64 == 0. */
+int vnm_gcn_layer(vnm_packed *p, const uint16_t *X, int ldx, int K, const uint16_t *W, int N,
+                  const float *bias, float *Y, int ldy);
+
+/* Batched SpMM: C_g = A_g B_g (+ bias_g) for `count` packed matrices of one
+ * value dtype and uneven sizes, all with n output columns, in one persistent
+ * launch captured with its row pre-fill as a HIP graph.  Not in the
+ * reference.  cols_per_lane = 0 tunes the launch shape on the batch itself.
+ * The operand pointers are fixed at creation; vnm_batch_run replays the graph
+ * on the context stream.  b_scales and bias may be NULL (per-entry NULLs
+ * allowed for bias). */
+typedef struct vnm_batch vnm_batch;
+int  vnm_batch_create(vnm_ctx *ctx, vnm_packed *const *packed, int count, const void *const *B,
+                      const int32_t *ldb, int n, int b_dtype, const float *const *b_scales,
+                      float *const *C, const int32_t *ldc, const float *const *bias,
+                      int cols_per_lane, vnm_batch **out);
+int  vnm_batch_run(vnm_batch *b);
+int  vnm_batch_info(const vnm_batch *b, int32_t *cols_per_lane, int32_t *units, int32_t *zero_rows);
+void vnm_batch_destroy(vnm_batch *b);
+
 /* ---------------------------------------------------------------- learned proposer */
 
 /* A graph network that ranks candidate column pairs for the N:M reorder so

@@ -1,0 +1,349 @@
+// Batched V:N:M SpMM and the GCN aggregation layer (gfx950).
+//
+// Batch: many packed matrices of uneven size multiplied in one persistent
+// launch.  The work items (64 row entries x one column tile) of all matrices
+// form one list (prefix offsets per matrix); a fixed grid of 8 workgroups per
+// CU walks it, each XCD taking a contiguous range so neighbouring items of
+// one matrix share that XCD's L2.  A device table holds every matrix's
+// pointers (SpmmArgs).  The batch is tuned once (values per lane, timed on the
+// whole batch) and then captured as a HIP graph of two kernels -- the pre-fill
+// of rows that keep nothing or are cut by an item boundary, and the SpMM --
+// replayed by vnm_batch_run.  Pre-fill is a kernel, not a memset node.
+//
+// GCN layer (the consumer of gnn/single/pyg_gcn.py:41-74, aggr='add'):
+// Y = A_vnm (X W) + bias.  X W runs on MFMA (BF16 inputs, FP32 accumulation,
+// BF16 result into a workspace kept with the packed matrix), then the SpMM
+// with the bias folded into its row flush and pre-fill.
+#include "graph.h"
+#include "hip_common.h"
+#include "packed.h"
+
+#include <algorithm>
+#include <chrono>
+#include <vector>
+
+struct vnm_batch {
+    vnm_ctx *ctx = nullptr;
+    int count = 0, n = 0, bt = 0, at = 0, vec = 0, T = 1;
+    int total_units = 0, nzero = 0, grid = 0, per_xcd = 0, waves_per_xcd = 0;
+    vnm::DBuf<vnm::SpmmArgs> descs;
+    vnm::DBuf<int32_t> unit_off;
+    vnm::DBuf<int2> zero;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~vnm_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace vnm {
+
+template <int AT, int BT, int VEC>
+__global__ __launch_bounds__(256) void k_spmm_batch(const SpmmArgs *__restrict__ descs,
+                                                    const int32_t *__restrict__ unit_off, int count, int T,
+                                                    int total, int per_xcd, int waves_per_xcd) {
+    const int wv = threadIdx.x >> 6;
+    const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+    for (int u = slot * 4 + wv; u < per_xcd; u += waves_per_xcd) {
+        const int unit = xcd * per_xcd + u;
+        if (unit >= total) break;  // wave-uniform: every wave reaches the end
+        // matrix of this unit: the last g with unit_off[g] <= unit
+        int lo = 0, hi = count - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (unit_off[mid] <= unit) lo = mid;
+            else hi = mid - 1;
+        }
+        const int local = unit - unit_off[lo];
+        spmm_unit<AT, BT, VEC>(descs[lo], local / T, local % T);
+    }
+}
+
+// Pre-fill the listed (matrix, row) pairs with the matrix's bias or zero.
+__global__ void k_zero_batch(const int2 *__restrict__ list, int nz, int n, const SpmmArgs *__restrict__ descs) {
+    const int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (w >= nz) return;
+    const int lane = threadIdx.x & 63;
+    const int2 e = list[w];
+    const SpmmArgs &a = descs[e.x];
+    float *cp = a.C + (size_t)e.y * a.ldc;
+    for (int c = lane * 4; c < n; c += 256)
+        *(float4 *)(cp + c) = a.bias ? *(const float4 *)(a.bias + c) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void k_tag_rows(const int32_t *__restrict__ rows, int nz, int g, int2 *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nz) out[i] = make_int2(g, rows[i]);
+}
+
+template <int AT, int BT, int VEC>
+static void launch_batch_v(vnm_batch *b, hipStream_t st) {
+    hipLaunchKernelGGL((k_spmm_batch<AT, BT, VEC>), dim3(b->grid), dim3(256), 0, st, b->descs.p, b->unit_off.p,
+                       b->count, b->T, b->total_units, b->per_xcd, b->waves_per_xcd);
+}
+
+template <int AT>
+static void launch_batch_a(vnm_batch *b, hipStream_t st) {
+    const int v = b->vec;
+    if (b->bt == VNM_DT_F32) {
+        if (v == 4) launch_batch_v<AT, VNM_DT_F32, 4>(b, st);
+        else if (v == 2) launch_batch_v<AT, VNM_DT_F32, 2>(b, st);
+        else launch_batch_v<AT, VNM_DT_F32, 1>(b, st);
+    } else if (b->bt == VNM_DT_BF16) {
+        if (v == 8) launch_batch_v<AT, VNM_DT_BF16, 8>(b, st);
+        else if (v == 4) launch_batch_v<AT, VNM_DT_BF16, 4>(b, st);
+        else if (v == 2) launch_batch_v<AT, VNM_DT_BF16, 2>(b, st);
+        else launch_batch_v<AT, VNM_DT_BF16, 1>(b, st);
+    } else {
+        if (v == 16) launch_batch_v<AT, VNM_DT_FP8, 16>(b, st);
+        else if (v == 8) launch_batch_v<AT, VNM_DT_FP8, 8>(b, st);
+        else if (v == 4) launch_batch_v<AT, VNM_DT_FP8, 4>(b, st);
+        else launch_batch_v<AT, VNM_DT_FP8, 2>(b, st);
+    }
+}
+
+// Set the launch shape for `vec` values per lane: tile count, units, grid.
+static void batch_shape(vnm_batch *b, const std::vector<int> &items, int vec, std::vector<int32_t> &off) {
+    b->vec = vec;
+    b->T = b->n / (64 * vec);
+    off.assign(b->count + 1, 0);
+    for (int g = 0; g < b->count; ++g) off[g + 1] = off[g] + items[g] * b->T;
+    b->total_units = off[b->count];
+    int ncu = 256;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->ctx->device) != hipSuccess || ncu < 8)
+        ncu = 256;
+    b->grid = std::max(8, (ncu * 8) / 8 * 8);  // 8 workgroups of 4 waves per CU
+    b->per_xcd = (b->total_units + 7) / 8;
+    b->waves_per_xcd = (b->grid / 8) * 4;
+}
+
+static int batch_enqueue(vnm_batch *b, hipStream_t st) {
+    if (b->nzero > 0)
+        hipLaunchKernelGGL(k_zero_batch, dim3((b->nzero + 3) / 4), dim3(256), 0, st, b->zero.p, b->nzero, b->n,
+                           b->descs.p);
+    if (b->total_units > 0) {
+        switch (b->at) {
+            case VNM_DT_F32: launch_batch_a<VNM_DT_F32>(b, st); break;
+            case VNM_DT_BF16: launch_batch_a<VNM_DT_BF16>(b, st); break;
+            case VNM_DT_F16: launch_batch_a<VNM_DT_F16>(b, st); break;
+            default: launch_batch_a<VNM_DT_FP8>(b, st); break;
+        }
+    }
+    VNM_LAUNCH_CHECK("k_spmm_batch");
+    return VNM_OK;
+}
+
+static int batch_create(vnm_ctx *ctx, vnm_packed *const *ps, int count, const void *const *B, const int32_t *ldb,
+                        int n, int bt, const float *const *bs, float *const *C, const int32_t *ldc,
+                        const float *const *bias, int vec, vnm_batch **out) {
+    if (!ctx || !ps || count < 1 || !B || !ldb || !C || !ldc || !out) return set_error(VNM_E_ARG, "bad arguments");
+    if (n <= 0 || n % 64) return set_error(VNM_E_ARG, "batch: n must be a positive multiple of 64");
+    if (bt != VNM_DT_F32 && bt != VNM_DT_BF16 && bt != VNM_DT_FP8)
+        return set_error(VNM_E_ARG, "batch: B dtype must be F32, BF16 or FP8");
+    const int at = ps[0] ? ps[0]->dtype : 0;
+    for (int g = 0; g < count; ++g) {
+        const vnm_packed *p = ps[g];
+        if (!p || p->ctx != ctx) return set_error(VNM_E_ARG, "batch: matrix %d missing or on another context", g);
+        if (p->dtype != at) return set_error(VNM_E_ARG, "batch: every matrix needs the same value dtype");
+        if (!B[g] || !C[g] || ldb[g] < n || ldc[g] < n || (bt == VNM_DT_FP8 && (!bs || !bs[g])))
+            return set_error(VNM_E_ARG, "batch: operands of matrix %d", g);
+        if ((ldb[g] * bsize_of(bt)) % 16 || ldc[g] % 4 || ((uintptr_t)B[g] & 15) || ((uintptr_t)C[g] & 15) ||
+            (bias && bias[g] && ((uintptr_t)bias[g] & 15)))
+            return set_error(VNM_E_ARG, "batch: matrix %d operands must be 16-byte aligned", g);
+    }
+    if (vec && !vec_ok(bt, n, vec)) return set_error(VNM_E_ARG, "batch: %d values per lane do not fit n", vec);
+    hipStream_t st = ctx->stream;
+    auto b = std::make_unique<vnm_batch>();
+    b->ctx = ctx;
+    b->count = count;
+    b->n = n;
+    b->bt = bt;
+    b->at = at;
+    std::vector<SpmmArgs> d(count);
+    std::vector<int> items(count);
+    int rc;
+    std::vector<std::pair<ZeroList *, int>> zl;
+    int nz = 0;
+    for (int g = 0; g < count; ++g) {
+        d[g] = packed_args(ps[g], B[g], ldb[g], bs ? bs[g] : nullptr, C[g], ldc[g], bias ? bias[g] : nullptr);
+        items[g] = (ps[g]->nent + ITEM - 1) / ITEM;
+        ZeroList *z;
+        if ((rc = zero_list(ps[g], ITEM, &z))) return rc;
+        zl.push_back({z, nz});
+        nz += z->n;
+    }
+    b->nzero = nz;
+    if ((rc = b->descs.alloc(count)) || (rc = b->unit_off.alloc(count + 1)) || (rc = b->zero.alloc(std::max(nz, 1))))
+        return rc;
+    VNM_HIP_TRY(hipMemcpyAsync(b->descs.p, d.data(), count * sizeof(SpmmArgs), hipMemcpyHostToDevice, st));
+    for (int g = 0; g < count; ++g)
+        if (zl[g].first->n)
+            hipLaunchKernelGGL(k_tag_rows, dim3((zl[g].first->n + 255) / 256), dim3(256), 0, st, zl[g].first->rows.p,
+                               zl[g].first->n, g, b->zero.p + zl[g].second);
+    VNM_LAUNCH_CHECK("k_tag_rows");
+    std::vector<int32_t> off;
+    if (vec == 0) {
+        // tuner: each usable shape, one warm-up and three timed batches
+        hipEvent_t e0, e1;
+        VNM_HIP_TRY(hipEventCreate(&e0));
+        VNM_HIP_TRY(hipEventCreate(&e1));
+        float best = 1e30f;
+        for (int v = 16; v >= 1; v /= 2) {
+            if (!vec_ok(bt, n, v)) continue;
+            batch_shape(b.get(), items, v, off);
+            VNM_HIP_TRY(hipMemcpyAsync(b->unit_off.p, off.data(), (count + 1) * 4, hipMemcpyHostToDevice, st));
+            if ((rc = batch_enqueue(b.get(), st))) break;
+            (void)hipEventRecord(e0, st);
+            for (int i = 0; i < 3 && rc == VNM_OK; ++i) rc = batch_enqueue(b.get(), st);
+            (void)hipEventRecord(e1, st);
+            float ms = 0.f;
+            if (rc || hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                if (!rc) rc = set_error(VNM_E_HIP, "batch tuning timing failed");
+                break;
+            }
+            if (getenv("VNM_DEBUG")) fprintf(stderr, "[vnm] batch tune bt=%d n=%d vec=%d %.4f ms\n", bt, n, v, ms / 3);
+            if (ms < best) { best = ms; vec = v; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+    }
+    batch_shape(b.get(), items, vec, off);
+    VNM_HIP_TRY(hipMemcpyAsync(b->unit_off.p, off.data(), (count + 1) * 4, hipMemcpyHostToDevice, st));
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    // capture the two kernels once; vnm_batch_run replays them
+    VNM_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    rc = batch_enqueue(b.get(), st);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return set_error(VNM_E_HIP, "batch capture failed: %s", hipGetErrorString(e));
+    b->graph = graph;
+    e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return set_error(VNM_E_HIP, "batch graph instantiation failed: %s", hipGetErrorString(e));
+    ctx->launches += 2;
+    *out = b.release();
+    return VNM_OK;
+}
+
+// ------------------------------------------------------------ GCN layer
+
+// H[R x N] (BF16) = X[R x K] (BF16, leading dimension ldx) W[K x N] (BF16,
+// row-major); K a multiple of 16, N of 64.  A workgroup computes 64 rows x
+// 64 columns (four waves of 16 rows): W's 64-column panel is staged in LDS
+// transposed (128 k per stage), and each wave issues
+// v_mfma_f32_16x16x16_bf16 on fragments lane l = (row l%16, k 4(l/16)..+3)
+// of X and (k 4(l/16)..+3, column l%16) of W, accumulating FP32.
+constexpr int GK = 128;
+
+__global__ __launch_bounds__(256) void k_gemm_xw(int R, int K, int N, const uint16_t *__restrict__ X, int ldx,
+                                                 const uint16_t *__restrict__ W, uint16_t *__restrict__ H) {
+    typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+    typedef float f32x4_t __attribute__((ext_vector_type(4)));
+    __shared__ uint16_t w[64 * (GK + 4)];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n0 = blockIdx.y * 64;
+    const int r0 = (blockIdx.x * 4 + wv) * 16;
+    const int m = lane & 15, kq = (lane >> 4) * 4;
+    const int row = r0 + m;
+    f32x4_t acc[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < K; kc += GK) {
+        const int kn = min(GK, K - kc);
+        __syncthreads();
+        for (int i = threadIdx.x; i < 64 * kn; i += 256) {
+            const int k = i / 64, c = i % 64;  // coalesced over c
+            w[c * (GK + 4) + k] = W[(size_t)(kc + k) * N + n0 + c];
+        }
+        __syncthreads();
+        if (r0 < R) {
+            for (int ks = 0; ks < kn; ks += 16) {
+                bf16x4_t a = bf16x4_t{0, 0, 0, 0};
+                if (row < R) a = *(const bf16x4_t *)(X + (size_t)row * ldx + kc + ks + kq);
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) {
+                    const bf16x4_t bb = *(const bf16x4_t *)&w[(nt * 16 + m) * (GK + 4) + ks + kq];
+                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[nt], 0, 0, 0);
+                }
+            }
+        }
+    }
+    if (r0 >= R) return;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int rr = r0 + kq + i;
+            if (rr < R) H[(size_t)rr * N + n0 + nt * 16 + m] = f32_to_bf16(acc[nt][i]);
+        }
+}
+
+static int gcn_layer(vnm_packed *p, const uint16_t *X, int ldx, int K, const uint16_t *W, int N,
+                     const float *bias, float *Y, int ldy) {
+    if (!p || !X || !W || !Y) return set_error(VNM_E_ARG, "gcn: null argument");
+    if (K <= 0 || K % 16 || N <= 0 || N % 64 || ldx < K || ldx % 4 || ldy < N || ldy % 4)
+        return set_error(VNM_E_ARG, "gcn: K must be a multiple of 16, N of 64, ldx >= K and ldy >= N (multiples of 4)");
+    if (((uintptr_t)X & 7) || ((uintptr_t)Y & 15) || (bias && ((uintptr_t)bias & 15)))
+        return set_error(VNM_E_ARG, "gcn: X must be 8-byte and Y, bias 16-byte aligned");
+    const int R = p->ncols;  // X W has one row per column of A
+    hipStream_t st = p->ctx->stream;
+    int rc;
+    if (p->gcn_h.n < (size_t)R * N) {
+        p->gcn_h.release();
+        if ((rc = p->gcn_h.alloc((size_t)R * N))) return rc;
+    }
+    KernelTimer kt(p->ctx, "k_gemm_xw");
+    hipLaunchKernelGGL(k_gemm_xw, dim3((R + 63) / 64, N / 64), dim3(256), 0, st, R, K, N, X, ldx, W, p->gcn_h.p);
+    kt.stop();
+    p->ctx->launches++;
+    VNM_LAUNCH_CHECK("k_gemm_xw");
+    return spmm(p, p->gcn_h.p, N, N, VNM_DT_BF16, nullptr, Y, ldy, bias);
+}
+
+}  // namespace vnm
+
+using namespace vnm;
+
+extern "C" {
+
+int vnm_batch_create(vnm_ctx *ctx, vnm_packed *const *ps, int count, const void *const *B, const int32_t *ldb,
+                     int n, int b_dtype, const float *const *b_scales, float *const *C, const int32_t *ldc,
+                     const float *const *bias, int cols_per_lane, vnm_batch **out) {
+    return batch_create(ctx, ps, count, B, ldb, n, b_dtype, b_scales, C, ldc, bias, cols_per_lane, out);
+}
+
+int vnm_batch_run(vnm_batch *b) {
+    if (!b) return set_error(VNM_E_ARG, "null batch");
+    KernelTimer kt(b->ctx, "k_spmm_batch");
+    hipError_t e = hipGraphLaunch(b->exec, b->ctx->stream);
+    kt.stop();
+    if (e != hipSuccess) return set_error(VNM_E_HIP, "batch graph launch failed: %s", hipGetErrorString(e));
+    b->ctx->launches += 2;
+    return VNM_OK;
+}
+
+int vnm_batch_info(const vnm_batch *b, int32_t *cols_per_lane, int32_t *units, int32_t *zero_rows) {
+    if (!b) return set_error(VNM_E_ARG, "null batch");
+    if (cols_per_lane) *cols_per_lane = b->vec;
+    if (units) *units = b->total_units;
+    if (zero_rows) *zero_rows = b->nzero;
+    return VNM_OK;
+}
+
+void vnm_batch_destroy(vnm_batch *b) {
+    if (!b) return;
+    if (b->ctx && b->ctx->stream) (void)hipStreamSynchronize(b->ctx->stream);
+    delete b;
+}
+
+int vnm_gcn_layer(vnm_packed *p, const uint16_t *X, int ldx, int K, const uint16_t *W, int N, const float *bias,
+                  float *Y, int ldy) {
+    return gcn_layer(p, X, ldx, K, W, N, bias, Y, ldy);
+}
+
+}  // extern "C"

@@ -38,6 +38,7 @@
 // conversion instruction.
 #include "graph.h"
 #include "hip_common.h"
+#include "packed.h"
 #include "sortlib.h"
 
 #include <hip/hip_bf16.h>
@@ -50,28 +51,6 @@
 #include <vector>
 
 namespace vnm {
-struct ZeroList {
-    int n = 0;
-    DBuf<int32_t> rows;  // rows that keep nothing or are cut by a group boundary
-};
-}  // namespace vnm
-
-struct vnm_packed {
-    vnm_ctx *ctx = nullptr;
-    int32_t V = 0, M = 0, nrows = 0, ncols = 0, dtype = 0;
-    int32_t nblocks = 0, nent = 0;
-    int64_t retained = 0;
-    vnm::DBuf<int32_t> rowptr, ent_row, ent_col, ent_blk, blk_seg;
-    vnm::DBuf<uint8_t> ent_idx, ent_val;                   // values: 2 per entry, dtype-sized
-    vnm::DBuf<uint16_t> blk_sel;
-    vnm::DBuf<float> blk_scale;
-    std::map<int, std::unique_ptr<vnm::ZeroList>> zero;    // by entries per item
-    std::map<int, int> shapes;                             // (b dtype << 24 | n) -> values per lane
-};
-
-namespace vnm {
-
-constexpr int ITEM = 64;  // entries per work item
 
 static int grid_of(int64_t n, int block) {
     int64_t g = (n + block - 1) / block;
@@ -85,52 +64,6 @@ static int bits_for(int64_t x) {
 }
 
 // ------------------------------------------------------------ value codecs
-
-// OCP FP8 E4M3 (bias 7, no infinities, 0x7F = NaN, max 448): round to
-// nearest even, saturating to +-448.
-__host__ __device__ inline uint8_t f32_to_e4m3(float f) {
-    uint32_t u;
-    memcpy(&u, &f, 4);
-    const uint8_t sign = (uint8_t)((u >> 24) & 0x80u);
-    const uint32_t a = u & 0x7FFFFFFFu;
-    if (a > 0x7F800000u) return (uint8_t)(sign | 0x7F);  // NaN
-    float af;
-    memcpy(&af, &a, 4);
-    if (af >= 448.0f) return (uint8_t)(sign | 0x7E);      // saturate
-    const int e = (int)(a >> 23) - 127;                    // unbiased exponent
-    if (e < -6) {
-        // subnormal range: value = m * 2^-9, m in [0, 7]
-        float q = af * 512.0f;                             // exact scaling
-        // round half to even
-        float r = q - floorf(q);
-        uint32_t m = (uint32_t)floorf(q);
-        if (r > 0.5f || (r == 0.5f && (m & 1u))) m += 1;
-        return (uint8_t)(sign | m);                        // m == 8 -> smallest normal 0x08
-    }
-    // normal: keep 3 mantissa bits, round half to even on the 20 dropped bits
-    uint32_t mant = a & 0x7FFFFFu;
-    uint32_t keep = mant >> 20, rest = mant & 0xFFFFFu;
-    uint32_t exp8 = (uint32_t)(e + 7);
-    if (rest > 0x80000u || (rest == 0x80000u && (keep & 1u))) {
-        keep += 1;
-        if (keep == 8) { keep = 0; exp8 += 1; }
-    }
-    uint32_t code = (exp8 << 3) | keep;
-    if (code > 0x7E) code = 0x7E;
-    return (uint8_t)(sign | code);
-}
-
-__host__ __device__ inline float e4m3_to_f32(uint8_t c) {
-    const uint32_t s = c & 0x80u, e = (c >> 3) & 0xFu, m = c & 7u;
-    float v;
-    if (e == 0) v = (float)m * (1.0f / 512.0f);
-    else if (e == 15 && m == 7) v = __builtin_nanf("");
-    else {
-        uint32_t u = ((e + 120u) << 23) | (m << 20);
-        memcpy(&v, &u, 4);
-    }
-    return s ? -v : v;
-}
 
 // ------------------------------------------------------------ packer kernels
 
@@ -351,113 +284,11 @@ __global__ void k_zero_list(const int32_t *__restrict__ flag, const int32_t *__r
 
 // ------------------------------------------------------------ SpMM
 
-template <int DT>
-__device__ __forceinline__ float load_aval(const uint8_t *p, int i) {
-    if constexpr (DT == VNM_DT_F32) return ((const float *)p)[i];
-    else if constexpr (DT == VNM_DT_BF16) return bf16_to_f32(((const uint16_t *)p)[i]);
-    else if constexpr (DT == VNM_DT_F16) return __half2float(((const __half *)p)[i]);
-    else return e4m3_to_f32(p[i]);
-}
+// ------------------------------------------------------------ SpMM kernel
 
-template <int BT>
-constexpr int bsize() { return BT == VNM_DT_F32 ? 4 : BT == VNM_DT_BF16 ? 2 : 1; }
-
-typedef float f32x2_t __attribute__((ext_vector_type(2)));
-
-// VEC consecutive B values (VEC * sizeof(value) bytes, 2..16) of one row.
-template <int BT, int VEC>
-struct BRaw {
-    static constexpr int BYTES = VEC * bsize<BT>();
-    static constexpr int NW = BYTES < 4 ? 1 : BYTES / 4;
-    uint32_t w[NW];
-};
-
-template <int BT, int VEC>
-__device__ __forceinline__ BRaw<BT, VEC> load_raw(const uint8_t *p) {
-    BRaw<BT, VEC> r;
-    constexpr int BY = BRaw<BT, VEC>::BYTES;
-    if constexpr (BY == 16) {
-        const uint4 t = *(const uint4 *)p;
-        r.w[0] = t.x; r.w[1] = t.y; r.w[2] = t.z; r.w[3] = t.w;
-    } else if constexpr (BY == 8) {
-        const uint2 t = *(const uint2 *)p;
-        r.w[0] = t.x; r.w[1] = t.y;
-    } else if constexpr (BY == 4) {
-        r.w[0] = *(const uint32_t *)p;
-    } else {
-        r.w[0] = *(const uint16_t *)p;
-    }
-    return r;
-}
-
-template <int BT, int VEC>
-__device__ __forceinline__ BRaw<BT, VEC> zero_raw() {
-    BRaw<BT, VEC> r;
-#pragma unroll
-    for (int i = 0; i < BRaw<BT, VEC>::NW; ++i) r.w[i] = 0;
-    return r;
-}
-
-// acc += w * (decoded raw)
-template <int BT, int VEC>
-__device__ __forceinline__ void fma_raw(float (&acc)[VEC], float w, const BRaw<BT, VEC> &raw) {
-    if constexpr (BT == VNM_DT_F32) {
-#pragma unroll
-        for (int i = 0; i < VEC; ++i) acc[i] = fmaf(w, __uint_as_float(raw.w[i]), acc[i]);
-    } else if constexpr (BT == VNM_DT_BF16) {
-#pragma unroll
-        for (int i = 0; i < VEC; ++i) {
-            const uint32_t x = raw.w[i / 2];
-            acc[i] = fmaf(w, __uint_as_float((i & 1) ? (x & 0xFFFF0000u) : (x << 16)), acc[i]);
-        }
-    } else {
-        // OCP e4m3 through the gfx950 conversion, two values per call
-#pragma unroll
-        for (int i = 0; i < VEC; i += 4) {
-            const uint32_t x = raw.w[i / 4];
-            const f32x2_t lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)x, false);
-            acc[i] = fmaf(w, lo.x, acc[i]);
-            acc[i + 1] = fmaf(w, lo.y, acc[i + 1]);
-            if constexpr (VEC >= 4) {
-                const f32x2_t hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)x, true);
-                acc[i + 2] = fmaf(w, hi.x, acc[i + 2]);
-                acc[i + 3] = fmaf(w, hi.y, acc[i + 3]);
-            }
-        }
-    }
-}
-
-template <int NV>
-__device__ __forceinline__ void flush_vec(float *__restrict__ C, int ldc, int row, int col0,
-                                          const float (&acc)[NV], bool atomic) {
-    float *cp = C + (size_t)row * ldc + col0;
-    if (atomic) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) atomicAdd(&cp[i], acc[i]);
-    } else if constexpr (NV % 4 == 0) {
-#pragma unroll
-        for (int i = 0; i < NV; i += 4) *(float4 *)(cp + i) = make_float4(acc[i], acc[i + 1], acc[i + 2], acc[i + 3]);
-    } else if constexpr (NV == 2) {
-        *(float2 *)cp = make_float2(acc[0], acc[1]);
-    } else {
-        cp[0] = acc[0];
-    }
-}
-
-// C = A_vnm B for one column tile of 64 * VEC columns: a wave takes a work
-// item of 64 row entries, lane k resolves entry k (row, both B rows, scaled
-// values), then the wave walks the entries with the entry data broadcast
-// through scalar reads, each lane gathering VEC values of the B rows four
-// entries deep and owning VEC columns of the output row.
 template <int AT, int BT, int VEC>
-__global__ __launch_bounds__(256) void k_spmm(
-    int E, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ ent_row,
-    const int32_t *__restrict__ ent_col, const uint8_t *__restrict__ ent_idx,
-    const uint8_t *__restrict__ ent_val, const int32_t *__restrict__ ent_blk,
-    const float *__restrict__ blk_scale, int M, const uint8_t *__restrict__ B, int ldb,
-    const float *__restrict__ b_scales, float *__restrict__ C, int ldc, int T, int items, int per_g) {
-    constexpr int SB = bsize<BT>(), U = 4;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+__global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int T, int items, int per_g) {
+    const int wv = threadIdx.x >> 6;
     // workgroups are dealt round-robin over the 8 XCDs
     const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
     int tile, iwg;
@@ -472,80 +303,19 @@ __global__ __launch_bounds__(256) void k_spmm(
     }
     const int item = iwg * 4 + wv;
     if (item >= items) return;  // wave-uniform
-    const int beg = item * ITEM;
-    const int cnt = min(ITEM, E - beg);
-    int rr = 0, c0 = 0, c1 = -1;
-    float a0 = 0.f, a1 = 0.f;
-    if (lane < cnt) {
-        const int e = beg + lane;
-        rr = ent_row[e];
-        c0 = ent_col[e];
-        const uint32_t code = ent_idx[e];
-        float sc = 1.f;
-        if constexpr (AT == VNM_DT_FP8) sc = blk_scale[ent_blk[e]];
-        a0 = load_aval<AT>(ent_val, 2 * e) * sc;
-        if (code & 0x10u) {
-            c1 = (c0 & ~(M - 1)) | (int)(code & 15u);
-            a1 = load_aval<AT>(ent_val, 2 * e + 1) * sc;
-        }
-        if constexpr (BT == VNM_DT_FP8) {
-            a0 *= b_scales[c0 >> 7];
-            if (c1 >= 0) a1 *= b_scales[c1 >> 7];
-        }
-    }
-    const int first_row = __builtin_amdgcn_readlane(rr, 0);
-    const int last_row = __builtin_amdgcn_readlane(rr, (cnt - 1) & 63);
-    const int col0 = tile * (64 * VEC) + lane * VEC;
-    const uint8_t *bcol = B + (size_t)col0 * SB;
-    float acc[VEC];
-#pragma unroll
-    for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
-    int cur = first_row;
-    for (int k = 0; k < cnt; k += U) {
-        BRaw<BT, VEC> raw[U][2];
-        float w[U][2];
-        int rowk[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int kk = k + u;  // wave-uniform
-            const bool ok = kk < cnt;
-            const int ck0 = __builtin_amdgcn_readlane(c0, kk & 63);
-            const int ck1 = __builtin_amdgcn_readlane(c1, kk & 63);
-            // slots past the item's end repeat its last row, so they never
-            // look like a row change
-            rowk[u] = ok ? __builtin_amdgcn_readlane(rr, kk & 63) : last_row;
-            w[u][0] = ok ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a0), kk & 63)) : 0.f;
-            w[u][1] = (ok && ck1 >= 0) ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a1), kk & 63)) : 0.f;
-            raw[u][0] = ok ? load_raw<BT, VEC>(bcol + (size_t)ck0 * ldb * SB) : zero_raw<BT, VEC>();
-            raw[u][1] = (ok && ck1 >= 0) ? load_raw<BT, VEC>(bcol + (size_t)ck1 * ldb * SB) : zero_raw<BT, VEC>();
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (rowk[u] != cur) {  // wave-uniform: the row changed
-                const bool cut = (cur == first_row && rowptr[cur] < beg) ||
-                                 (cur == last_row && rowptr[cur + 1] > beg + cnt);
-                flush_vec<VEC>(C, ldc, cur, col0, acc, cut);
-#pragma unroll
-                for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
-                cur = rowk[u];
-            }
-            fma_raw<BT, VEC>(acc, w[u][0], raw[u][0]);
-            fma_raw<BT, VEC>(acc, w[u][1], raw[u][1]);
-        }
-    }
-    const bool cut = (cur == first_row && rowptr[cur] < beg) || (cur == last_row && rowptr[cur + 1] > beg + cnt);
-    flush_vec<VEC>(C, ldc, cur, col0, acc, cut);
+    spmm_unit<AT, BT, VEC>(a, item, tile);
 }
 
-// Zero the rows of a zero list (rows that keep nothing, rows cut by a group
-// boundary): one wave per row, float4 stores.
-__global__ void k_zero_rows(const int32_t *__restrict__ list, int nz, int n, float *__restrict__ C,
-                            int ldc) {
+// Pre-fill the rows of a zero list (rows that keep nothing, rows cut by an
+// item boundary) with the bias or zero: one wave per row, float4 stores.
+__global__ void k_zero_rows(const int32_t *__restrict__ list, int nz, int n, float *__restrict__ C, int ldc,
+                            const float *__restrict__ bias) {
     const int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (w >= nz) return;
     const int lane = threadIdx.x & 63;
     float *cp = C + (size_t)list[w] * ldc;
-    for (int c = lane * 4; c < n; c += 256) *(float4 *)(cp + c) = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int c = lane * 4; c < n; c += 256)
+        *(float4 *)(cp + c) = bias ? *(const float4 *)(bias + c) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // ------------------------------------------------------------ host
@@ -709,7 +479,7 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
 }
 
 // Rows to zero for groups of gsz entries, built on first use.
-static int zero_list(vnm_packed *p, int gsz, ZeroList **out) {
+int zero_list(vnm_packed *p, int gsz, ZeroList **out) {
     auto it = p->zero.find(gsz);
     if (it != p->zero.end()) {
         *out = it->second.get();
@@ -736,19 +506,40 @@ static int zero_list(vnm_packed *p, int gsz, ZeroList **out) {
     return VNM_OK;
 }
 
-static int bsize_of(int bt) { return bt == VNM_DT_F32 ? 4 : bt == VNM_DT_BF16 ? 2 : 1; }
+int bsize_of(int bt) { return bt == VNM_DT_F32 ? 4 : bt == VNM_DT_BF16 ? 2 : 1; }
 
 // Values per lane usable for (bt, n): 2 to 16 bytes per lane load, and the
 // 64 * vec column tile divides n.
-static bool vec_ok(int bt, int n, int vec) {
+bool vec_ok(int bt, int n, int vec) {
     if (vec != 1 && vec != 2 && vec != 4 && vec != 8 && vec != 16) return false;
     const int bytes = vec * bsize_of(bt);
     return bytes >= 2 && bytes <= 16 && n % (64 * vec) == 0;
 }
 
+SpmmArgs packed_args(const vnm_packed *p, const void *B, int ldb, const float *bs, float *C, int ldc,
+                     const float *bias) {
+    SpmmArgs a;
+    a.rowptr = p->rowptr.p;
+    a.ent_row = p->ent_row.p;
+    a.ent_col = p->ent_col.p;
+    a.ent_idx = p->ent_idx.p;
+    a.ent_val = p->ent_val.p;
+    a.ent_blk = p->ent_blk.p;
+    a.blk_scale = p->blk_scale.p;
+    a.B = (const uint8_t *)B;
+    a.b_scales = bs;
+    a.C = C;
+    a.bias = bias;
+    a.E = p->nent;
+    a.M = p->M;
+    a.ldb = ldb;
+    a.ldc = ldc;
+    return a;
+}
+
 template <int AT, int BT, int VEC>
 static void launch_v(vnm_packed *p, const void *B, int ldb, int n, const float *bs, float *C, int ldc,
-                     hipStream_t st) {
+                     const float *bias, hipStream_t st) {
     const int T = n / (64 * VEC);
     const int items = (p->nent + ITEM - 1) / ITEM;
     const int iwg = (items + 3) / 4;
@@ -759,47 +550,46 @@ static void launch_v(vnm_packed *p, const void *B, int ldb, int n, const float *
     } else {
         per_g = (iwg * T + 7) / 8;
     }
-    hipLaunchKernelGGL((k_spmm<AT, BT, VEC>), dim3(8 * per_g), dim3(256), 0, st, p->nent, p->rowptr.p,
-                       p->ent_row.p, p->ent_col.p, p->ent_idx.p, p->ent_val.p, p->ent_blk.p,
-                       p->blk_scale.p, p->M, (const uint8_t *)B, ldb, bs, C, ldc, T, items, per_g);
+    hipLaunchKernelGGL((k_spmm<AT, BT, VEC>), dim3(8 * per_g), dim3(256), 0, st, packed_args(p, B, ldb, bs, C, ldc, bias),
+                       T, items, per_g);
 }
 
 template <int AT>
 static void launch_a(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, const float *bs, float *C,
-                     int ldc, hipStream_t st) {
+                     int ldc, const float *bias, hipStream_t st) {
     if (bt == VNM_DT_F32) {
-        if (vec == 4) launch_v<AT, VNM_DT_F32, 4>(p, B, ldb, n, bs, C, ldc, st);
-        else if (vec == 2) launch_v<AT, VNM_DT_F32, 2>(p, B, ldb, n, bs, C, ldc, st);
-        else launch_v<AT, VNM_DT_F32, 1>(p, B, ldb, n, bs, C, ldc, st);
+        if (vec == 4) launch_v<AT, VNM_DT_F32, 4>(p, B, ldb, n, bs, C, ldc, bias, st);
+        else if (vec == 2) launch_v<AT, VNM_DT_F32, 2>(p, B, ldb, n, bs, C, ldc, bias, st);
+        else launch_v<AT, VNM_DT_F32, 1>(p, B, ldb, n, bs, C, ldc, bias, st);
     } else if (bt == VNM_DT_BF16) {
-        if (vec == 8) launch_v<AT, VNM_DT_BF16, 8>(p, B, ldb, n, bs, C, ldc, st);
-        else if (vec == 4) launch_v<AT, VNM_DT_BF16, 4>(p, B, ldb, n, bs, C, ldc, st);
-        else if (vec == 2) launch_v<AT, VNM_DT_BF16, 2>(p, B, ldb, n, bs, C, ldc, st);
-        else launch_v<AT, VNM_DT_BF16, 1>(p, B, ldb, n, bs, C, ldc, st);
+        if (vec == 8) launch_v<AT, VNM_DT_BF16, 8>(p, B, ldb, n, bs, C, ldc, bias, st);
+        else if (vec == 4) launch_v<AT, VNM_DT_BF16, 4>(p, B, ldb, n, bs, C, ldc, bias, st);
+        else if (vec == 2) launch_v<AT, VNM_DT_BF16, 2>(p, B, ldb, n, bs, C, ldc, bias, st);
+        else launch_v<AT, VNM_DT_BF16, 1>(p, B, ldb, n, bs, C, ldc, bias, st);
     } else {
-        if (vec == 16) launch_v<AT, VNM_DT_FP8, 16>(p, B, ldb, n, bs, C, ldc, st);
-        else if (vec == 8) launch_v<AT, VNM_DT_FP8, 8>(p, B, ldb, n, bs, C, ldc, st);
-        else if (vec == 4) launch_v<AT, VNM_DT_FP8, 4>(p, B, ldb, n, bs, C, ldc, st);
-        else launch_v<AT, VNM_DT_FP8, 2>(p, B, ldb, n, bs, C, ldc, st);
+        if (vec == 16) launch_v<AT, VNM_DT_FP8, 16>(p, B, ldb, n, bs, C, ldc, bias, st);
+        else if (vec == 8) launch_v<AT, VNM_DT_FP8, 8>(p, B, ldb, n, bs, C, ldc, bias, st);
+        else if (vec == 4) launch_v<AT, VNM_DT_FP8, 4>(p, B, ldb, n, bs, C, ldc, bias, st);
+        else launch_v<AT, VNM_DT_FP8, 2>(p, B, ldb, n, bs, C, ldc, bias, st);
     }
 }
 
 // One SpMM with `vec` values per lane: zero the listed rows, then the gather kernel.
 static int spmm_v(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, const float *bs, float *C,
-                  int ldc, bool timed) {
+                  int ldc, const float *bias, bool timed) {
     hipStream_t st = p->ctx->stream;
     ZeroList *z;
     int rc = zero_list(p, ITEM, &z);
     if (rc) return rc;
     if (z->n > 0)
-        hipLaunchKernelGGL(k_zero_rows, dim3((z->n + 3) / 4), dim3(256), 0, st, z->rows.p, z->n, n, C, ldc);
+        hipLaunchKernelGGL(k_zero_rows, dim3((z->n + 3) / 4), dim3(256), 0, st, z->rows.p, z->n, n, C, ldc, bias);
     if (p->nent > 0) {
         KernelTimer kt(timed ? p->ctx : nullptr, "k_spmm");
         switch (p->dtype) {
-            case VNM_DT_F32: launch_a<VNM_DT_F32>(p, vec, B, ldb, n, bt, bs, C, ldc, st); break;
-            case VNM_DT_BF16: launch_a<VNM_DT_BF16>(p, vec, B, ldb, n, bt, bs, C, ldc, st); break;
-            case VNM_DT_F16: launch_a<VNM_DT_F16>(p, vec, B, ldb, n, bt, bs, C, ldc, st); break;
-            default: launch_a<VNM_DT_FP8>(p, vec, B, ldb, n, bt, bs, C, ldc, st); break;
+            case VNM_DT_F32: launch_a<VNM_DT_F32>(p, vec, B, ldb, n, bt, bs, C, ldc, bias, st); break;
+            case VNM_DT_BF16: launch_a<VNM_DT_BF16>(p, vec, B, ldb, n, bt, bs, C, ldc, bias, st); break;
+            case VNM_DT_F16: launch_a<VNM_DT_F16>(p, vec, B, ldb, n, bt, bs, C, ldc, bias, st); break;
+            default: launch_a<VNM_DT_FP8>(p, vec, B, ldb, n, bt, bs, C, ldc, bias, st); break;
         }
         kt.stop();
     }
@@ -814,7 +604,7 @@ static int shape_key(int bt, int n) { return (bt << 24) | n; }
 // shape (one warm-up and three timed calls each, HIP events on the context
 // stream) and keep the fastest; the choice stays with the packed matrix.
 static int tune_vec(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc,
-                    int *best_v) {
+                    const float *bias, int *best_v) {
     hipStream_t st = p->ctx->stream;
     hipEvent_t e0, e1;
     VNM_HIP_TRY(hipEventCreate(&e0));
@@ -823,9 +613,9 @@ static int tune_vec(vnm_packed *p, const void *B, int ldb, int n, int bt, const 
     int bv = 0, rc = VNM_OK;
     for (int v = 16; v >= 1 && rc == VNM_OK; v /= 2) {
         if (!vec_ok(bt, n, v)) continue;
-        if ((rc = spmm_v(p, v, B, ldb, n, bt, bs, C, ldc, false))) break;
+        if ((rc = spmm_v(p, v, B, ldb, n, bt, bs, C, ldc, bias, false))) break;
         if (hipEventRecord(e0, st) != hipSuccess) { rc = set_error(VNM_E_HIP, "event record failed"); break; }
-        for (int i = 0; i < 3 && rc == VNM_OK; ++i) rc = spmm_v(p, v, B, ldb, n, bt, bs, C, ldc, false);
+        for (int i = 0; i < 3 && rc == VNM_OK; ++i) rc = spmm_v(p, v, B, ldb, n, bt, bs, C, ldc, bias, false);
         if (rc) break;
         float ms = 0.f;
         if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
@@ -843,7 +633,8 @@ static int tune_vec(vnm_packed *p, const void *B, int ldb, int n, int bt, const 
     return VNM_OK;
 }
 
-int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc) {
+int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc,
+         const float *bias) {
     if (!p || !B || !C || n <= 0 || n % 64 != 0 || ldb < n || ldc < n)
         return set_error(VNM_E_ARG, "spmm: n must be a positive multiple of 64 with ldb, ldc >= n");
     if (bt != VNM_DT_F32 && bt != VNM_DT_BF16 && bt != VNM_DT_FP8)
@@ -858,11 +649,11 @@ int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, 
     if (it != p->shapes.end()) {
         v = it->second;
     } else {
-        int rc = tune_vec(p, B, ldb, n, bt, bs, C, ldc, &v);
+        int rc = tune_vec(p, B, ldb, n, bt, bs, C, ldc, bias, &v);
         if (rc) return rc;
         p->shapes[shape_key(bt, n)] = v;
     }
-    return spmm_v(p, v, B, ldb, n, bt, bs, C, ldc, true);
+    return spmm_v(p, v, B, ldb, n, bt, bs, C, ldc, bias, true);
 }
 
 }  // namespace vnm
@@ -906,7 +697,7 @@ void vnm_packed_destroy(vnm_packed *p) {
 
 int vnm_spmm(vnm_packed *p, const void *B, int ldb, int n, int b_dtype, const float *b_scales,
              float *C, int ldc) {
-    return spmm(p, B, ldb, n, b_dtype, b_scales, C, ldc);
+    return spmm(p, B, ldb, n, b_dtype, b_scales, C, ldc, nullptr);
 }
 
 int vnm_spmm_set_shape(vnm_packed *p, int b_dtype, int n, int cols_per_lane) {

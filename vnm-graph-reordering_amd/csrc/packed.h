@@ -1,0 +1,290 @@
+// The packed V:N:M format and the SpMM device code shared by the single and
+// batched SpMM kernels (pack.hip, batch.hip).  Format and kernel design are
+// described in pack.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+
+#include "graph.h"
+#include "hip_common.h"
+
+namespace vnm {
+struct ZeroList {
+    int n = 0;
+    DBuf<int32_t> rows;  // rows that keep nothing or are cut by a group boundary
+};
+}  // namespace vnm
+
+struct vnm_packed {
+    vnm_ctx *ctx = nullptr;
+    int32_t V = 0, M = 0, nrows = 0, ncols = 0, dtype = 0;
+    int32_t nblocks = 0, nent = 0;
+    int64_t retained = 0;
+    vnm::DBuf<int32_t> rowptr, ent_row, ent_col, ent_blk, blk_seg;
+    vnm::DBuf<uint8_t> ent_idx, ent_val;                   // values: 2 per entry, dtype-sized
+    vnm::DBuf<uint16_t> blk_sel;
+    vnm::DBuf<float> blk_scale;
+    std::map<int, std::unique_ptr<vnm::ZeroList>> zero;    // by entries per item
+    std::map<int, int> shapes;                             // (b dtype << 24 | n) -> values per lane
+    vnm::DBuf<uint16_t> gcn_h;                             // X W of the last GCN layer (BF16)
+};
+
+namespace vnm {
+
+constexpr int ITEM = 64;  // row entries per work item
+
+// ------------------------------------------------------------ value codecs
+
+// OCP FP8 E4M3 (bias 7, no infinities, 0x7F = NaN, max 448): round to
+// nearest even, saturating to +-448.
+__host__ __device__ inline uint8_t f32_to_e4m3(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    const uint8_t sign = (uint8_t)((u >> 24) & 0x80u);
+    const uint32_t a = u & 0x7FFFFFFFu;
+    if (a > 0x7F800000u) return (uint8_t)(sign | 0x7F);  // NaN
+    float af;
+    memcpy(&af, &a, 4);
+    if (af >= 448.0f) return (uint8_t)(sign | 0x7E);      // saturate
+    const int e = (int)(a >> 23) - 127;                    // unbiased exponent
+    if (e < -6) {
+        // subnormal range: value = m * 2^-9, m in [0, 7]
+        float q = af * 512.0f;                             // exact scaling
+        // round half to even
+        float r = q - floorf(q);
+        uint32_t m = (uint32_t)floorf(q);
+        if (r > 0.5f || (r == 0.5f && (m & 1u))) m += 1;
+        return (uint8_t)(sign | m);                        // m == 8 -> smallest normal 0x08
+    }
+    // normal: keep 3 mantissa bits, round half to even on the 20 dropped bits
+    uint32_t mant = a & 0x7FFFFFu;
+    uint32_t keep = mant >> 20, rest = mant & 0xFFFFFu;
+    uint32_t exp8 = (uint32_t)(e + 7);
+    if (rest > 0x80000u || (rest == 0x80000u && (keep & 1u))) {
+        keep += 1;
+        if (keep == 8) { keep = 0; exp8 += 1; }
+    }
+    uint32_t code = (exp8 << 3) | keep;
+    if (code > 0x7E) code = 0x7E;
+    return (uint8_t)(sign | code);
+}
+
+__host__ __device__ inline float e4m3_to_f32(uint8_t c) {
+    const uint32_t s = c & 0x80u, e = (c >> 3) & 0xFu, m = c & 7u;
+    float v;
+    if (e == 0) v = (float)m * (1.0f / 512.0f);
+    else if (e == 15 && m == 7) v = __builtin_nanf("");
+    else {
+        uint32_t u = ((e + 120u) << 23) | (m << 20);
+        memcpy(&v, &u, 4);
+    }
+    return s ? -v : v;
+}
+
+// ------------------------------------------------------------ SpMM device code
+
+template <int DT>
+__device__ __forceinline__ float load_aval(const uint8_t *p, int i) {
+    if constexpr (DT == VNM_DT_F32) return ((const float *)p)[i];
+    else if constexpr (DT == VNM_DT_BF16) return bf16_to_f32(((const uint16_t *)p)[i]);
+    else if constexpr (DT == VNM_DT_F16) return __half2float(((const __half *)p)[i]);
+    else return e4m3_to_f32(p[i]);
+}
+
+template <int BT>
+constexpr int bsize() { return BT == VNM_DT_F32 ? 4 : BT == VNM_DT_BF16 ? 2 : 1; }
+
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+// VEC consecutive B values (VEC * sizeof(value) bytes, 2..16) of one row.
+template <int BT, int VEC>
+struct BRaw {
+    static constexpr int BYTES = VEC * bsize<BT>();
+    static constexpr int NW = BYTES < 4 ? 1 : BYTES / 4;
+    uint32_t w[NW];
+};
+
+template <int BT, int VEC>
+__device__ __forceinline__ BRaw<BT, VEC> load_raw(const uint8_t *p) {
+    BRaw<BT, VEC> r;
+    constexpr int BY = BRaw<BT, VEC>::BYTES;
+    if constexpr (BY == 16) {
+        const uint4 t = *(const uint4 *)p;
+        r.w[0] = t.x; r.w[1] = t.y; r.w[2] = t.z; r.w[3] = t.w;
+    } else if constexpr (BY == 8) {
+        const uint2 t = *(const uint2 *)p;
+        r.w[0] = t.x; r.w[1] = t.y;
+    } else if constexpr (BY == 4) {
+        r.w[0] = *(const uint32_t *)p;
+    } else {
+        r.w[0] = *(const uint16_t *)p;
+    }
+    return r;
+}
+
+template <int BT, int VEC>
+__device__ __forceinline__ BRaw<BT, VEC> zero_raw() {
+    BRaw<BT, VEC> r;
+#pragma unroll
+    for (int i = 0; i < BRaw<BT, VEC>::NW; ++i) r.w[i] = 0;
+    return r;
+}
+
+// acc += w * (decoded raw)
+template <int BT, int VEC>
+__device__ __forceinline__ void fma_raw(float (&acc)[VEC], float w, const BRaw<BT, VEC> &raw) {
+    if constexpr (BT == VNM_DT_F32) {
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) acc[i] = fmaf(w, __uint_as_float(raw.w[i]), acc[i]);
+    } else if constexpr (BT == VNM_DT_BF16) {
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) {
+            const uint32_t x = raw.w[i / 2];
+            acc[i] = fmaf(w, __uint_as_float((i & 1) ? (x & 0xFFFF0000u) : (x << 16)), acc[i]);
+        }
+    } else {
+        // OCP e4m3 through the gfx950 conversion, two values per call
+#pragma unroll
+        for (int i = 0; i < VEC; i += 4) {
+            const uint32_t x = raw.w[i / 4];
+            const f32x2_t lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)x, false);
+            acc[i] = fmaf(w, lo.x, acc[i]);
+            acc[i + 1] = fmaf(w, lo.y, acc[i + 1]);
+            if constexpr (VEC >= 4) {
+                const f32x2_t hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)x, true);
+                acc[i + 2] = fmaf(w, hi.x, acc[i + 2]);
+                acc[i + 3] = fmaf(w, hi.y, acc[i + 3]);
+            }
+        }
+    }
+}
+
+// Write a finished output row slice (plus the bias, when given), or add a
+// partial one atomically (the row was pre-filled with the bias or zero).
+template <int NV>
+__device__ __forceinline__ void flush_vec(float *__restrict__ C, int ldc, int row, int col0,
+                                          const float (&acc)[NV], bool atomic, const float *__restrict__ bias) {
+    float *cp = C + (size_t)row * ldc + col0;
+    if (atomic) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) atomicAdd(&cp[i], acc[i]);
+        return;
+    }
+    float v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = bias ? acc[i] + bias[col0 + i] : acc[i];
+    if constexpr (NV % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; i += 4) *(float4 *)(cp + i) = make_float4(v[i], v[i + 1], v[i + 2], v[i + 3]);
+    } else if constexpr (NV == 2) {
+        *(float2 *)cp = make_float2(v[0], v[1]);
+    } else {
+        cp[0] = v[0];
+    }
+}
+
+// One packed matrix as the SpMM kernels see it.
+struct SpmmArgs {
+    const int32_t *rowptr, *ent_row, *ent_col;
+    const uint8_t *ent_idx, *ent_val;
+    const int32_t *ent_blk;
+    const float *blk_scale;
+    const uint8_t *B;
+    const float *b_scales;
+    float *C;
+    const float *bias;  // nullptr or n floats added to every output row
+    int E, M, ldb, ldc;
+};
+
+// C = A_vnm B for work item `item` and column tile `tile` (64 * VEC
+// columns), one wave: lane k resolves entry k of the item (row, both B rows,
+// scaled values), then the wave walks the entries with the entry data
+// broadcast through scalar reads, each lane gathering VEC values of the B
+// rows four entries deep and owning VEC columns of the output row.
+template <int AT, int BT, int VEC>
+__device__ __forceinline__ void spmm_unit(const SpmmArgs &a, int item, int tile) {
+    constexpr int SB = bsize<BT>(), U = 4;
+    const int lane = threadIdx.x & 63;
+    const int beg = item * ITEM;
+    const int cnt = min(ITEM, a.E - beg);
+    const int32_t *__restrict__ rowptr = a.rowptr;
+    int rr = 0, c0 = 0, c1 = -1;
+    float a0 = 0.f, a1 = 0.f;
+    if (lane < cnt) {
+        const int e = beg + lane;
+        rr = a.ent_row[e];
+        c0 = a.ent_col[e];
+        const uint32_t code = a.ent_idx[e];
+        float sc = 1.f;
+        if constexpr (AT == VNM_DT_FP8) sc = a.blk_scale[a.ent_blk[e]];
+        a0 = load_aval<AT>(a.ent_val, 2 * e) * sc;
+        if (code & 0x10u) {
+            c1 = (c0 & ~(a.M - 1)) | (int)(code & 15u);
+            a1 = load_aval<AT>(a.ent_val, 2 * e + 1) * sc;
+        }
+        if constexpr (BT == VNM_DT_FP8) {
+            a0 *= a.b_scales[c0 >> 7];
+            if (c1 >= 0) a1 *= a.b_scales[c1 >> 7];
+        }
+    }
+    const int first_row = __builtin_amdgcn_readlane(rr, 0);
+    const int last_row = __builtin_amdgcn_readlane(rr, (cnt - 1) & 63);
+    const int col0 = tile * (64 * VEC) + lane * VEC;
+    const uint8_t *bcol = a.B + (size_t)col0 * SB;
+    const int ldb = a.ldb;
+    float acc[VEC];
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
+    int cur = first_row;
+    for (int k = 0; k < cnt; k += U) {
+        BRaw<BT, VEC> raw[U][2];
+        float w[U][2];
+        int rowk[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k + u;  // wave-uniform
+            const bool ok = kk < cnt;
+            const int ck0 = __builtin_amdgcn_readlane(c0, kk & 63);
+            const int ck1 = __builtin_amdgcn_readlane(c1, kk & 63);
+            // slots past the item's end repeat its last row, so they never
+            // look like a row change
+            rowk[u] = ok ? __builtin_amdgcn_readlane(rr, kk & 63) : last_row;
+            w[u][0] = ok ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a0), kk & 63)) : 0.f;
+            w[u][1] = (ok && ck1 >= 0) ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a1), kk & 63)) : 0.f;
+            raw[u][0] = ok ? load_raw<BT, VEC>(bcol + (size_t)ck0 * ldb * SB) : zero_raw<BT, VEC>();
+            raw[u][1] = (ok && ck1 >= 0) ? load_raw<BT, VEC>(bcol + (size_t)ck1 * ldb * SB) : zero_raw<BT, VEC>();
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (rowk[u] != cur) {  // wave-uniform: the row changed
+                const bool cut = (cur == first_row && rowptr[cur] < beg) ||
+                                 (cur == last_row && rowptr[cur + 1] > beg + cnt);
+                flush_vec<VEC>(a.C, a.ldc, cur, col0, acc, cut, a.bias);
+#pragma unroll
+                for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
+                cur = rowk[u];
+            }
+            fma_raw<BT, VEC>(acc, w[u][0], raw[u][0]);
+            fma_raw<BT, VEC>(acc, w[u][1], raw[u][1]);
+        }
+    }
+    const bool cut = (cur == first_row && rowptr[cur] < beg) || (cur == last_row && rowptr[cur + 1] > beg + cnt);
+    flush_vec<VEC>(a.C, a.ldc, cur, col0, acc, cut, a.bias);
+}
+
+// ------------------------------------------------------------ host helpers (pack.hip)
+
+int zero_list(vnm_packed *p, int gsz, ZeroList **out);
+int bsize_of(int bt);
+bool vec_ok(int bt, int n, int vec);
+SpmmArgs packed_args(const vnm_packed *p, const void *B, int ldb, const float *bs, float *C, int ldc,
+                     const float *bias);
+int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc,
+         const float *bias);
+
+}  // namespace vnm

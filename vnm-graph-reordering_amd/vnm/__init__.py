@@ -36,7 +36,8 @@ EXPORTS = [
     "vnm_nm_trace", "vnm_lp_model_create", "vnm_lp_model_destroy", "vnm_lp_model_get",
     "vnm_lp_features", "vnm_lp_forward", "vnm_lp_loss_grad", "vnm_lp_train_step",
     "vnm_pair_score", "vnm_lp_reorder", "vnm_hybrid_learned", "vnm_graph_clone",
-    "vnm_spmm_set_shape", "vnm_spmm_get_shape",
+    "vnm_spmm_set_shape", "vnm_spmm_get_shape", "vnm_gcn_layer", "vnm_batch_create",
+    "vnm_batch_run", "vnm_batch_info", "vnm_batch_destroy",
 ]
 
 LP_NPARAMS = 22784
@@ -184,6 +185,13 @@ def lib():
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, P(HybridStats)]
     L.vnm_graph_clone.argtypes = [vp, P(vp)]
     L.vnm_spmm_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.vnm_gcn_layer.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int]
+    L.vnm_batch_create.argtypes = [vp, P(vp), ctypes.c_int, P(vp), i32p, ctypes.c_int, ctypes.c_int, P(vp),
+                                   P(vp), i32p, P(vp), ctypes.c_int, P(vp)]
+    L.vnm_batch_run.argtypes = [vp]
+    L.vnm_batch_info.argtypes = [vp, i32p, i32p, i32p]
+    L.vnm_batch_destroy.argtypes = [vp]
+    L.vnm_batch_destroy.restype = None
     L.vnm_spmm_get_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p]
     _lib = L
     return L
@@ -560,11 +568,57 @@ class Packed:
         _check(lib().vnm_spmm_get_shape(self.h, b_dtype, n, ctypes.byref(g)))
         return g.value
 
+    def gcn_layer(self, X_ptr, ldx, K, W_ptr, N, Y_ptr, ldy, bias_ptr=None):
+        """Y = A_vnm (X W) + bias (X, W BF16 device arrays; Y FP32)."""
+        _check(lib().vnm_gcn_layer(self.h, X_ptr, ldx, K, W_ptr, N, bias_ptr, Y_ptr, ldy))
+
     def close(self):
         if self.h:
             lib().vnm_packed_destroy(self.h)
             self.h = None
             self.graph.ctx._release(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Batched SpMM over packed matrices (one persistent launch, HIP graph).
+    B, C, scales and bias are lists of device pointers (ints) per matrix."""
+
+    def __init__(self, ctx, packed, B, ldb, n, b_dtype, C, ldc, b_scales=None, bias=None, cols_per_lane=0):
+        k = len(packed)
+        VP = ctypes.c_void_p * k
+        self._keep = packed
+        ps = VP(*[p.h.value if isinstance(p.h, ctypes.c_void_p) else p.h for p in packed])
+        bb, cc = VP(*B), VP(*C)
+        lb = np.ascontiguousarray(ldb, np.int32)
+        lc = np.ascontiguousarray(ldc, np.int32)
+        sc = VP(*b_scales) if b_scales is not None else None
+        bi = VP(*[x or 0 for x in bias]) if bias is not None else None
+        h = ctypes.c_void_p()
+        _check(lib().vnm_batch_create(ctx.h, ps, k, bb, _p(lb, ctypes.c_int32), n, b_dtype, sc, cc,
+                                      _p(lc, ctypes.c_int32), bi, cols_per_lane, ctypes.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        ctx._adopt(self)
+
+    def run(self):
+        _check(lib().vnm_batch_run(self.h))
+
+    def info(self):
+        v, u, z = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().vnm_batch_info(self.h, ctypes.byref(v), ctypes.byref(u), ctypes.byref(z)))
+        return {"cols_per_lane": v.value, "units": u.value, "zero_rows": z.value}
+
+    def close(self):
+        if self.h:
+            lib().vnm_batch_destroy(self.h)
+            self.h = None
+            self.ctx._release(self)
 
     def __del__(self):
         try:
