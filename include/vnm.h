@@ -136,8 +136,20 @@ typedef struct {
 /* Replaces reorderUtil::run_onepair / _double / _reverse / _reverse_double
  * (sched 0..3, reorder/reorder.h:1805-2474) and run (sched >= 8, :963-1100),
  * called as main.cu:19-53 does (maxiter = 10).  The whole greedy runs on the
- * GPU; the stats are the reference's return vector. */
+ * GPU; the stats are the reference's return vector.  sched 4..7 (M = 4) are
+ * the allpairs schedules as vnm_nm_reorder_allpairs with clean = 0. */
 int vnm_nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *st);
+
+/* Replaces run_allpairs / _double / _reverse / _reverse_double (sched 4..7,
+ * M = 4, reorder/reorder.h:1102-1803; unreachable from the reference CLI):
+ * single and double column swaps scored by GetSegSwapScoreAll (:133-260).
+ * clean = 0 reproduces the reference's visited filter, which never drops
+ * single swaps (:1194-1197), and its pairtab double swaps (moves 33..35 apply
+ * a different pair than they score, reorder/swappair.h:54-97); when that
+ * filter lets a node be swapped twice the reference relabel is no longer a
+ * permutation and the call returns VNM_E_UB_PATH.  clean = 1 drops every move
+ * touching a swapped node and applies the move that was scored. */
+int vnm_nm_reorder_allpairs(vnm_graph *g, int N, int sched, int maxiter, int clean, vnm_nm_stats *st);
 
 /* vnm_nm_reorder with the reference's swap decisions recorded (what the
  * verbose swap print of run_onepair shows, reorder/reorder.h:1913-1918, as

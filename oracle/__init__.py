@@ -91,6 +91,9 @@ def lib():
         L.orc_reload.argtypes = [P(Coo), ctypes.c_int]
         L.orc_nm_reorder_sample.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, i32p]
+        L.orc_allpairs_reorder.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           i32p, P(ctypes.c_int64)]
+        L.orc_swap_score_all.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p]
         f64p = P(ctypes.c_double)
         L.orc_lp_features.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, f64p]
         L.orc_lp_loss_grad.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, f64p, i32p, f64p,
@@ -315,3 +318,21 @@ def lp_loss_grad(m, N, M, params, pairs, target, want_grad=True):
                                   _ptr(grad, ctypes.c_double) if want_grad else None,
                                   _ptr(head, ctypes.c_double)), "lp_loss_grad")
     return loss.value, grad, head
+
+
+def allpairs_reorder(m, N, sched, maxiter=10, clean=False):
+    """Allpairs schedule 4..7 (M = 4), in place; returns (stats, overwrites)."""
+    st = np.zeros(5, np.int32)
+    ow = ctypes.c_int64(0)
+    c = m.c()
+    _check(lib().orc_allpairs_reorder(ctypes.byref(c), N, sched, maxiter, int(clean),
+                                      _ptr(st, ctypes.c_int32), ctypes.byref(ow)), "allpairs_reorder")
+    m.sync(c)
+    return st, ow.value
+
+
+def swap_score_all(m, N, s1, s2):
+    out = np.zeros(104, np.int32)
+    _check(lib().orc_swap_score_all(ctypes.byref(m.c()), N, s1, s2, _ptr(out, ctypes.c_int32)),
+           "swap_score_all")
+    return out

@@ -80,6 +80,14 @@ int orc_nm_reorder(orc_coo *coo, int N, int M, int sched, int maxiter,
                    int32_t *stats, orc_swap_rec *trace, int64_t trace_cap,
                    int64_t *trace_len);
 
+/* Allpairs schedules 4..7, M = 4 (reorder/reorder.h:1102-1803): clean = 0
+ * is the reference as written, clean = 1 filters every move by visited
+ * nodes and applies the double swap that was scored. */
+int orc_allpairs_reorder(orc_coo *coo, int N, int sched, int maxiter, int clean,
+                         int32_t *stats, int64_t *overwrites);
+/* GetSegSwapScoreAll (reorder/reorder.h:133-260): 52 {lgain, rgain}. */
+int orc_swap_score_all(const orc_coo *coo, int N, int seg1, int seg2, int32_t *score);
+
 /* ---- VENOM (reordering.py) */
 int orc_venom_eval(const orc_coo *coo, int M, int V, int32_t *inv,
                    int32_t *inv_strict);

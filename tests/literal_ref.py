@@ -234,6 +234,175 @@ def nm_reorder(R, rows, cols, vals, N, M, sched, maxiter=10):
     return [init_size, len(lst), init_cnt, total(lst), rnd - 1]
 
 
+# ------------------------------------------------------------ allpairs ----
+
+# GetSegSwapScoreAll's bit compositions (reorder/reorder.h:185-227) and the
+# SwapPairs pairtab (reorder/swappair.h:54-97), as data.
+AP_POS = [
+    [1, 2, 6, 7, 0, 3, 4, 5], [1, 2, 5, 7, 0, 3, 4, 6], [1, 2, 5, 6, 0, 3, 4, 7],
+    [1, 2, 4, 7, 0, 3, 5, 6], [1, 2, 4, 6, 0, 3, 5, 7], [1, 2, 4, 5, 0, 3, 6, 7],
+    [0, 1, 4, 5, 2, 3, 6, 7], [0, 1, 4, 6, 2, 3, 5, 7], [0, 1, 4, 7, 2, 3, 5, 6],
+    [0, 1, 5, 6, 2, 3, 4, 7], [0, 1, 5, 7, 2, 3, 4, 6], [0, 1, 6, 7, 2, 3, 4, 5],
+    [0, 2, 4, 5, 1, 3, 6, 7], [0, 2, 4, 6, 1, 3, 5, 7], [0, 2, 4, 7, 1, 3, 5, 6],
+    [0, 2, 5, 6, 1, 3, 4, 7], [0, 2, 5, 7, 1, 3, 4, 6], [0, 2, 6, 7, 1, 3, 4, 5],
+    [0, 3, 4, 5, 1, 2, 6, 7], [0, 3, 4, 6, 1, 2, 5, 7], [0, 3, 4, 7, 1, 2, 5, 6],
+    [0, 3, 5, 6, 1, 2, 4, 7], [0, 3, 5, 7, 1, 2, 4, 6], [0, 3, 6, 7, 1, 2, 4, 5],
+    [1, 3, 6, 7, 0, 2, 4, 5], [1, 3, 5, 7, 0, 2, 4, 6], [1, 3, 5, 6, 0, 2, 4, 7],
+    [1, 3, 4, 7, 0, 2, 5, 6], [1, 3, 4, 6, 0, 2, 5, 7], [1, 3, 4, 5, 0, 2, 6, 7],
+    [2, 3, 4, 5, 0, 1, 6, 7], [2, 3, 4, 6, 0, 1, 5, 7], [2, 3, 4, 7, 0, 1, 5, 6],
+    [2, 3, 5, 6, 0, 1, 4, 7], [2, 3, 5, 7, 0, 1, 4, 6], [2, 3, 6, 7, 0, 1, 4, 5]]
+AP_PAIRTAB = [
+    [0, 6, 3, 7], [0, 5, 3, 7], [0, 5, 3, 6], [0, 4, 3, 7], [0, 4, 3, 6], [0, 4, 3, 5],
+    [2, 4, 3, 5], [2, 4, 3, 6], [2, 4, 3, 7], [2, 5, 3, 6], [2, 5, 3, 7], [2, 6, 3, 7],
+    [1, 4, 3, 5], [1, 4, 3, 6], [1, 4, 3, 7], [1, 5, 3, 6], [1, 5, 3, 7], [1, 6, 3, 7],
+    [1, 4, 2, 5], [1, 4, 2, 6], [1, 4, 2, 7], [1, 5, 2, 6], [1, 5, 2, 7], [1, 6, 2, 7],
+    [0, 6, 2, 7], [0, 5, 2, 7], [0, 5, 2, 6], [0, 4, 2, 7], [0, 4, 2, 6], [0, 4, 2, 5],
+    [0, 4, 1, 5], [0, 4, 1, 6], [0, 4, 1, 7], [0, 5, 1, 7], [0, 5, 1, 6], [0, 6, 1, 5]]
+
+
+def exchange_of_pos(i):
+    """(l positions given away, r positions taken) of double swap i as scored."""
+    kept = [p for p in AP_POS[i][:4] if p < 4]
+    taken = sorted(p - 4 for p in AP_POS[i][:4] if p >= 4)
+    return sorted(set(range(4)) - set(kept)), taken
+
+
+def seg_score_all(bm, N, seg1, seg2):
+    """GetSegSwapScoreAll per row (M = 4): 16 singles, 36 doubles."""
+    M = 4
+    sc = [0] * 104
+    for row in bm:
+        l, r = row[seg1], row[seg2]
+        base = seg_score([[l, r]], N, M, 0, 1)
+        for i in range(32):
+            sc[i] += base[i]
+        bits = [(l >> (M - 1 - i)) & 1 for i in range(M)] + [(r >> (M - 1 - i)) & 1 for i in range(M)]
+        for i in range(36):
+            ln = sum(bits[AP_POS[i][j]] << (M - 1 - j) for j in range(M))
+            rn = sum(bits[AP_POS[i][M + j]] << (M - 1 - j) for j in range(M))
+            sc[2 * (16 + i)] += (popc(l) > N) - (popc(ln) > N)
+            sc[2 * (16 + i) + 1] += (popc(r) > N) - (popc(rn) > N)
+    return sc
+
+
+def allpairs_reorder(R, rows, cols, vals, N, sched, clean=False, maxiter=10):
+    """run_allpairs & siblings (sched 4..7, M = 4), literal dense form.
+    Returns (stats, number of swaps over an already relabelled node)."""
+    M = 4
+    nseg = (R + M - 1) // M
+    newid = [-1] * R
+    base = sched - 4
+    rev = base in (2, 3)
+    drain = base in (1, 3)
+    state = {"bm": dense_bitmaps(R, M, rows, cols, vals), "over": 0}
+
+    def locate_list():
+        cnt = [0] * nseg
+        for row in state["bm"]:
+            for s_ in range(nseg):
+                if popc(row[s_]) > N:
+                    cnt[s_] += 1
+        pq = StdPQ(cmp_reverse if rev else cmp)
+        for s_ in range(nseg):
+            if cnt[s_] != 0:
+                pq.push((s_, cnt[s_]))
+        return pq
+
+    def total(pq):
+        q = pq.copy()
+        t = 0
+        while len(q):
+            t += q.top()[1]
+            q.pop()
+        return t
+
+    def min_outdeg_seg():
+        zero = [sum(1 for row in state["bm"] if row[s_] == 0) for s_ in range(nseg)]
+        return sorted((zero[s_], s_) for s_ in range(nseg))[-1][1]
+
+    def pair_list(s1, s2):
+        sc = seg_score_all(state["bm"], N, s1, s2)
+        pq = StdPQ(lambda a, b: swap_less((0, 0, a[4], a[5]), (0, 0, b[4], b[5])))
+        for i in range(16):
+            pq.push((s1 * M + i // M, s2 * M + i % M, -1, -1, sc[2 * i], sc[2 * i + 1]))
+        for i in range(36):
+            if clean:
+                lo, tk = exchange_of_pos(i)
+                a, b, c, d = lo[0], tk[0], lo[1], tk[1]
+            else:
+                a, b, c, d = AP_PAIRTAB[i][0], AP_PAIRTAB[i][1] - 4, AP_PAIRTAB[i][2], AP_PAIRTAB[i][3] - 4
+            pq.push((s1 * M + a, s2 * M + b, s1 * M + c, s2 * M + d, sc[2 * (16 + i)], sc[2 * (16 + i) + 1]))
+        return pq
+
+    def visited(v):
+        return v != -1 and newid[v] != -1
+
+    def swap(a, b):
+        if newid[a] != -1 or newid[b] != -1:
+            state["over"] += 1
+        newid[a] = b
+        newid[b] = a
+
+    def apply(p):
+        swap(p[0], p[1])
+        if p[2] != -1 and p[3] != -1:
+            swap(p[2], p[3])
+
+    lst = locate_list()
+    init_cnt = total(lst)
+    init_size = len(lst)
+    rnd = 1
+    MAXITER = max(maxiter, init_size)
+    while rnd <= MAXITER and len(lst) >= 1:
+        if len(lst) == 1:
+            apply(pair_list(lst.top()[0], min_outdeg_seg()).top())
+        while len(lst) >= 2:
+            pseg, pcnt = lst.top()
+            lst.pop()
+            if drain:
+                temp = StdPQ(cmp if rev else cmp_reverse)
+                while len(lst):
+                    temp.push(lst.top())
+                    lst.pop()
+            else:
+                temp = lst.copy()
+            upd = StdPQ(cmp_reverse if rev else cmp)
+            while len(temp):
+                if pcnt <= 0 or all(visited(pseg * M + i) for i in range(M)):
+                    break
+                tseg, tcnt = temp.top()
+                pq = pair_list(pseg, tseg)
+                if clean:
+                    while len(pq) and any(visited(v) for v in pq.top()[:4]):
+                        pq.pop()
+                else:
+                    while len(pq) and (visited(pq.top()[0]) or visited(pq.top()[1])) and \
+                            (visited(pq.top()[2]) or visited(pq.top()[3])):
+                        pq.pop()
+                if len(pq):
+                    p = pq.top()
+                    apply(p)
+                    pcnt -= p[4]
+                    tcnt -= p[5]
+                if tcnt > 0:
+                    upd.push((tseg, tcnt))
+                temp.pop()
+            lst = upd
+        for i in range(len(rows)):
+            u, v = rows[i], cols[i]
+            if newid[u] != -1:
+                rows[i] = newid[u]
+            if newid[v] != -1:
+                cols[i] = newid[v]
+        for i in range(R):
+            newid[i] = -1
+        state["bm"] = dense_bitmaps(R, M, rows, cols, vals)
+        lst = locate_list()
+        rnd += 1
+    lst = locate_list()
+    return [init_size, len(lst), init_cnt, total(lst), rnd - 1], state["over"]
+
+
 # ---------------------------------------------------------------- VENOM ----
 
 def gray_code(n):

@@ -261,3 +261,64 @@ def test_retained_nnz_brute_force():
                                              for i in range(b0, min(b0 + V, R))))
                     tot += best
             assert oracle.retained_nnz(m, V, 2, M) == tot
+
+
+# ---------------------------------------------------------------- allpairs (M = 4)
+
+def test_allpairs_tables_scored_vs_applied():
+    # the double swap each score entry describes equals the applied pairtab
+    # entry for codes 0..32 and differs for 33..35 (the reference mismatch)
+    for i in range(36):
+        lo, tk = L.exchange_of_pos(i)
+        pt = L.AP_PAIRTAB[i]
+        applied = (sorted([pt[0], pt[2]]), sorted([pt[1] - 4, pt[3] - 4]))
+        assert (applied == (lo, tk)) == (i < 33), i
+
+
+def _dense_m4(R, dens, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.random((R, R)) < dens
+    r, c = np.nonzero(A)
+    return oracle.from_tuples(R, R, r.astype(np.int32), c.astype(np.int32),
+                              rng.uniform(0.1, 1, r.size).astype(np.float32), 4), A
+
+
+def test_allpairs_score_brute_force():
+    # gains from relabelling the columns of a dense matrix and recounting
+    N = 2
+    for seed in range(4):
+        m, A = _dense_m4(32, 0.35, seed)
+        nseg = 8
+        rng = np.random.default_rng(seed + 10)
+
+        def cnt(D, sg):
+            return int((D[:, sg * 4:sg * 4 + 4].sum(axis=1) > N).sum())
+        for _ in range(6):
+            s1, s2 = rng.choice(nseg, 2, replace=False)
+            sc = oracle.swap_score_all(m, N, int(s1), int(s2))
+            cols = list(range(s1 * 4, s1 * 4 + 4)) + list(range(s2 * 4, s2 * 4 + 4))
+            for i in range(52):
+                if i < 16:
+                    b1, b2 = divmod(i, 4)
+                    order = list(range(8))
+                    order[b1], order[4 + b2] = order[4 + b2], order[b1]
+                else:
+                    order = L.AP_POS[i - 16]
+                D = A.copy()
+                D[:, cols] = A[:, [cols[k] for k in order]]
+                assert sc[2 * i] == cnt(A, s1) - cnt(D, s1), (seed, s1, s2, i)
+                assert sc[2 * i + 1] == cnt(A, s2) - cnt(D, s2), (seed, s1, s2, i)
+
+
+@pytest.mark.parametrize("sched", [4, 5, 6, 7])
+@pytest.mark.parametrize("clean", [False, True])
+def test_allpairs_matches_literal_restatement(sched, clean):
+    for seed in range(3):
+        m, _ = _dense_m4(24 + 8 * seed, 0.3, 100 + seed)
+        rows, cols, vals = list(m.row), list(m.col), list(m.val)
+        want, over = L.allpairs_reorder(m.nrows, rows, cols, vals, 2, sched, clean=clean)
+        got, ow = oracle.allpairs_reorder(m, 2, sched, clean=clean)
+        assert list(got) == want and ow == over, (seed, list(got), want, ow, over)
+        assert list(m.row) == rows and list(m.col) == cols
+        if clean:
+            assert ow == 0

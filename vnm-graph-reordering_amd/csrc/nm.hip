@@ -1011,8 +1011,11 @@ int nm_reorder_ex(vnm_graph *g, int N, int sched, int maxiter, int round_cap, in
                   int64_t trace_cap, int64_t *trace_len, vnm_nm_stats *stats) {
     int rc = check_nm_args(g, N);
     if (rc) return rc;
-    if (sched < 0 || (sched >= 4 && sched <= 7))
-        return set_error(VNM_E_ARG, "schedule %d: the GPU engine runs schedules 0-3 and >= 8", sched);
+    if (sched >= 4 && sched <= 7) {
+        if (d_trace || round_cap > 0) return set_error(VNM_E_ARG, "allpairs schedules record no trace");
+        return nm_reorder_allpairs(g, N, sched, maxiter, 0, stats);
+    }
+    if (sched < 0) return set_error(VNM_E_ARG, "schedule %d is negative", sched);
     if ((rc = build_csc(g))) return rc;
     if ((rc = locate_all(g, N, g->cnt.p, g->nzr.p))) return rc;
     hipStream_t st = g->ctx->stream;

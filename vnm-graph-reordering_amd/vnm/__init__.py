@@ -37,7 +37,7 @@ EXPORTS = [
     "vnm_lp_features", "vnm_lp_forward", "vnm_lp_loss_grad", "vnm_lp_train_step",
     "vnm_pair_score", "vnm_lp_reorder", "vnm_hybrid_learned", "vnm_graph_clone",
     "vnm_spmm_set_shape", "vnm_spmm_get_shape", "vnm_gcn_layer", "vnm_batch_create",
-    "vnm_batch_run", "vnm_batch_info", "vnm_batch_destroy",
+    "vnm_batch_run", "vnm_batch_info", "vnm_batch_destroy", "vnm_nm_reorder_allpairs",
 ]
 
 LP_NPARAMS = 22784
@@ -184,6 +184,8 @@ def lib():
     L.vnm_hybrid_learned.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, P(HybridStats)]
     L.vnm_graph_clone.argtypes = [vp, P(vp)]
+    L.vnm_nm_reorder_allpairs.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          P(NMStats)]
     L.vnm_spmm_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.vnm_gcn_layer.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, vp, ctypes.c_int]
     L.vnm_batch_create.argtypes = [vp, P(vp), ctypes.c_int, P(vp), i32p, ctypes.c_int, ctypes.c_int, P(vp),
@@ -410,6 +412,11 @@ class Graph:
     def nm_reorder(self, N, sched, maxiter=10):
         st = NMStats()
         _check(lib().vnm_nm_reorder(self.h, N, sched, maxiter, ctypes.byref(st)))
+        return st.as_list()
+
+    def nm_reorder_allpairs(self, N, sched, maxiter=10, clean=False):
+        st = NMStats()
+        _check(lib().vnm_nm_reorder_allpairs(self.h, N, sched, maxiter, int(clean), ctypes.byref(st)))
         return st.as_list()
 
     def nm_trace(self, N, sched, maxiter=10, round_cap=0, cap=1 << 20):
