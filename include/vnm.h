@@ -88,7 +88,8 @@ void *vnm_ctx_stream(vnm_ctx *ctx);
 int64_t vnm_ctx_launches(vnm_ctx *ctx);
 /* Kernel timers (the reference's GpuTimer / START_TIMER, util/utility.h:
  * 177-186,267-301): when enabled, HIP events bracket each launch of
- * k_spmm, k_nm_greedy, k_score_rows and k_locate_all on the context stream.
+ * k_spmm, k_score_rows and k_locate_all, and each whole N:M greedy run
+ * ("nm_greedy"), on the context stream.
  * Enabling or disabling resets the totals; reading waits for the events. */
 int vnm_ctx_timing(vnm_ctx *ctx, int enable);
 int vnm_ctx_timer_read(vnm_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);

@@ -96,8 +96,8 @@ __global__ __launch_bounds__(256) void k_ap_slots(int R, const int32_t *__restri
     __syncthreads();
     const int total = table_columns(sel, 1, 4, inv, cptr, cols, pre);
     uint32_t *slot = tab + (size_t)blockIdx.x * (R >> 2);
-    if (clear) table_clear(slot, nullptr, 4, R, cols, pre, 4, total, crow);
-    else table_scatter(slot, nullptr, 4, R, cols, pre, 4, total, crow);
+    if (clear) table_clear(slot, 4, R, cols, pre, 4, total, crow);
+    else table_scatter(slot, 4, R, cols, pre, 4, total, crow);
 }
 
 // Histogram of (l, r) bitmap pairs for every unordered pair of the K

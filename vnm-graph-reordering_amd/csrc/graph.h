@@ -74,6 +74,8 @@ struct vnm_graph {
     vnm::DBuf<int32_t> cptr, crow;        // CSC of nonzero entries (N:M run)
     vnm::DBuf<int32_t> perm, inv, cnt, nzr, ws;
     vnm::DBuf<uint32_t> tab;              // per-slot row bitmaps (bytes in words)
+    vnm::DBuf<uint32_t> rt;               // N:M greedy row records A and B (16 B per row each)
+    vnm::DBuf<int32_t> gst;               // N:M greedy device state
     size_t tab_slots = 0;
     // bumped whenever the COO is relabelled or re-sorted; caches built from
     // the COO (the proposer's adjacency) are valid for one epoch

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(SS_T) void k_lp_seg_stats(
         if (tid < 4) segc[tid] = 0;
         __syncthreads();
         const int total = table_columns(sel, 1, M, inv, ptr, cols, pre);
-        table_scatter(slot, nullptr, M, R, cols, pre, M, total, idx);
+        table_scatter(slot, M, R, cols, pre, M, total, idx);
         __syncthreads();
         for (int e = tid; e < total; e += SS_T) {
             const int t = entry_col(pre, M, e);
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(SS_T) void k_lp_seg_stats(
             if ((bits >> (M - 1 - t)) == 1u) atomicAdd(&segc[k], 1);
         }
         __syncthreads();
-        table_clear(slot, nullptr, M, R, cols, pre, M, total, idx);
+        table_clear(slot, M, R, cols, pre, M, total, idx);
         const int s = sel[0];
         if (tid < M * 4) {
             const int node = ld_l2(&inv[s * M + (tid >> 2)]);
@@ -692,8 +692,8 @@ __global__ __launch_bounds__(256) void k_lp_fill_pairs(int R, int M, const Cand 
     __syncthreads();
     const int total = table_columns(sel, 1, M, inv, ptr, cols, pre);
     uint32_t *slot = tab + (size_t)blockIdx.x * (R >> 2);
-    if (clear) table_clear(slot, nullptr, M, R, cols, pre, M, total, idx);
-    else table_scatter(slot, nullptr, M, R, cols, pre, M, total, idx);
+    if (clear) table_clear(slot, M, R, cols, pre, M, total, idx);
+    else table_scatter(slot, M, R, cols, pre, M, total, idx);
 }
 
 // Exact M x M gains of candidate i's segment pair (p, q) from the entries of

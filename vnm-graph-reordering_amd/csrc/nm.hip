@@ -26,7 +26,8 @@
 //                   slot bytes with wave ballots and scalar popcounts
 //                   (GetSegSwapScore, reorder/reorder.h:75-131)
 //   k_pq_pop_order: libstdc++ priority_queue pop order (reorder/swappair.h)
-//   k_nm_greedy   : the whole greedy schedule in one persistent workgroup
+//   k_gr_*        : one greedy round as a chain of kernels (see the greedy
+//                   section below)
 //   k_relabel     : the Reorder kernel (reorder/reorder.h:23-35), applied once
 #include "graph.h"
 #include "hip_common.h"
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(LOC_T) void k_locate_all(
         if (threadIdx.x == 0) { sel[0] = s; c_inv = 0; c_nz = 0; }
         __syncthreads();
         int total = table_columns(sel, 1, M, inv, cptr, cols, pre);
-        table_scatter(slot, nullptr, M, R, cols, pre, M, total, crow);
+        table_scatter(slot, M, R, cols, pre, M, total, crow);
         __syncthreads();
         int my_inv = 0, my_nz = 0;
         for (int e = threadIdx.x; e < total; e += blockDim.x) {
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(LOC_T) void k_locate_all(
         if (my_inv) atomicAdd(&c_inv, my_inv);
         if (my_nz) atomicAdd(&c_nz, my_nz);
         __syncthreads();
-        table_clear(slot, nullptr, M, R, cols, pre, M, total, crow);
+        table_clear(slot, M, R, cols, pre, M, total, crow);
         if (threadIdx.x == 0) { cnt[s] = c_inv; nzr[s] = c_nz; }
         __syncthreads();
     }
@@ -180,8 +181,8 @@ __global__ __launch_bounds__(256) void k_fill_slots(int R, int M, const int32_t 
     __syncthreads();
     int total = table_columns(sel, 1, M, inv, cptr, cols, pre);
     uint32_t *slot = tab + (size_t)blockIdx.x * (R >> 2);
-    if (clear) table_clear(slot, nullptr, M, R, cols, pre, M, total, crow);
-    else table_scatter(slot, nullptr, M, R, cols, pre, M, total, crow);
+    if (clear) table_clear(slot, M, R, cols, pre, M, total, crow);
+    else table_scatter(slot, M, R, cols, pre, M, total, crow);
 }
 
 // Gain matrix of slot 0 against slots 1..ntgt over all R rows.
@@ -285,40 +286,81 @@ __global__ void k_compose(int32_t *__restrict__ total, const int32_t *__restrict
 }
 
 // ------------------------------------------------------------ greedy
+//
+// One N:M run is a chain of kernels per round on the context's stream:
+//
+//   k_gr_select   (1 workgroup)   commit the last round's recount, stop or
+//                                 choose the round's segments, their columns
+//   k_gr_fill_a   (grid)          clear records B (last recount), fill A
+//   k_gr_score    (grid)          gains of every pair from records A
+//   k_gr_greedy   (1 workgroup)   priority queues, sequential greedy, commit
+//                                 to perm/inv, columns after the commit
+//   k_gr_fill_b   (grid)          clear records A, fill B
+//   k_gr_count    (grid)          cnt / nzr of the round's segments from B
+//
+// The passes over the round's entries, which hub segments make large, run
+// on every CU; the sequential parts keep one workgroup.  Kernel boundaries
+// order everything, so no workgroup waits on another.  The host enqueues
+// rounds in batches; once k_gr_select sets `done` every later kernel
+// returns at entry.  Records: 16 bytes per row, byte j = the row's M-bit
+// bitmap in slot j of the round (j < KMAX); A holds the segments' columns
+// before the round's swaps, B after; both are zero between runs.
 
-constexpr int GT = 1024;         // threads of the persistent workgroup
-constexpr int NW = GT / 64;      // its waves
+constexpr int GT = 1024;         // threads of the single-workgroup kernels
+constexpr int NW = GT / 64;      // their waves
+constexpr int PT = 256;          // threads of the pass kernels
+constexpr int PNW = PT / 64;
 constexpr int KMAX = 9;          // primary + up to M targets (M <= 8)
 constexpr int MMAX = 8;
 constexpr int NPAIR_T = KMAX * (KMAX - 1) / 2 + 1;  // unordered pairs + one self pair
 constexpr int NQ = (KMAX - 1) * (KMAX - 1);         // ordered pairs the greedy can visit
 constexpr int POPP = 16;                            // pops precomputed per ordered pair
+constexpr int NACC = KMAX * MMAX + NPAIR_T * 2 * MMAX * MMAX;  // E and both, flattened
 
-struct NMArgs {
-    int R, M, N, nseg, sched, maxiter;
-    const int32_t *cptr, *crow;
-    int32_t *perm, *inv, *cnt, *nzr;
-    uint32_t *tab;       // KMAX slots of R bytes, zero on entry and exit
-    int32_t *out;        // stats[5], status, work, nswaps
-    int32_t *trace;      // optional: 9 ints per swap
-    int64_t trace_cap;
-    int round_cap;       // stop after this many rounds (training traces; INT_MAX = none)
+// The columns of one set of records: slot j, bit b = column j*M + b.
+struct ColSet {
+    ColSpan cols[KMAX * MMAX];
+    int pre[KMAX * MMAX + 1];
+    int cb[KMAX + 1];  // 64-entry chunks of slot j: [cb[j], cb[j+1])
+    int K, total;
 };
 
+// Device state of one run.
+struct GreedyState {
+    // parameters
+    int R, M, N, nseg, sched, round_cap;
+    int64_t trace_cap;
+    int32_t *trace;  // optional: 9 ints per swap
+    // run state
+    int round, L, totcnt, maxiter, init_size, init_cnt, status, done, pending;
+    int K, self_pair, size1;
+    int sel[KMAX], selcnt[KMAX];
+    int64_t trace_len, work;
+    ColSet S, C;  // columns of records A (score) and B (recount)
+    int acc[NACC];
+    int newcnt[KMAX], newnzr[KMAX];
+    uint64_t clk[8];
+};
+
+// Kernel arguments of every round kernel: pointers fixed for the graph.
+struct NMArgs {
+    GreedyState *st;
+    const int32_t *cptr, *crow;
+    int32_t *perm, *inv, *cnt, *nzr;
+    uint32_t *rta, *rtb;
+};
+
+// LDS of the single-workgroup kernels.
 struct GreedyShared {
     int sel[KMAX];
     int selcnt[KMAX];
-    int K, self_pair, size1, cont;
-    int L, totcnt, round, maxiter, init_size, init_cnt, status;
-    ColSpan cols[KMAX * MMAX];
-    int pre[KMAX * MMAX + 1];
-    int cb[KMAX + 1];
+    int K, self_pair, size1, cont, status, L;
+    ColSet cs;
     int E[KMAX][MMAX];
     // per unordered pair (j < j2) and one self pair: the "both rows" sums
     //   Bl[b1*M+b2] = sum over rows in both of y_b2 * (x_b1 ? A : B)
     //   Br[b1*M+b2] = sum over rows in both of x_b1 * (y_b2 ? D : C)
     int both[NPAIR_T][2][MMAX * MMAX];
-    int newcnt[KMAX], newnzr[KMAX];
     // slot-major (element / slot / pop index first, list second), so the
     // lanes of a wave building different lists touch consecutive bytes
     uint8_t rank[MMAX * MMAX][NQ];
@@ -332,9 +374,6 @@ struct GreedyShared {
     uint64_t wbest[NW];
     uint64_t topk[KMAX + 1];
     int64_t wkeys[NW][64];
-    uint64_t phase_clk[8];
-    int64_t trace_len;
-    int64_t work;
 };
 
 __device__ __forceinline__ uint64_t key_fwd(int s, int c) {
@@ -364,7 +403,7 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
 
 // Top-k (k <= KMAX + 1) keys of the block into S.topk (0 = none); tk holds
 // each thread's descending local list.
-__device__ void block_topk(uint64_t (&tk)[KMAX + 1], int k, GreedyShared &S) {
+__device__ __forceinline__ void block_topk(uint64_t (&tk)[KMAX + 1], int k, GreedyShared &S) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int it = 0; it < k; ++it) {
         uint64_t m = wave_max_u64(tk[0]);
@@ -393,7 +432,7 @@ __device__ void block_topk(uint64_t (&tk)[KMAX + 1], int k, GreedyShared &S) {
     __syncthreads();
 }
 
-__device__ uint64_t block_max_u64(uint64_t v, GreedyShared &S) {
+__device__ __forceinline__ uint64_t block_max_u64(uint64_t v, GreedyShared &S) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     v = wave_max_u64(v);
     if (lane == 0) S.wbest[w] = v;
@@ -442,7 +481,7 @@ __device__ __forceinline__ int q_index(int a, int b, int K) {
     return n + (a - 1) * (n - 1) + (b - 1 - (b > a ? 1 : 0));
 }
 
-__device__ void q_pair(int q, int K, int &a, int &b) {
+__device__ __forceinline__ void q_pair(int q, int K, int &a, int &b) {
     const int n = K - 1;
     if (q < n) { a = 0; b = q + 1; return; }
     q -= n;
@@ -451,67 +490,163 @@ __device__ void q_pair(int q, int K, int &a, int &b) {
     b = 1 + r + (r + 1 >= a ? 1 : 0);
 }
 
-// Decode entry e of slot j: its column bit b and row r.
-__device__ __forceinline__ void slot_entry(const GreedyShared &S, const int32_t *crow, int j, int M,
-                                           int e, int &b, int &r) {
-    b = entry_col(S.pre + j * M, M, e);
-    const int t = j * M + b;
-    r = crow[S.cols[t].start + (e - S.pre[t])];
-}
+// ---- column sets and record passes
 
-// Chunk table of the round: slot j owns 64-entry chunks [cb[j], cb[j+1]).
-__device__ int chunk_table(GreedyShared &S, int K, int M) {
+// Chunk table of a column set (thread 0, then a barrier).
+__device__ __forceinline__ void chunk_table(ColSet &cs, int K, int M) {
     if (threadIdx.x == 0) {
         int acc = 0;
         for (int j = 0; j < K; ++j) {
-            S.cb[j] = acc;
-            acc += (S.pre[j * M + M] - S.pre[j * M] + 63) / 64;
+            cs.cb[j] = acc;
+            acc += (cs.pre[j * M + M] - cs.pre[j * M] + 63) / 64;
         }
-        S.cb[K] = acc;
+        cs.cb[K] = acc;
+        cs.K = K;
     }
     __syncthreads();
-    return S.cb[K];
 }
 
-__device__ __forceinline__ int chunk_slot(const GreedyShared &S, int K, int c) {
+__device__ __forceinline__ int chunk_slot(const ColSet &cs, int K, int c) {
     int j = 0;
-    while (j + 1 < K && S.cb[j + 1] <= c) ++j;
+    while (j + 1 < K && cs.cb[j + 1] <= c) ++j;
     return j;
 }
 
-// Gains of every pair of the round's segments (the GetSegSwapScore sums,
-// reorder/reorder.h:75-131, in closed form).  Waves take 64-entry chunks of
-// one slot each; the owner entries of a chunk are folded in with ballots, so
-// each (b1, b2) counter takes one LDS atomic per chunk instead of one per row,
-// and the partner-slot bytes of a chunk are loaded together.
-__device__ void greedy_score_phase(const NMArgs &A, GreedyShared &S) {
-    const int M = A.M, N = A.N, R = A.R, K = S.K, MM = M * M;
+// Entry e of slot j: its column bit b and row r.
+__device__ __forceinline__ void slot_entry(const ColSet &cs, const int32_t *crow, int j, int M, int e,
+                                           int &b, int &r) {
+    b = entry_col(cs.pre + j * M, M, e);
+    const int t = j * M + b;
+    r = crow[cs.cols[t].start + (e - cs.pre[t])];
+}
+
+__device__ __forceinline__ void copy_words(void *dst, const void *src, int bytes) {
+    const int n = bytes / 4;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) ((int *)dst)[i] = ((const int *)src)[i];
+}
+
+__device__ __forceinline__ uint32_t rec_byte(const uint4 &v, int j) {
+    const uint32_t w = (j >> 2) == 0 ? v.x : (j >> 2) == 1 ? v.y : v.z;
+    return (w >> ((j & 3) * 8)) & 0xFFu;
+}
+
+// Set (CLEAR = false) or zero the records of this workgroup's share of the
+// entries of a column set.
+template <bool CLEAR>
+__device__ __forceinline__ void rec_pass(uint32_t *rt, const ColSet &cs, int M, const int32_t *crow) {
+    const int nc = cs.K * M, total = cs.total;
+    const int stride = gridDim.x * blockDim.x;
+    for (int e0 = blockIdx.x * blockDim.x + threadIdx.x; e0 < total; e0 += stride * TAB_U) {
+        int t[TAB_U], r[TAB_U];
+        table_entries(cs.cols, cs.pre, nc, total, crow, e0, stride, t, r);
+#pragma unroll
+        for (int u = 0; u < TAB_U; ++u) {
+            if (t[u] < 0) continue;
+            if constexpr (CLEAR) {
+                *(uint4 *)(rt + 4 * (size_t)r[u]) = make_uint4(0u, 0u, 0u, 0u);
+            } else {
+                const int j = t[u] / M, b = t[u] % M;
+                atomicOr(rt + 4 * (size_t)r[u] + (j >> 2), (1u << (M - 1 - b)) << ((j & 3) * 8));
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void clock_add(GreedyState *st, int i, uint64_t t0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->clk[i] += __builtin_amdgcn_s_memrealtime() - t0;
+}
+
+// Records B of the last round's recount are cleared, records A of this
+// round's segments filled.
+__global__ __launch_bounds__(PT) void k_gr_fill_a(NMArgs A) {
+    __shared__ ColSet sa, sb;
+    __shared__ int done;
+    GreedyState *st = A.st;
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    if (threadIdx.x == 0) done = st->done;
+    __syncthreads();
+    if (done) return;
+    copy_words(&sa, &st->S, sizeof(ColSet));
+    copy_words(&sb, &st->C, sizeof(ColSet));
+    __syncthreads();
+    const int M = st->M;
+    rec_pass<true>(A.rtb, sb, M, A.crow);
+    rec_pass<false>(A.rta, sa, M, A.crow);
+    clock_add(st, 1, t0);
+}
+
+// Records A of this round are cleared, records B filled with the columns
+// after the round's swaps.
+__global__ __launch_bounds__(PT) void k_gr_fill_b(NMArgs A) {
+    __shared__ ColSet sa, sb;
+    __shared__ int done;
+    GreedyState *st = A.st;
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    if (threadIdx.x == 0) done = st->done;
+    __syncthreads();
+    if (done) return;
+    copy_words(&sa, &st->S, sizeof(ColSet));
+    copy_words(&sb, &st->C, sizeof(ColSet));
+    __syncthreads();
+    const int M = st->M;
+    rec_pass<true>(A.rta, sa, M, A.crow);
+    rec_pass<false>(A.rtb, sb, M, A.crow);
+    clock_add(st, 4, t0);
+}
+
+// After a run: both record sets back to zero (zeroing a clean record is
+// harmless, so this needs no knowledge of where the run stopped).
+__global__ __launch_bounds__(PT) void k_gr_cleanup(NMArgs A) {
+    __shared__ ColSet sa, sb;
+    GreedyState *st = A.st;
+    copy_words(&sa, &st->S, sizeof(ColSet));
+    copy_words(&sb, &st->C, sizeof(ColSet));
+    __syncthreads();
+    const int M = st->M;
+    rec_pass<true>(A.rta, sa, M, A.crow);
+    rec_pass<true>(A.rtb, sb, M, A.crow);
+}
+
+// Gains of every pair of the round's segments from records A (the
+// GetSegSwapScore sums, reorder/reorder.h:75-131, in closed form).  A wave
+// takes a 64-entry chunk of one slot; the owner entries (the entry carrying
+// the row's leading bit, one per row) are folded in with ballots, so each
+// (b1, b2) counter takes one LDS atomic per chunk; each workgroup adds its
+// nonzero sums to the device accumulators.
+__global__ __launch_bounds__(PT) void k_gr_score(NMArgs A) {
+    __shared__ ColSet sa;
+    __shared__ int E[KMAX][MMAX];
+    __shared__ int both[NPAIR_T][2][MMAX * MMAX];
+    __shared__ int done;
+    GreedyState *st = A.st;
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    if (threadIdx.x == 0) done = st->done;
+    __syncthreads();
+    if (done) return;
+    copy_words(&sa, &st->S, sizeof(ColSet));
+    for (int t = threadIdx.x; t < KMAX * MMAX; t += PT) (&E[0][0])[t] = 0;
+    for (int t = threadIdx.x; t < NPAIR_T * 2 * MMAX * MMAX; t += PT) (&both[0][0][0])[t] = 0;
+    __syncthreads();
+    const int M = st->M, N = st->N, MM = M * M;
+    const int K = sa.K, self_pair = st->self_pair;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int nc = K * M;
-    for (int t = threadIdx.x; t < KMAX * MMAX; t += GT) (&S.E[0][0])[t] = 0;
-    for (int t = threadIdx.x; t < NPAIR_T * 2 * MMAX * MMAX; t += GT) (&S.both[0][0][0])[t] = 0;
-    const int total = table_columns(S.sel, K, M, A.inv, A.cptr, S.cols, S.pre);
-    table_scatter(A.tab, nullptr, M, R, S.cols, S.pre, nc, total, A.crow);
-    const int nch = chunk_table(S, K, M);  // (barrier inside)
+    const int nch = sa.cb[K];
     const int b1 = lane / M, b2 = lane % M;
-    for (int c = w; c < nch; c += NW) {
-        const int j = chunk_slot(S, K, c);
-        const int e1 = S.pre[j * M + M];
-        const int e = S.pre[j * M] + (c - S.cb[j]) * 64 + lane;
+    for (int c = blockIdx.x * PNW + w; c < nch; c += gridDim.x * PNW) {
+        const int j = chunk_slot(sa, K, c);
+        const int e1 = sa.pre[j * M + M];
+        const int e = sa.pre[j * M] + (c - sa.cb[j]) * 64 + lane;
         uint32_t l = 0;
-        int r = 0;
+        uint4 rec = make_uint4(0u, 0u, 0u, 0u);
         bool act = false;
         if (e < e1) {
-            int b;
-            slot_entry(S, A.crow, j, M, e, b, r);
-            l = slot_byte(A.tab, j, R, r);
+            int b, r;
+            slot_entry(sa, A.crow, j, M, e, b, r);
+            rec = *(const uint4 *)(A.rta + 4 * (size_t)r);
+            l = rec_byte(rec, j);
             act = (l >> (M - 1 - b)) == 1u;  // the row's leading-bit entry
             if (!act) l = 0;
         }
-        uint32_t rrs[KMAX];
-#pragma unroll
-        for (int j2 = 0; j2 < KMAX; ++j2)
-            rrs[j2] = (act && j2 > j && j2 < K) ? slot_byte(A.tab, j2, R, r) : 0u;
         const int pl = popc32(l);
         const uint64_t Am = __ballot(act && pl == N + 1), Bm = __ballot(act && pl == N);
         uint64_t Lb1 = 0, Lb2 = 0, Lme = 0;
@@ -523,12 +658,12 @@ __device__ void greedy_score_phase(const NMArgs &A, GreedyShared &S) {
         }
         if (lane < M) {
             const int v = __popcll(Lme & Am);
-            if (v) atomicAdd(&S.E[j][lane], v);
+            if (v) atomicAdd(&E[j][lane], v);
         }
 #pragma unroll
         for (int j2 = 0; j2 < KMAX; ++j2) {
             if (j2 <= j || j2 >= K) continue;
-            const uint32_t rr = rrs[j2];
+            const uint32_t rr = act ? rec_byte(rec, j2) : 0u;
             const int pr = popc32(rr);
             const uint64_t Cm = __ballot(pr == N), Dm = __ballot(pr == N + 1);
             uint64_t Rb = 0;
@@ -540,59 +675,73 @@ __device__ void greedy_score_phase(const NMArgs &A, GreedyShared &S) {
                 const int vl = __popcll(Rb & ((Lb1 & Am) | (~Lb1 & Bm)));
                 const int vr = __popcll(Lb1 & ((Rb & Dm) | (~Rb & Cm)));
                 const int pi = tri_index(j, j2, K);
-                if (vl) atomicAdd(&S.both[pi][0][lane], vl);
-                if (vr) atomicAdd(&S.both[pi][1][lane], vr);
+                if (vl) atomicAdd(&both[pi][0][lane], vl);
+                if (vr) atomicAdd(&both[pi][1][lane], vr);
             }
         }
-        if (S.self_pair && lane < MM) {  // seg1 == seg2: r = l for every row
+        if (self_pair && lane < MM) {  // seg1 == seg2: r = l for every row
             const int vl = __popcll(Lb2 & ((Lb1 & Am) | (~Lb1 & Bm)));
             const int vr = __popcll(Lb1 & ((Lb2 & Am) | (~Lb2 & Bm)));
-            if (vl) atomicAdd(&S.both[NPAIR_T - 1][0][lane], vl);
-            if (vr) atomicAdd(&S.both[NPAIR_T - 1][1][lane], vr);
+            if (vl) atomicAdd(&both[NPAIR_T - 1][0][lane], vl);
+            if (vr) atomicAdd(&both[NPAIR_T - 1][1][lane], vr);
         }
     }
     __syncthreads();
-    table_clear(A.tab, nullptr, M, R, S.cols, S.pre, nc, total, A.crow);
-    if (threadIdx.x == 0) S.work += total;
-    __syncthreads();
+    if (blockIdx.x * PNW < nch) {
+        for (int t = threadIdx.x; t < NACC; t += PT) {
+            const int v = t < KMAX * MMAX ? (&E[0][0])[t] : (&both[0][0][0])[t - KMAX * MMAX];
+            if (v) atomicAdd(&st->acc[t], v);
+        }
+    }
+    clock_add(st, 2, t0);
 }
 
-// cnt / nzr of the round's segments after the swaps (locate_invalid_segment).
-__device__ void greedy_recount_phase(const NMArgs &A, GreedyShared &S) {
-    const int M = A.M, N = A.N, R = A.R, K = S.K;
+// cnt / nzr of the round's segments after the swaps, from records B
+// (locate_invalid_segment on the touched segments).
+__global__ __launch_bounds__(PT) void k_gr_count(NMArgs A) {
+    __shared__ ColSet sb;
+    __shared__ int cnt[KMAX], nzr[KMAX];
+    __shared__ int done;
+    GreedyState *st = A.st;
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    if (threadIdx.x == 0) done = st->done;
+    __syncthreads();
+    if (done) return;
+    copy_words(&sb, &st->C, sizeof(ColSet));
+    if (threadIdx.x < KMAX) { cnt[threadIdx.x] = 0; nzr[threadIdx.x] = 0; }
+    __syncthreads();
+    const int M = st->M, N = st->N, K = sb.K;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int nc = K * M;
-    if (threadIdx.x < KMAX) { S.newcnt[threadIdx.x] = 0; S.newnzr[threadIdx.x] = 0; }
-    const int total = table_columns(S.sel, K, M, A.inv, A.cptr, S.cols, S.pre);
-    table_scatter(A.tab, nullptr, M, R, S.cols, S.pre, nc, total, A.crow);
-    const int nch = chunk_table(S, K, M);
-    for (int c = w; c < nch; c += NW) {
-        const int j = chunk_slot(S, K, c);
-        const int e1 = S.pre[j * M + M];
-        const int e = S.pre[j * M] + (c - S.cb[j]) * 64 + lane;
+    const int nch = sb.cb[K];
+    for (int c = blockIdx.x * PNW + w; c < nch; c += gridDim.x * PNW) {
+        const int j = chunk_slot(sb, K, c);
+        const int e1 = sb.pre[j * M + M];
+        const int e = sb.pre[j * M] + (c - sb.cb[j]) * 64 + lane;
         bool act = false;
         uint32_t l = 0;
         if (e < e1) {
             int b, r;
-            slot_entry(S, A.crow, j, M, e, b, r);
-            l = slot_byte(A.tab, j, R, r);
+            slot_entry(sb, A.crow, j, M, e, b, r);
+            l = rec_byte(*(const uint4 *)(A.rtb + 4 * (size_t)r), j);
             act = (l >> (M - 1 - b)) == 1u;
         }
         const uint64_t o = __ballot(act), v = __ballot(act && popc32(l) > N);
         if (lane == 0) {
-            if (o) atomicAdd(&S.newnzr[j], (int)__popcll(o));
-            if (v) atomicAdd(&S.newcnt[j], (int)__popcll(v));
+            if (o) atomicAdd(&nzr[j], (int)__popcll(o));
+            if (v) atomicAdd(&cnt[j], (int)__popcll(v));
         }
     }
     __syncthreads();
-    table_clear(A.tab, nullptr, M, R, S.cols, S.pre, nc, total, A.crow);
-    if (threadIdx.x == 0) S.work += total;
-    __syncthreads();
+    if (threadIdx.x < K) {
+        if (cnt[threadIdx.x]) atomicAdd(&st->newcnt[threadIdx.x], cnt[threadIdx.x]);
+        if (nzr[threadIdx.x]) atomicAdd(&st->newnzr[threadIdx.x], nzr[threadIdx.x]);
+    }
+    clock_add(st, 5, t0);
 }
 
 // First element of ordered pair q (slots a -> b) in pop order whose two
 // vertices are unvisited; -1 when the queue runs empty.
-__device__ int first_free(GreedyShared &S, int q, int a, int b, int M) {
+__device__ __forceinline__ int first_free(GreedyShared &S, int q, int a, int b, int M) {
     const int npre = min(POPP, M * M);
     for (int k = 0; k < npre; ++k) {
         int e = S.popped[k][q];
@@ -614,7 +763,7 @@ __device__ __forceinline__ bool list_before(int sa, int ca, int sb, int cb, bool
 }
 
 // The sequential part of one round (reorder/reorder.h:1837-1932), one thread.
-__device__ void greedy_apply(GreedyShared &S, int M, bool rev, bool drain) {
+__device__ __forceinline__ void greedy_apply(GreedyShared &S, int M, bool rev, bool drain) {
     const int K = S.K;
     S.nsw = 0;
     for (int i = 0; i < K * M; ++i) S.vis[i] = 0;
@@ -682,92 +831,83 @@ __device__ void greedy_apply(GreedyShared &S, int M, bool rev, bool drain) {
     }
 }
 
-// Phase clock (wall_clock64 ticks, 100 MHz on gfx950) accumulated per phase
-// for the diagnostic counters returned in out[8..15].
-#define PHASE_MARK(i)                                                     \
-    do {                                                                  \
-        if (tid == 0) {                                                   \
-            uint64_t _t = __builtin_amdgcn_s_memrealtime();               \
-            S.phase_clk[i] += _t - t_last;                                \
-            t_last = _t;                                                  \
-        }                                                                 \
-    } while (0)
-
-__global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
+// First kernel of a round: commit the last round's recount, decide whether
+// the run goes on (reorder/reorder.h:1819-1826: rounds <= max(maxiter, list
+// size), a list of at least one (two for schedules >= 8) segments), choose
+// the round's segments and their columns.
+__global__ __launch_bounds__(GT) void k_gr_select(NMArgs A) {
     __shared__ GreedyShared S;
-    uint64_t t_last = __builtin_amdgcn_s_memrealtime();
+    GreedyState *st = A.st;
     const int tid = threadIdx.x;
     const int lane = tid & 63, w = tid >> 6;
-    const int M = A.M, R = A.R, nseg = A.nseg, MM = M * M;
-    const bool plain = A.sched >= 8;
-    const bool rev = (A.sched == 2 || A.sched == 3);
-    const bool drain = (A.sched == 1 || A.sched == 3);
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    if (tid == 0) S.cont = !st->done;
+    __syncthreads();
+    if (!S.cont) return;
+    const int M = st->M, R = st->R, nseg = st->nseg, MM = M * M;
+    const bool plain = st->sched >= 8;
+    const bool rev = (st->sched == 2 || st->sched == 3);
+    const bool drain = (st->sched == 1 || st->sched == 3);
     const bool tfwd = drain ? rev : !rev;  // traversal order uses the forward key
     const int min_size = plain ? 2 : 1;
-
-    // initial list size and count (reorder/reorder.h:1808-1810)
+    (void)lane; (void)w; (void)MM;
     if (tid == 0) {
-        S.L = 0; S.totcnt = 0; S.work = 0;
-        for (int i = 0; i < 8; ++i) S.phase_clk[i] = 0;
-    }
-    __syncthreads();
-    {
-        int lsum = 0, csum = 0;
-        for (int s = tid; s < nseg; s += GT) {
-            int c = A.cnt[s];
-            lsum += c != 0;
-            csum += c;
+        if (st->pending) {
+            for (int j = 0; j < st->K; ++j) {
+                const int s = st->sel[j];
+                const int oc = ld_l2(&A.cnt[s]), nc = st->newcnt[j];
+                st_l2(&A.cnt[s], nc);
+                st_l2(&A.nzr[s], st->newnzr[j]);
+                st->L += (nc != 0) - (oc != 0);
+                st->totcnt += nc - oc;
+            }
+            st->round += 1;
+            st->pending = 0;
         }
-        if (lsum) atomicAdd(&S.L, lsum);
-        if (csum) atomicAdd(&S.totcnt, csum);
+        S.cont = st->round <= st->maxiter && st->round <= st->round_cap && st->L >= min_size &&
+                 st->status == 0;
+        if (!S.cont) st->done = 1;
+        S.L = st->L;
     }
     __syncthreads();
-    if (tid == 0) {
-        S.init_size = S.L;
-        S.init_cnt = S.totcnt;
-        S.maxiter = A.maxiter > S.L ? A.maxiter : S.L;  // :1819
-        S.round = 1;
-        S.status = 0;
-        S.trace_len = 0;
-    }
-    __syncthreads();
+    if (!S.cont) return;
+    const int Lr = S.L;
 
-    while (true) {
-        if (tid == 0)
-            S.cont = (S.round <= S.maxiter && S.round <= A.round_cap && S.L >= min_size && S.status == 0);
+    // ---- phase A: the round's segments
+    if (!plain && Lr == 1) {
+        // the single invalid segment, and the segment with the most empty
+        // rows, ties to the larger id (reorder/reorder.h:767-797)
+        uint64_t best_l = 0, best_z = 0;
+        for (int s = tid; s < nseg; s += GT) {
+            int c = ld_l2(&A.cnt[s]);
+            if (c != 0) best_l = 0x8000000000000000ull | (uint32_t)s;
+            uint64_t z = ((uint64_t)(uint32_t)(R - ld_l2(&A.nzr[s])) << 32) | (uint32_t)s;
+            best_z = z > best_z ? z : best_z;
+        }
+        uint64_t bl = block_max_u64(best_l, S);
+        uint64_t bz = block_max_u64(best_z, S);
+        if (tid == 0) {
+            int top = (int)(uint32_t)bl, minseg = (int)(uint32_t)bz;
+            S.size1 = 1;
+            S.sel[0] = top;
+            S.selcnt[0] = ld_l2(&A.cnt[top]);
+            if (minseg == top) { S.K = 1; S.self_pair = 1; }
+            else { S.K = 2; S.self_pair = 0; S.sel[1] = minseg; S.selcnt[1] = ld_l2(&A.cnt[minseg]); }
+        }
         __syncthreads();
-        if (!S.cont) break;
-
-        // ---- phase A: the round's segments
-        if (!plain && S.L == 1) {
-            // the single invalid segment, and the segment with the most empty
-            // rows, ties to the larger id (reorder/reorder.h:767-797)
-            uint64_t best_l = 0, best_z = 0;
-            for (int s = tid; s < nseg; s += GT) {
-                int c = ld_l2(&A.cnt[s]);
-                if (c != 0) best_l = 0x8000000000000000ull | (uint32_t)s;
-                uint64_t z = ((uint64_t)(uint32_t)(R - ld_l2(&A.nzr[s])) << 32) | (uint32_t)s;
-                best_z = z > best_z ? z : best_z;
-            }
-            uint64_t bl = block_max_u64(best_l, S);
-            uint64_t bz = block_max_u64(best_z, S);
-            if (tid == 0) {
-                int top = (int)(uint32_t)bl, minseg = (int)(uint32_t)bz;
-                S.size1 = 1;
-                S.sel[0] = top;
-                S.selcnt[0] = ld_l2(&A.cnt[top]);
-                if (minseg == top) { S.K = 1; S.self_pair = 1; }
-                else { S.K = 2; S.self_pair = 0; S.sel[1] = minseg; S.selcnt[1] = ld_l2(&A.cnt[minseg]); }
-            }
-            __syncthreads();
-        } else {
-            const int k = min(M, S.L - 1) + 1;  // targets + the primary itself
-            uint64_t tk[KMAX + 1];
+    } else {
+        const int k = min(M, Lr - 1) + 1;  // targets + the primary itself
+        uint64_t tk[KMAX + 1];
 #pragma unroll
-            for (int i = 0; i <= KMAX; ++i) tk[i] = 0;
-            uint64_t best_l = 0;
-            for (int s = tid; s < nseg; s += GT) {
-                int c = ld_l2(&A.cnt[s]);
+        for (int i = 0; i <= KMAX; ++i) tk[i] = 0;
+        uint64_t best_l = 0;
+        for (int s0 = tid; s0 < nseg; s0 += GT * 4) {
+            int cv[4];  // four loads in flight per thread
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cv[u] = s0 + u * GT < nseg ? ld_l2(&A.cnt[s0 + u * GT]) : 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int c = cv[u], s = s0 + u * GT;
                 if (c == 0) continue;
                 uint64_t lk = rev ? key_rev(s, c) : key_fwd(s, c);
                 best_l = lk > best_l ? lk : best_l;
@@ -779,126 +919,206 @@ __global__ __launch_bounds__(GT) void k_nm_greedy(NMArgs A) {
                     tk[i] = hi;
                 }
             }
-            uint64_t bl = block_max_u64(best_l, S);
-            block_topk(tk, k, S);
-            if (tid == 0) {
-                int primary = seg_of_key(bl, !rev);
-                S.size1 = 0;
-                S.self_pair = 0;
-                S.sel[0] = primary;
-                S.selcnt[0] = ld_l2(&A.cnt[primary]);
-                int K = 1;
-                for (int i = 0; i < k && K < k; ++i) {
-                    if (S.topk[i] == 0) break;
-                    int s = seg_of_key(S.topk[i], tfwd);
-                    if (s == primary) continue;
-                    S.sel[K] = s;
-                    S.selcnt[K] = ld_l2(&A.cnt[s]);
-                    ++K;
-                }
-                S.K = K;
-            }
-            __syncthreads();
         }
-
-        PHASE_MARK(0);
-        // ---- phase C: gains of every pair of the round's segments
-        greedy_score_phase(A, S);
-        PHASE_MARK(1);
-
-        // ---- phase D: priority queues: dense ranks (one wave per list), then
-        // one thread per list pushes all pairs and pops the first POPP
-        {
-            const int K = S.K;
-            const int nq = S.size1 ? 1 : (K - 1) * (K - 1);
-            for (int q = w; q < nq; q += NW) {
-                int a, b;
-                if (S.size1) { a = 0; b = S.self_pair ? 0 : 1; }
-                else q_pair(q, K, a, b);
-                int64_t key = 0;
-                if (lane < MM) {
-                    int lg, rg;
-                    pair_gain(S, K, M, a, b, lane, lg, rg);
-                    key = swap_key(lg, rg);
-                    S.wkeys[w][lane] = key;
-                }
-                __builtin_amdgcn_wave_barrier();
-                int rnk = 0;
-                if (lane < MM)
-                    for (int j = 0; j < MM; ++j) rnk += S.wkeys[w][j] < key;
-                if (lane < MM) S.rank[lane][q] = (uint8_t)rnk;
-                __builtin_amdgcn_wave_barrier();
-            }
-            __syncthreads();
-            if (tid < nq) {
-                HeapRef H{&S.heap[0][tid], &S.rank[0][tid], 0, NQ};
-                for (int e = 0; e < MM; ++e) heap_push(H, e);
-                const int npre = min(POPP, MM);
-                for (int k = 0; k < npre; ++k) S.popped[k][tid] = (uint8_t)heap_pop(H);
-                S.heapn[tid] = H.n;
-            }
-            __syncthreads();
-        }
-
-        PHASE_MARK(2);
-        // ---- phase E: the sequential greedy (one thread)
-        if (tid == 0) greedy_apply(S, M, rev, drain);
-        __syncthreads();
-        PHASE_MARK(3);
-        if (S.status != 0) break;
-
-        // ---- phase F: commit the swaps to perm/inv (reorder(), :392-418)
-        if (tid < S.nsw) {
-            int v1 = S.swv1[tid], v2 = S.swv2[tid];
-            if (v1 != v2) {
-                int b1 = ld_l2(&A.inv[v1]), b2 = ld_l2(&A.inv[v2]);
-                st_l2(&A.inv[v1], b2);
-                st_l2(&A.inv[v2], b1);
-                st_l2(&A.perm[b1], v2);
-                st_l2(&A.perm[b2], v1);
-            }
-            if (A.trace) {
-                int64_t n = S.trace_len + tid;
-                if (n < A.trace_cap) {
-                    int32_t *t = A.trace + n * 9;
-                    t[0] = S.round; t[1] = S.swp[tid]; t[2] = S.swt[tid];
-                    t[3] = v1; t[4] = v2; t[5] = S.swlg[tid]; t[6] = S.swrg[tid];
-                    t[7] = S.swpc[tid]; t[8] = S.swtc[tid];
-                }
-            }
-        }
-        __syncthreads();
-        if (tid == 0) S.trace_len += S.nsw;
-        __syncthreads();
-
-        PHASE_MARK(4);
-        // ---- phase G: recount the touched segments (locate_invalid_segment)
-        greedy_recount_phase(A, S);
-        PHASE_MARK(5);
+        uint64_t bl = block_max_u64(best_l, S);
+        block_topk(tk, k, S);
         if (tid == 0) {
-            for (int j = 0; j < S.K; ++j) {
-                int s = S.sel[j];
-                int oc = ld_l2(&A.cnt[s]);
-                int nc = S.newcnt[j];
-                st_l2(&A.cnt[s], nc);
-                st_l2(&A.nzr[s], S.newnzr[j]);
-                S.L += (nc != 0) - (oc != 0);
-                S.totcnt += nc - oc;
+            int primary = seg_of_key(bl, !rev);
+            S.size1 = 0;
+            S.self_pair = 0;
+            S.sel[0] = primary;
+            S.selcnt[0] = ld_l2(&A.cnt[primary]);
+            int K = 1;
+            for (int i = 0; i < k && K < k; ++i) {
+                if (S.topk[i] == 0) break;
+                int s = seg_of_key(S.topk[i], tfwd);
+                if (s == primary) continue;
+                S.sel[K] = s;
+                S.selcnt[K] = ld_l2(&A.cnt[s]);
+                ++K;
             }
-            S.round += 1;
+            S.K = K;
         }
         __syncthreads();
     }
+
+
+    const int K = S.K;
+    const int total = table_columns(S.sel, K, M, A.inv, A.cptr, S.cs.cols, S.cs.pre);
+    if (tid == 0) S.cs.total = total;
+    chunk_table(S.cs, K, M);
+    copy_words(&st->S, &S.cs, sizeof(ColSet));
+    if (tid < KMAX) {
+        st->sel[tid] = S.sel[tid];
+        st->selcnt[tid] = S.selcnt[tid];
+    }
     if (tid == 0) {
-        A.out[0] = S.init_size;
-        A.out[1] = S.L;
-        A.out[2] = S.init_cnt;
-        A.out[3] = S.totcnt;
-        A.out[4] = S.round - 1;
-        A.out[5] = S.status;
-        A.out[6] = (int32_t)min(S.work, (int64_t)0x7FFFFFFF);
-        A.out[7] = (int32_t)S.trace_len;
-        for (int i = 0; i < 6; ++i) A.out[8 + i] = (int32_t)min(S.phase_clk[i] / 100, (uint64_t)0x7FFFFFFF);
+        st->K = K;
+        st->size1 = S.size1;
+        st->self_pair = S.self_pair;
+        st->work += total;
+    }
+    clock_add(st, 0, t0);
+}
+
+// The round's sequential part: priority queues of every ordered pair of the
+// round's segments, the greedy (reorder/reorder.h:1837-1932), the commit to
+// perm/inv, and the columns of the segments after the commit.
+__global__ __launch_bounds__(GT) void k_gr_greedy(NMArgs A) {
+    __shared__ GreedyShared S;
+    GreedyState *st = A.st;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, w = tid >> 6;
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    if (tid == 0) S.cont = !st->done;
+    __syncthreads();
+    if (!S.cont) return;
+    const int M = st->M, R = st->R, nseg = st->nseg, MM = M * M;
+    const bool plain = st->sched >= 8;
+    const bool rev = (st->sched == 2 || st->sched == 3);
+    const bool drain = (st->sched == 1 || st->sched == 3);
+    const bool tfwd = drain ? rev : !rev;  // traversal order uses the forward key
+    const int min_size = plain ? 2 : 1;
+    (void)R; (void)nseg; (void)plain; (void)tfwd; (void)min_size;
+    if (tid < KMAX) {
+        S.sel[tid] = st->sel[tid];
+        S.selcnt[tid] = st->selcnt[tid];
+    }
+    if (tid == 0) {
+        S.K = st->K;
+        S.size1 = st->size1;
+        S.self_pair = st->self_pair;
+        S.status = 0;
+    }
+    // the pair gains summed by k_gr_score; zeroed for the next round
+    for (int t = tid; t < NACC; t += GT) {
+        const int v = st->acc[t];
+        st->acc[t] = 0;
+        if (t < KMAX * MMAX) (&S.E[0][0])[t] = v;
+        else (&S.both[0][0][0])[t - KMAX * MMAX] = v;
+    }
+    __syncthreads();
+
+    // ---- phase D: priority queues: dense ranks (one wave per list), then
+    // one thread per list pushes all pairs and pops the first POPP
+    {
+        const int K = S.K;
+        const int nq = S.size1 ? 1 : (K - 1) * (K - 1);
+        for (int q = w; q < nq; q += NW) {
+            int a, b;
+            if (S.size1) { a = 0; b = S.self_pair ? 0 : 1; }
+            else q_pair(q, K, a, b);
+            int64_t key = 0;
+            if (lane < MM) {
+                int lg, rg;
+                pair_gain(S, K, M, a, b, lane, lg, rg);
+                key = swap_key(lg, rg);
+                S.wkeys[w][lane] = key;
+            }
+            __builtin_amdgcn_wave_barrier();
+            int rnk = 0;
+            if (lane < MM)
+                for (int j = 0; j < MM; ++j) rnk += S.wkeys[w][j] < key;
+            if (lane < MM) S.rank[lane][q] = (uint8_t)rnk;
+            __builtin_amdgcn_wave_barrier();
+        }
+        __syncthreads();
+        if (tid < nq) {
+            HeapRef H{&S.heap[0][tid], &S.rank[0][tid], 0, NQ};
+            for (int e = 0; e < MM; ++e) heap_push(H, e);
+            const int npre = min(POPP, MM);
+            for (int k = 0; k < npre; ++k) S.popped[k][tid] = (uint8_t)heap_pop(H);
+            S.heapn[tid] = H.n;
+        }
+        __syncthreads();
+    }
+
+
+    // ---- phase E: the sequential greedy (one thread)
+    if (tid == 0) greedy_apply(S, M, rev, drain);
+    __syncthreads();
+    if (S.status != 0) {
+        if (tid == 0) {
+            st->status = S.status;
+            st->done = 1;
+        }
+        return;
+    }
+
+    // ---- phase F: commit the swaps to perm/inv (reorder(), :392-418)
+    if (tid < S.nsw) {
+        int v1 = S.swv1[tid], v2 = S.swv2[tid];
+        if (v1 != v2) {
+            int b1 = ld_l2(&A.inv[v1]), b2 = ld_l2(&A.inv[v2]);
+            st_l2(&A.inv[v1], b2);
+            st_l2(&A.inv[v2], b1);
+            st_l2(&A.perm[b1], v2);
+            st_l2(&A.perm[b2], v1);
+        }
+        if (st->trace) {
+            int64_t n = st->trace_len + tid;
+            if (n < st->trace_cap) {
+                int32_t *t = st->trace + n * 9;
+                t[0] = st->round; t[1] = S.swp[tid]; t[2] = S.swt[tid];
+                t[3] = v1; t[4] = v2; t[5] = S.swlg[tid]; t[6] = S.swrg[tid];
+                t[7] = S.swpc[tid]; t[8] = S.swtc[tid];
+            }
+        }
+    }
+    __syncthreads();
+    if (tid == 0) st->trace_len += S.nsw;
+
+
+    __syncthreads();
+    // columns of the round's segments after the commit, for the recount
+    const int K = S.K;
+    const int total = table_columns(S.sel, K, M, A.inv, A.cptr, S.cs.cols, S.cs.pre);
+    if (tid == 0) S.cs.total = total;
+    chunk_table(S.cs, K, M);
+    copy_words(&st->C, &S.cs, sizeof(ColSet));
+    if (tid < KMAX) {
+        st->newcnt[tid] = 0;
+        st->newnzr[tid] = 0;
+    }
+    if (tid == 0) {
+        st->pending = 1;
+        st->work += total;
+    }
+    clock_add(st, 3, t0);
+}
+
+// Start of a run: parameters, initial list size and count
+// (reorder/reorder.h:1808-1819).
+__global__ __launch_bounds__(GT) void k_gr_init(NMArgs A, int R, int M, int N, int nseg, int sched,
+                                                int maxiter, int round_cap, int32_t *trace,
+                                                int64_t trace_cap) {
+    __shared__ int sL, sC;
+    GreedyState *st = A.st;
+    const int tid = threadIdx.x;
+    if (tid == 0) { sL = 0; sC = 0; }
+    __syncthreads();
+    int lsum = 0, csum = 0;
+    for (int s = tid; s < nseg; s += GT) {
+        const int c = A.cnt[s];
+        lsum += c != 0;
+        csum += c;
+    }
+    if (lsum) atomicAdd(&sL, lsum);
+    if (csum) atomicAdd(&sC, csum);
+    for (int t = tid; t < NACC; t += GT) st->acc[t] = 0;
+    __syncthreads();
+    if (tid == 0) {
+        st->R = R; st->M = M; st->N = N; st->nseg = nseg; st->sched = sched;
+        st->round_cap = round_cap; st->trace = trace; st->trace_cap = trace_cap;
+        st->round = 1; st->L = sL; st->totcnt = sC;
+        st->init_size = sL; st->init_cnt = sC;
+        st->maxiter = maxiter > sL ? maxiter : sL;
+        st->status = 0; st->done = 0; st->pending = 0;
+        st->K = 0; st->self_pair = 0; st->size1 = 0;
+        st->trace_len = 0; st->work = 0;
+        st->S.K = 0; st->S.total = 0; st->S.cb[0] = 0;
+        st->C.K = 0; st->C.total = 0; st->C.cb[0] = 0;
+        for (int i = 0; i < 8; ++i) st->clk[i] = 0;
     }
 }
 
@@ -1007,6 +1227,14 @@ static int check_nm_args(vnm_graph *g, int N) {
     return VNM_OK;
 }
 
+// Grid of the record pass kernels: two workgroups per CU.
+static int pass_grid(vnm_graph *g) {
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g->ctx->device) != hipSuccess || ncu < 1)
+        ncu = 1;
+    return 2 * ncu;
+}
+
 int nm_reorder_ex(vnm_graph *g, int N, int sched, int maxiter, int round_cap, int32_t *d_trace,
                   int64_t trace_cap, int64_t *trace_len, vnm_nm_stats *stats) {
     int rc = check_nm_args(g, N);
@@ -1019,34 +1247,65 @@ int nm_reorder_ex(vnm_graph *g, int N, int sched, int maxiter, int round_cap, in
     if ((rc = build_csc(g))) return rc;
     if ((rc = locate_all(g, N, g->cnt.p, g->nzr.p))) return rc;
     hipStream_t st = g->ctx->stream;
-    int32_t *out = g->ws.p + NPAIR_T * 2 * 64;
+    const size_t st_words = (sizeof(GreedyState) + 3) / 4;
+    if (!g->gst.p && (rc = g->gst.alloc(st_words))) return rc;
+    if (!g->rt.p) {
+        // records A and B, 16 bytes per row each, zero between runs
+        if ((rc = g->rt.alloc((size_t)g->nrows * 8))) return rc;
+        VNM_HIP_TRY(hipMemsetAsync(g->rt.p, 0, (size_t)g->nrows * 32, st));
+    }
     NMArgs A;
-    A.R = g->nrows; A.M = g->M; A.N = N; A.nseg = g->nseg(); A.sched = sched; A.maxiter = maxiter;
+    A.st = (GreedyState *)g->gst.p;
     A.cptr = g->cptr.p; A.crow = g->crow.p; A.perm = g->perm.p; A.inv = g->inv.p;
-    A.cnt = g->cnt.p; A.nzr = g->nzr.p; A.tab = g->tab.p; A.out = out;
-    A.trace = d_trace; A.trace_cap = d_trace ? trace_cap : 0;
-    A.round_cap = round_cap > 0 ? round_cap : 0x7FFFFFFF;
-    KernelTimer kt(g->ctx, "k_nm_greedy");
-    hipLaunchKernelGGL(k_nm_greedy, dim3(1), dim3(GT), 0, st, A);
-    kt.stop();
+    A.cnt = g->cnt.p; A.nzr = g->nzr.p;
+    A.rta = g->rt.p; A.rtb = g->rt.p + (size_t)g->nrows * 4;
+    const int gp = pass_grid(g);
+    KernelTimer kt(g->ctx, "nm_greedy");
+    hipLaunchKernelGGL(k_gr_init, dim3(1), dim3(GT), 0, st, A, g->nrows, g->M, N, g->nseg(), sched, maxiter,
+                       round_cap > 0 ? round_cap : 0x7FFFFFFF, d_trace, d_trace ? trace_cap : (int64_t)0);
     g->ctx->launches++;
-    VNM_LAUNCH_CHECK("k_nm_greedy");
-    int32_t h[16];
-    VNM_HIP_TRY(hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, st));
+    VNM_LAUNCH_CHECK("k_gr_init");
+    // rounds are enqueued in growing batches; the kernels of rounds past the
+    // end return at entry
+    int32_t done = 0;
+    for (int batch = 4; !done; batch = std::min(batch * 2, 64)) {
+        for (int b = 0; b < batch; ++b) {
+            hipLaunchKernelGGL(k_gr_select, dim3(1), dim3(GT), 0, st, A);
+            hipLaunchKernelGGL(k_gr_fill_a, dim3(gp), dim3(PT), 0, st, A);
+            hipLaunchKernelGGL(k_gr_score, dim3(gp), dim3(PT), 0, st, A);
+            hipLaunchKernelGGL(k_gr_greedy, dim3(1), dim3(GT), 0, st, A);
+            hipLaunchKernelGGL(k_gr_fill_b, dim3(gp), dim3(PT), 0, st, A);
+            hipLaunchKernelGGL(k_gr_count, dim3(gp), dim3(PT), 0, st, A);
+        }
+        g->ctx->launches += 6 * batch;
+        VNM_LAUNCH_CHECK("N:M round kernels");
+        VNM_HIP_TRY(hipMemcpyAsync(&done, &A.st->done, 4, hipMemcpyDeviceToHost, st));
+        VNM_HIP_TRY(hipStreamSynchronize(st));
+    }
+    hipLaunchKernelGGL(k_gr_cleanup, dim3(gp), dim3(PT), 0, st, A);
+    g->ctx->launches++;
+    VNM_LAUNCH_CHECK("k_gr_cleanup");
+    kt.stop();
+    GreedyState h;
+    VNM_HIP_TRY(hipMemcpyAsync(&h, A.st, sizeof h, hipMemcpyDeviceToHost, st));
     VNM_HIP_TRY(hipStreamSynchronize(st));
     if (getenv("VNM_DEBUG"))
-        fprintf(stderr, "[vnm] greedy rounds=%d entries=%d phase_us A=%d C=%d D=%d E=%d F=%d G=%d\n",
-                h[4], h[6], h[8], h[9], h[10], h[11], h[12], h[13]);
-    if (h[5] == VNM_E_UB_PATH)
+        fprintf(stderr, "[vnm] greedy rounds=%d entries=%lld kernel_us select=%llu fill_a=%llu score=%llu "
+                        "greedy=%llu fill_b=%llu count=%llu\n",
+                h.round - 1, (long long)h.work, (unsigned long long)(h.clk[0] / 100),
+                (unsigned long long)(h.clk[1] / 100), (unsigned long long)(h.clk[2] / 100),
+                (unsigned long long)(h.clk[3] / 100), (unsigned long long)(h.clk[4] / 100),
+                (unsigned long long)(h.clk[5] / 100));
+    if (h.status == VNM_E_UB_PATH)
         return set_error(VNM_E_UB_PATH, "greedy reached top() of an empty swap queue "
                                         "(undefined in the reference)");
     if ((rc = relabel(g, g->perm.p))) return rc;
-    if (trace_len) *trace_len = h[7];
-    stats->init_size = h[0];
-    stats->final_size = h[1];
-    stats->init_cnt = h[2];
-    stats->final_cnt = h[3];
-    stats->rounds = h[4];
+    if (trace_len) *trace_len = h.trace_len;
+    stats->init_size = h.init_size;
+    stats->final_size = h.L;
+    stats->init_cnt = h.init_cnt;
+    stats->final_cnt = h.totcnt;
+    stats->rounds = h.round - 1;
     return VNM_OK;
 }
 

@@ -6,17 +6,23 @@
 // segment's M column lists (CSC over run-start ids, current column s*M + b =
 // run-start column inv[s*M + b]), read back once per entry, then cleared by
 // the same entries, so slots are all-zero between uses.
+//
+// The entry passes are latency-bound (a binary search in LDS, then a
+// dependent load of the entry's row): each thread resolves TAB_U entries
+// before it issues their loads, so TAB_U chains are in flight at once.
 #pragma once
 
 #include "hip_common.h"
 
 namespace vnm {
 
+constexpr int TAB_U = 4;
+
 // Column list of slot j, bit b: run-start column inv[seg*M + b].
 struct ColSpan { int start, len; };
 
-// Fill per-column spans for K segments (K*M columns) and their prefix sums.
-// Must be called by all threads; returns the total entry count.
+// Fill per-column spans for K segments (K*M <= 128 columns) and their prefix
+// sums.  Must be called by all threads; returns the total entry count.
 __device__ inline int table_columns(const int *sel, int K, int M, const int32_t *inv,
                              const int32_t *cptr, ColSpan *cols, int *pre) {
     const int nc = K * M;
@@ -28,13 +34,20 @@ __device__ inline int table_columns(const int *sel, int K, int M, const int32_t 
         cols[t].len = cptr[c0 + 1] - s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < nc; ++t) {
-            pre[t] = acc;
-            acc += cols[t].len;
+    if (threadIdx.x < 64) {
+        // exclusive scan of the lengths, two columns per lane
+        const int lane = threadIdx.x;
+        const int l0 = 2 * lane < nc ? cols[2 * lane].len : 0;
+        const int l1 = 2 * lane + 1 < nc ? cols[2 * lane + 1].len : 0;
+        int v = l0 + l1;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(v, d, 64);
+            if (lane >= d) v += t;
         }
-        pre[nc] = acc;
+        const int ex = v - l0 - l1;
+        if (2 * lane < nc) pre[2 * lane] = ex;
+        if (2 * lane + 1 < nc) pre[2 * lane + 1] = ex + l0;
+        if (lane == 63) pre[nc] = v;
     }
     __syncthreads();
     return pre[nc];
@@ -58,27 +71,52 @@ __device__ __forceinline__ uint32_t slot_byte(uint32_t *tab, int j, int R, int r
     return (ld_l2(slot_word(tab, j, R, r)) >> ((r & 3) * 8)) & 0xFFu;
 }
 
-__device__ inline void table_scatter(uint32_t *tab, const int *slot_of_col, int M, int R,
-                              const ColSpan *cols, const int *pre, int nc, int total,
-                              const int32_t *crow) {
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        int t = entry_col(pre, nc, e);
-        int j = slot_of_col ? slot_of_col[t] : t / M;
-        int b = t % M;
-        int r = crow[cols[t].start + (e - pre[t])];
-        atomicOr(slot_word(tab, j, R, r), (1u << (M - 1 - b)) << ((r & 3) * 8));
+// Resolve up to TAB_U entries of this thread (entries e0, e0 + stride, ...):
+// their column index t (-1 past total) and row.
+__device__ __forceinline__ void table_entries(const ColSpan *cols, const int *pre, int nc, int total,
+                                              const int32_t *crow, int e0, int stride,
+                                              int (&t)[TAB_U], int (&r)[TAB_U]) {
+#pragma unroll
+    for (int u = 0; u < TAB_U; ++u) {
+        const int e = e0 + u * stride;
+        t[u] = e < total ? entry_col(pre, nc, e) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < TAB_U; ++u) {
+        const int e = e0 + u * stride;
+        r[u] = t[u] >= 0 ? crow[cols[t[u]].start + (e - pre[t[u]])] : 0;
     }
 }
 
-__device__ inline void table_clear(uint32_t *tab, const int *slot_of_col, int M, int R,
-                            const ColSpan *cols, const int *pre, int nc, int total,
-                            const int32_t *crow) {
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        int t = entry_col(pre, nc, e);
-        int j = slot_of_col ? slot_of_col[t] : t / M;
-        int r = crow[cols[t].start + (e - pre[t])];
-        st_l2(slot_word(tab, j, R, r), 0u);
+// One pass over this workgroup's share (part of parts) of the entries of
+// nc columns: set (CLEAR = false) or clear the row bytes of slots slot0 +
+// t / M.  Entry e belongs to part (e / blockDim.x) % parts.
+template <bool CLEAR>
+__device__ inline void table_pass(uint32_t *tab, int slot0, int M, int R, const ColSpan *cols,
+                                  const int *pre, int nc, int total, const int32_t *crow, int part,
+                                  int parts) {
+    const int stride = parts * blockDim.x;
+    for (int e0 = part * blockDim.x + threadIdx.x; e0 < total; e0 += stride * TAB_U) {
+        int t[TAB_U], r[TAB_U];
+        table_entries(cols, pre, nc, total, crow, e0, stride, t, r);
+#pragma unroll
+        for (int u = 0; u < TAB_U; ++u) {
+            if (t[u] < 0) continue;
+            uint32_t *w = slot_word(tab, slot0 + t[u] / M, R, r[u]);
+            if constexpr (CLEAR) st_l2(w, 0u);
+            else atomicOr(w, (1u << (M - 1 - t[u] % M)) << ((r[u] & 3) * 8));
+        }
     }
+}
+
+__device__ inline void table_scatter(uint32_t *tab, int M, int R, const ColSpan *cols, const int *pre,
+                                     int nc, int total, const int32_t *crow) {
+    table_pass<false>(tab, 0, M, R, cols, pre, nc, total, crow, 0, 1);
+}
+
+__device__ inline void table_clear(uint32_t *tab, int M, int R, const ColSpan *cols, const int *pre,
+                                   int nc, int total, const int32_t *crow) {
+    table_pass<true>(tab, 0, M, R, cols, pre, nc, total, crow, 0, 1);
 }
 
 }  // namespace vnm
