@@ -46,55 +46,56 @@ __device__ __forceinline__ int64_t swap_key(int lg, int rg) {
 }
 
 // std::priority_queue<SwapPair*> with the non-strict Comparator, emulated
-// exactly as libstdc++ pushes (__push_heap) and pops (pop_heap ->
-// __adjust_heap -> __push_heap).  The heap holds element indices; each
-// element carries its dense rank among the list's keys (the number of
-// strictly smaller keys), so rank[a] <= rank[b] exactly when
-// key[a] <= key[b], i.e. when Comparator(a, b) holds.
+// exactly as libstdc++ pushes (push_back + __push_heap) and pops (pop_heap ->
+// __adjust_heap -> __push_heap).  A heap entry packs rank << 8 | element,
+// where rank is the element's dense rank among the list's keys (the number
+// of strictly smaller keys), so Comparator(a, b) holds exactly when
+// rank(a) <= rank(b) and each comparison reads one entry.  One thread owns a
+// heap; slot s sits at h[s * stride], so the threads of a wave working on
+// interleaved heaps touch consecutive LDS words.
 struct HeapRef {
-    uint8_t *h;           // heap slots (element indices), slot s at h[s * stride]
-    const uint8_t *rank;  // rank of element e at rank[e * stride]
-    int n;                // current size
-    int stride;           // interleave of several heaps (conflict-free LDS)
+    uint16_t *h;
+    int n;       // current size
+    int stride;
 };
 
-__device__ __forceinline__ int hget(const HeapRef &H, int slot) { return H.h[slot * H.stride]; }
-__device__ __forceinline__ void hset(HeapRef &H, int slot, int v) { H.h[slot * H.stride] = (uint8_t)v; }
+__device__ __forceinline__ uint32_t hget(const HeapRef &H, int slot) { return H.h[slot * H.stride]; }
+__device__ __forceinline__ void hset(HeapRef &H, int slot, uint32_t v) { H.h[slot * H.stride] = (uint16_t)v; }
+__device__ __forceinline__ bool heap_le(uint32_t a, uint32_t b) { return (a >> 8) <= (b >> 8); }
 
-__device__ __forceinline__ bool heap_less(const HeapRef &H, int a, int b) {
-    return H.rank[a * H.stride] <= H.rank[b * H.stride];
-}
-
-__device__ __forceinline__ void heap_sift_up(HeapRef &H, int hole, int top, int value) {
+__device__ __forceinline__ void heap_sift_up(HeapRef &H, int hole, int top, uint32_t value) {
     int parent = (hole - 1) / 2;
-    while (hole > top && heap_less(H, hget(H, parent), value)) {
-        hset(H, hole, hget(H, parent));
+    while (hole > top) {
+        const uint32_t pv = hget(H, parent);
+        if (!heap_le(pv, value)) break;
+        hset(H, hole, pv);
         hole = parent;
         parent = (hole - 1) / 2;
     }
     hset(H, hole, value);
 }
 
-__device__ __forceinline__ void heap_push(HeapRef &H, int value) {
-    hset(H, H.n, value);
+__device__ __forceinline__ void heap_push(HeapRef &H, uint32_t value) {
     H.n += 1;
     heap_sift_up(H, H.n - 1, 0, value);
 }
 
-// Returns the popped element (the former top).
-__device__ __forceinline__ int heap_pop(HeapRef &H) {
-    int top_el = hget(H, 0);
-    int n = H.n;
+// Returns the popped entry (the former top).
+__device__ __forceinline__ uint32_t heap_pop(HeapRef &H) {
+    const uint32_t top_el = hget(H, 0);
+    const int n = H.n;
     if (n > 1) {
-        int last = n - 1;
-        int value = hget(H, last);
+        const int last = n - 1;
+        const uint32_t value = hget(H, last);
         hset(H, last, top_el);
         // __adjust_heap(first, 0, len = last, value)
         int len = last, hole = 0, second = 0;
         while (second < (len - 1) / 2) {
             second = 2 * (second + 1);
-            if (heap_less(H, hget(H, second), hget(H, second - 1))) second--;
-            hset(H, hole, hget(H, second));
+            const uint32_t r = hget(H, second), l = hget(H, second - 1);
+            uint32_t c = r;
+            if (heap_le(r, l)) { second--; c = l; }
+            hset(H, hole, c);
             hole = second;
         }
         if ((len & 1) == 0 && second == (len - 2) / 2) {

@@ -246,26 +246,23 @@ __global__ __launch_bounds__(SC_ROWS) void k_score_rows(int R, int M, int N, int
 
 // ------------------------------------------------------------ heap pop order
 
+// One thread per list: dense ranks of the list's M*M keys, then the same
+// heap code as the greedy pushes every pair and pops them all.
 __global__ void k_pq_pop_order(const int32_t *__restrict__ scores, int count, int M,
                                int32_t *__restrict__ order) {
-    extern __shared__ unsigned char smem[];
+    extern __shared__ uint16_t hsm[];  // slot s of thread t at [s * blockDim.x + t]
     const int MM = M * M;
-    // slot-major layout: heap slot s of thread t at [s * blockDim.x + t]
-    uint8_t *ranks = smem;
-    uint8_t *heaps = smem + (size_t)blockDim.x * MM;
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= count) return;
     const int32_t *sc = scores + (size_t)q * 2 * MM;
-    uint8_t *rank = ranks + threadIdx.x;
+    HeapRef H{hsm + threadIdx.x, 0, (int)blockDim.x};
     for (int i = 0; i < MM; ++i) {
-        int64_t ki = swap_key(sc[2 * i], sc[2 * i + 1]);
+        const int64_t ki = swap_key(sc[2 * i], sc[2 * i + 1]);
         int r = 0;
         for (int j = 0; j < MM; ++j) r += swap_key(sc[2 * j], sc[2 * j + 1]) < ki;
-        rank[i * blockDim.x] = (uint8_t)r;
+        heap_push(H, (uint32_t)((r << 8) | i));
     }
-    HeapRef H{heaps + threadIdx.x, rank, 0, (int)blockDim.x};
-    for (int i = 0; i < MM; ++i) heap_push(H, i);
-    for (int k = 0; k < MM; ++k) order[(size_t)q * MM + k] = heap_pop(H);
+    for (int k = 0; k < MM; ++k) order[(size_t)q * MM + k] = (int32_t)(heap_pop(H) & 0xFFu);
 }
 
 // ------------------------------------------------------------ relabel
@@ -315,6 +312,9 @@ constexpr int MMAX = 8;
 constexpr int NPAIR_T = KMAX * (KMAX - 1) / 2 + 1;  // unordered pairs + one self pair
 constexpr int NQ = (KMAX - 1) * (KMAX - 1);         // ordered pairs the greedy can visit
 constexpr int POPP = 16;                            // pops precomputed per ordered pair
+constexpr int DEEP_AFTER = 8;                       // heap pops in a round's greedy that make the
+                                                    // next round precompute whole pop orders
+static_assert(NQ <= 64, "one heap thread per ordered pair, all in wave 0");
 constexpr int NACC = KMAX * MMAX + NPAIR_T * 2 * MMAX * MMAX;  // E and both, flattened
 
 // The columns of one set of records: slot j, bit b = column j*M + b.
@@ -333,13 +333,14 @@ struct GreedyState {
     int32_t *trace;  // optional: 9 ints per swap
     // run state
     int round, L, totcnt, maxiter, init_size, init_cnt, status, done, pending;
+    int deep;  // the last round's greedy popped heaps often: precompute whole pop orders
     int K, self_pair, size1;
     int sel[KMAX], selcnt[KMAX];
     int64_t trace_len, work;
     ColSet S, C;  // columns of records A (score) and B (recount)
     int acc[NACC];
     int newcnt[KMAX], newnzr[KMAX];
-    uint64_t clk[8];
+    uint64_t clk[16];
 };
 
 // Kernel arguments of every round kernel: pointers fixed for the graph.
@@ -361,13 +362,13 @@ struct GreedyShared {
     //   Bl[b1*M+b2] = sum over rows in both of y_b2 * (x_b1 ? A : B)
     //   Br[b1*M+b2] = sum over rows in both of x_b1 * (y_b2 ? D : C)
     int both[NPAIR_T][2][MMAX * MMAX];
-    // slot-major (element / slot / pop index first, list second), so the
-    // lanes of a wave building different lists touch consecutive bytes
-    uint8_t rank[MMAX * MMAX][NQ];
-    uint8_t heap[MMAX * MMAX][NQ];
-    uint8_t popped[POPP][NQ];
+    // per ordered pair q: the heap (slot-major, lists interleaved), its first
+    // first npre pops, the next of those not yet consumed, and the heap's size
+    uint16_t heap[MMAX * MMAX][NQ];
+    uint8_t popped[MMAX * MMAX][NQ];
+    int qcur[NQ];
     int heapn[NQ];
-    uint8_t vis[KMAX * MMAX];
+    int npre, heap_pops;  // pops precomputed this round; pops the greedy took from heaps
     int swv1[64], swv2[64], swlg[64], swrg[64], swp[64], swt[64], swpc[64], swtc[64];
     int nsw;
     uint64_t wtop[NW][KMAX + 1];
@@ -386,19 +387,33 @@ __device__ __forceinline__ int seg_of_key(uint64_t k, bool fwd) {
     uint32_t lo = (uint32_t)k;
     return fwd ? (int)(0xFFFFFFFFu - lo) : (int)lo;
 }
+__device__ __forceinline__ int cnt_of_key(uint64_t k, bool fwd) {
+    const int hi = (int)(uint32_t)(k >> 32);
+    return fwd ? hi : 0x7FFFFFFF - hi;
+}
 
-__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
-    uint32_t lo = __shfl((uint32_t)v, src, 64), hi = __shfl((uint32_t)(v >> 32), src, 64);
-    return ((uint64_t)hi << 32) | lo;
+// Maximum over the wave, every lane gets it: a running maximum along DPP
+// row shifts (1, 2, 4, 8 within rows of 16), then row broadcasts 15 and 31;
+// lane 63 ends with the maximum.  Lanes without a DPP source read 0, the
+// identity of an unsigned maximum.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp_max_step(uint64_t v) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, ROWS, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, ROWS, 0xF, false);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    return o > v ? o : v;
 }
 
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
-    const int lane = threadIdx.x & 63;
-    for (int d = 32; d >= 1; d >>= 1) {
-        uint64_t o = shfl_u64(v, lane ^ d);
-        v = o > v ? o : v;
-    }
-    return v;
+    v = dpp_max_step<0x111, 0xF>(v);  // row_shr:1
+    v = dpp_max_step<0x112, 0xF>(v);  // row_shr:2
+    v = dpp_max_step<0x114, 0xF>(v);  // row_shr:4
+    v = dpp_max_step<0x118, 0xF>(v);  // row_shr:8
+    v = dpp_max_step<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+    v = dpp_max_step<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 63);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Top-k (k <= KMAX + 1) keys of the block into S.topk (0 = none); tk holds
@@ -739,18 +754,54 @@ __global__ __launch_bounds__(PT) void k_gr_count(NMArgs A) {
     clock_add(st, 5, t0);
 }
 
-// First element of ordered pair q (slots a -> b) in pop order whose two
-// vertices are unvisited; -1 when the queue runs empty.
-__device__ __forceinline__ int first_free(GreedyShared &S, int q, int a, int b, int M) {
-    const int npre = min(POPP, M * M);
-    for (int k = 0; k < npre; ++k) {
-        int e = S.popped[k][q];
-        if (!S.vis[a * M + e / M] && !S.vis[b * M + e % M]) return e;
+// Visited vertices of a round: bit a*M + v for vertex v of slot a (< 72).
+struct Visited {
+    uint64_t lo = 0, hi = 0;
+    __device__ __forceinline__ bool has(int i) const {
+        return ((i < 64 ? lo >> i : hi >> (i - 64)) & 1ull) != 0;
     }
-    HeapRef H{&S.heap[0][q], &S.rank[0][q], S.heapn[q], NQ};
+    __device__ __forceinline__ void set(int i) {
+        if (i < 64) lo |= 1ull << i;
+        else hi |= 1ull << (i - 64);
+    }
+    // every vertex of slot a (M divides 64, so a slot's bits share a word)
+    __device__ __forceinline__ bool full(int a, int M) const {
+        const int i = a * M;
+        const uint64_t m = (1ull << M) - 1;
+        return (((i < 64 ? lo >> i : hi >> (i - 64))) & m) == m;
+    }
+};
+
+// First element of ordered pair q (slots a -> b) in pop order whose two
+// vertices are unvisited; -1 when the queue runs empty.  Skipped pops are
+// consumed (a visited vertex stays visited for the round), so each queue
+// resumes where its last search stopped: first the precomputed pops, one
+// per lane and a ballot, then the heap itself.  Called by a whole wave with
+// the same arguments in every lane.
+__device__ __forceinline__ int first_free(GreedyShared &S, int q, int a, int b, int M, const Visited &vis) {
+    const int lane = threadIdx.x & 63;
+    const int npre = S.npre;  // <= 64
+    const int k0 = S.qcur[q];
+    if (k0 < npre) {
+        const int k = k0 + lane;
+        const int e = k < npre ? (int)S.popped[k][q] : 0;
+        const bool ok = k < npre && !vis.has(a * M + e / M) && !vis.has(b * M + e % M);
+        const uint64_t m = __ballot(ok);
+        if (m) {
+            const int i = __builtin_ctzll(m);
+            S.qcur[q] = k0 + i;
+            return __builtin_amdgcn_readlane(e, i);
+        }
+        S.qcur[q] = npre;
+    }
+    HeapRef H{&S.heap[0][q], S.heapn[q], NQ};
     while (H.n > 0) {
-        int e = heap_pop(H);
-        if (!S.vis[a * M + e / M] && !S.vis[b * M + e % M]) { S.heapn[q] = H.n; return e; }
+        const int e = (int)(heap_pop(H) & 0xFFu);
+        S.heap_pops += 1;
+        if (!vis.has(a * M + e / M) && !vis.has(b * M + e % M)) {
+            S.heapn[q] = H.n;
+            return e;
+        }
     }
     S.heapn[q] = 0;
     return -1;
@@ -762,13 +813,16 @@ __device__ __forceinline__ bool list_before(int sa, int ca, int sb, int cb, bool
     return ca != cb ? ca < cb : sa > sb;
 }
 
-// The sequential part of one round (reorder/reorder.h:1837-1932), one thread.
-__device__ __forceinline__ void greedy_apply(GreedyShared &S, int M, bool rev, bool drain) {
+// The sequential part of one round (reorder/reorder.h:1837-1932), run by one
+// wave in lockstep: every lane holds the same values and writes the same
+// LDS words, and first_free spreads its search over the lanes.  Returns the
+// number of swaps.
+__device__ __forceinline__ int greedy_apply(GreedyShared &S, int M, bool rev, bool drain) {
     const int K = S.K;
-    S.nsw = 0;
-    for (int i = 0; i < K * M; ++i) S.vis[i] = 0;
+    int nsw = 0;
+    Visited vis;
     auto add_swap = [&](int a, int b, int e, int lg, int rg, int pc, int tc) {
-        int n = S.nsw++;
+        const int n = nsw++;
         S.swv1[n] = S.sel[a] * M + e / M;
         S.swv2[n] = S.sel[b] * M + e % M;
         S.swlg[n] = lg; S.swrg[n] = rg;
@@ -783,7 +837,7 @@ __device__ __forceinline__ void greedy_apply(GreedyShared &S, int M, bool rev, b
         int lg, rg;
         pair_gain(S, K, M, 0, b, e, lg, rg);
         add_swap(0, b, e, lg, rg, 0, 0);
-        return;
+        return nsw;
     }
     int cur = 0, pc = S.selcnt[0];
     int tgt[KMAX], tcnt[KMAX], nt = K - 1;
@@ -791,22 +845,20 @@ __device__ __forceinline__ void greedy_apply(GreedyShared &S, int M, bool rev, b
     while (true) {
         int upd[KMAX], ucnt[KMAX], nu = 0;
         for (int i = 0; i < nt; ++i) {
-            bool all = true;
-            for (int v = 0; v < M; ++v) all = all && S.vis[cur * M + v];
-            if (pc <= 0 || all) break;  // reorder/reorder.h:1878
+            if (pc <= 0 || vis.full(cur, M)) break;  // reorder/reorder.h:1878
             const int t = tgt[i];
             const int q = q_index(cur, t, K);
-            int e = first_free(S, q, cur, t, M);
+            int e = first_free(S, q, cur, t, M, vis);
             if (e < 0) {
                 // top() of an empty queue: the reference would read the last
                 // popped pair, which has a visited endpoint (:1908)
                 S.status = VNM_E_UB_PATH;
-                return;
+                return nsw;
             }
             int lg, rg;
             pair_gain(S, K, M, cur, t, e, lg, rg);
-            S.vis[cur * M + e / M] = 1;
-            S.vis[t * M + e % M] = 1;
+            vis.set(cur * M + e / M);
+            vis.set(t * M + e % M);
             pc -= lg;
             int tc = tcnt[i] - rg;
             add_swap(cur, t, e, lg, rg, pc, tc);
@@ -829,6 +881,7 @@ __device__ __forceinline__ void greedy_apply(GreedyShared &S, int M, bool rev, b
             tcnt[i] = ucnt[src];
         }
     }
+    return nsw;
 }
 
 // First kernel of a round: commit the last round's recount, decide whether
@@ -851,16 +904,23 @@ __global__ __launch_bounds__(GT) void k_gr_select(NMArgs A) {
     const bool tfwd = drain ? rev : !rev;  // traversal order uses the forward key
     const int min_size = plain ? 2 : 1;
     (void)lane; (void)w; (void)MM;
+    // the last round's segments (distinct), one thread each
+    if (tid == 0) { S.K = 0; S.L = 0; S.status = 0; }
+    __syncthreads();
+    const int pending = st->pending;
+    if (pending && tid < st->K) {
+        const int s = st->sel[tid];
+        const int oc = ld_l2(&A.cnt[s]), nc = st->newcnt[tid];
+        st_l2(&A.cnt[s], nc);
+        st_l2(&A.nzr[s], st->newnzr[tid]);
+        atomicAdd(&S.L, (nc != 0) - (oc != 0));
+        atomicAdd(&S.status, nc - oc);
+    }
+    __syncthreads();
     if (tid == 0) {
-        if (st->pending) {
-            for (int j = 0; j < st->K; ++j) {
-                const int s = st->sel[j];
-                const int oc = ld_l2(&A.cnt[s]), nc = st->newcnt[j];
-                st_l2(&A.cnt[s], nc);
-                st_l2(&A.nzr[s], st->newnzr[j]);
-                st->L += (nc != 0) - (oc != 0);
-                st->totcnt += nc - oc;
-            }
+        if (pending) {
+            st->L += S.L;
+            st->totcnt += S.status;
             st->round += 1;
             st->pending = 0;
         }
@@ -872,6 +932,7 @@ __global__ __launch_bounds__(GT) void k_gr_select(NMArgs A) {
     __syncthreads();
     if (!S.cont) return;
     const int Lr = S.L;
+    const uint64_t tA = __builtin_amdgcn_s_memrealtime();
 
     // ---- phase A: the round's segments
     if (!plain && Lr == 1) {
@@ -927,14 +988,14 @@ __global__ __launch_bounds__(GT) void k_gr_select(NMArgs A) {
             S.size1 = 0;
             S.self_pair = 0;
             S.sel[0] = primary;
-            S.selcnt[0] = ld_l2(&A.cnt[primary]);
+            S.selcnt[0] = cnt_of_key(bl, !rev);
             int K = 1;
             for (int i = 0; i < k && K < k; ++i) {
                 if (S.topk[i] == 0) break;
                 int s = seg_of_key(S.topk[i], tfwd);
                 if (s == primary) continue;
                 S.sel[K] = s;
-                S.selcnt[K] = ld_l2(&A.cnt[s]);
+                S.selcnt[K] = cnt_of_key(S.topk[i], tfwd);
                 ++K;
             }
             S.K = K;
@@ -943,10 +1004,16 @@ __global__ __launch_bounds__(GT) void k_gr_select(NMArgs A) {
     }
 
 
+    const uint64_t tB = __builtin_amdgcn_s_memrealtime();
     const int K = S.K;
     const int total = table_columns(S.sel, K, M, A.inv, A.cptr, S.cs.cols, S.cs.pre);
     if (tid == 0) S.cs.total = total;
     chunk_table(S.cs, K, M);
+    if (tid == 0) {
+        st->clk[8] += tA - t0;
+        st->clk[9] += tB - tA;
+        st->clk[10] += __builtin_amdgcn_s_memrealtime() - tB;
+    }
     copy_words(&st->S, &S.cs, sizeof(ColSet));
     if (tid < KMAX) {
         st->sel[tid] = S.sel[tid];
@@ -999,8 +1066,10 @@ __global__ __launch_bounds__(GT) void k_gr_greedy(NMArgs A) {
     }
     __syncthreads();
 
-    // ---- phase D: priority queues: dense ranks (one wave per list), then
-    // one thread per list pushes all pairs and pops the first POPP
+    const uint64_t tD = __builtin_amdgcn_s_memrealtime();
+    // ---- phase D: priority queues: dense ranks (one wave per list), then one
+    // thread per list pushes all pairs and pops the first POPP (all of them
+    // after a round whose greedy popped the heaps often)
     {
         const int K = S.K;
         const int nq = S.size1 ? 1 : (K - 1) * (K - 1);
@@ -1019,23 +1088,41 @@ __global__ __launch_bounds__(GT) void k_gr_greedy(NMArgs A) {
             int rnk = 0;
             if (lane < MM)
                 for (int j = 0; j < MM; ++j) rnk += S.wkeys[w][j] < key;
-            if (lane < MM) S.rank[lane][q] = (uint8_t)rnk;
+            // the push order is element order: stage entry e in slot e
+            if (lane < MM) S.heap[lane][q] = (uint16_t)((rnk << 8) | lane);
             __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
+        if (tid == 0) st->clk[11] += __builtin_amdgcn_s_memrealtime() - tD;
+        // one thread per list, all in wave 0: the heap steps are bound by
+        // instruction issue, so 64 lists share every instruction
+        const int npre = st->deep ? MM : min(POPP, MM);
+        if (tid == 0) {
+            S.npre = npre;
+            S.heap_pops = 0;
+        }
         if (tid < nq) {
-            HeapRef H{&S.heap[0][tid], &S.rank[0][tid], 0, NQ};
-            for (int e = 0; e < MM; ++e) heap_push(H, e);
-            const int npre = min(POPP, MM);
-            for (int k = 0; k < npre; ++k) S.popped[k][tid] = (uint8_t)heap_pop(H);
+            HeapRef H{&S.heap[0][tid], 0, NQ};
+            for (int e = 0; e < MM; ++e) heap_push(H, hget(H, e));
+            for (int k = 0; k < npre; ++k) S.popped[k][tid] = (uint8_t)(heap_pop(H) & 0xFFu);
             S.heapn[tid] = H.n;
+            S.qcur[tid] = 0;
         }
         __syncthreads();
     }
 
-
-    // ---- phase E: the sequential greedy (one thread)
-    if (tid == 0) greedy_apply(S, M, rev, drain);
+    const uint64_t tE = __builtin_amdgcn_s_memrealtime();
+    // ---- phase E: the sequential greedy (wave 0 in lockstep)
+    if (w == 0) {
+        const int nsw = greedy_apply(S, M, rev, drain);
+        if (lane == 0) {
+            S.nsw = nsw;
+            st->deep = S.heap_pops > DEEP_AFTER;
+            const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
+            st->clk[6] += tE - tD;
+            st->clk[7] += t1 - tE;
+        }
+    }
     __syncthreads();
     if (S.status != 0) {
         if (tid == 0) {
@@ -1113,12 +1200,12 @@ __global__ __launch_bounds__(GT) void k_gr_init(NMArgs A, int R, int M, int N, i
         st->round = 1; st->L = sL; st->totcnt = sC;
         st->init_size = sL; st->init_cnt = sC;
         st->maxiter = maxiter > sL ? maxiter : sL;
-        st->status = 0; st->done = 0; st->pending = 0;
+        st->status = 0; st->done = 0; st->pending = 0; st->deep = 0;
         st->K = 0; st->self_pair = 0; st->size1 = 0;
         st->trace_len = 0; st->work = 0;
         st->S.K = 0; st->S.total = 0; st->S.cb[0] = 0;
         st->C.K = 0; st->C.total = 0; st->C.cb[0] = 0;
-        for (int i = 0; i < 8; ++i) st->clk[i] = 0;
+        for (int i = 0; i < 16; ++i) st->clk[i] = 0;
     }
 }
 
@@ -1291,11 +1378,16 @@ int nm_reorder_ex(vnm_graph *g, int N, int sched, int maxiter, int round_cap, in
     VNM_HIP_TRY(hipStreamSynchronize(st));
     if (getenv("VNM_DEBUG"))
         fprintf(stderr, "[vnm] greedy rounds=%d entries=%lld kernel_us select=%llu fill_a=%llu score=%llu "
-                        "greedy=%llu fill_b=%llu count=%llu\n",
+                        "greedy=%llu (heaps %llu, sequential %llu) fill_b=%llu count=%llu\n",
                 h.round - 1, (long long)h.work, (unsigned long long)(h.clk[0] / 100),
                 (unsigned long long)(h.clk[1] / 100), (unsigned long long)(h.clk[2] / 100),
-                (unsigned long long)(h.clk[3] / 100), (unsigned long long)(h.clk[4] / 100),
+                (unsigned long long)(h.clk[3] / 100), (unsigned long long)(h.clk[6] / 100),
+                (unsigned long long)(h.clk[7] / 100), (unsigned long long)(h.clk[4] / 100),
                 (unsigned long long)(h.clk[5] / 100));
+    if (getenv("VNM_DEBUG"))
+        fprintf(stderr, "[vnm]   select: commit %llu, segments %llu, columns %llu; heaps: ranks %llu\n",
+                (unsigned long long)(h.clk[8] / 100), (unsigned long long)(h.clk[9] / 100),
+                (unsigned long long)(h.clk[10] / 100), (unsigned long long)(h.clk[11] / 100));
     if (h.status == VNM_E_UB_PATH)
         return set_error(VNM_E_UB_PATH, "greedy reached top() of an empty swap queue "
                                         "(undefined in the reference)");
@@ -1386,9 +1478,8 @@ int vnm_pq_pop_order(vnm_ctx *ctx, const int32_t *scores, int count, int M, int3
     hipStream_t st = ctx->stream;
     VNM_HIP_TRY(hipMemcpyAsync(ds.p, scores, (size_t)count * 2 * MM * 4, hipMemcpyHostToDevice, st));
     const int bt = 64;
-    size_t shm = (size_t)bt * MM * 2;
-    hipLaunchKernelGGL(k_pq_pop_order, dim3((count + bt - 1) / bt), dim3(bt), shm, st, ds.p, count,
-                       M, dord.p);
+    hipLaunchKernelGGL(k_pq_pop_order, dim3((count + bt - 1) / bt), dim3(bt), (size_t)bt * MM * 2, st, ds.p,
+                       count, M, dord.p);
     ctx->launches++;
     VNM_LAUNCH_CHECK("k_pq_pop_order");
     VNM_HIP_TRY(hipMemcpyAsync(order, dord.p, (size_t)count * MM * 4, hipMemcpyDeviceToHost, st));
