@@ -76,6 +76,10 @@ struct vnm_graph {
     vnm::DBuf<uint32_t> tab;              // per-slot row bitmaps (bytes in words)
     vnm::DBuf<uint32_t> rt;               // N:M greedy row records A and B (16 B per row each)
     vnm::DBuf<int32_t> gst;               // N:M greedy device state
+    // a batch of N:M greedy rounds captured once (its kernels read only the
+    // buffers above, which live as long as the graph)
+    hipGraph_t nm_graph = nullptr;
+    hipGraphExec_t nm_exec = nullptr;
     size_t tab_slots = 0;
     // bumped whenever the COO is relabelled or re-sorted; caches built from
     // the COO (the proposer's adjacency) are valid for one epoch
@@ -85,6 +89,8 @@ struct vnm_graph {
     int32_t nseg() const { return (nrows + M - 1) / M; }
     ~vnm_graph() {
         if (lp_cache_free) lp_cache_free(lp_cache);
+        if (nm_exec) (void)hipGraphExecDestroy(nm_exec);
+        if (nm_graph) (void)hipGraphDestroy(nm_graph);
     }
 };
 

@@ -1314,6 +1314,37 @@ static int check_nm_args(vnm_graph *g, int N) {
     return VNM_OK;
 }
 
+constexpr int ROUND_BATCH = 16;  // greedy rounds per captured graph launch
+
+// Capture ROUND_BATCH rounds (six kernels each) on the context stream.
+static int capture_rounds(vnm_graph *g, const NMArgs &A, int gp) {
+    hipStream_t st = g->ctx->stream;
+    VNM_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int b = 0; b < ROUND_BATCH; ++b) {
+        hipLaunchKernelGGL(k_gr_select, dim3(1), dim3(GT), 0, st, A);
+        hipLaunchKernelGGL(k_gr_fill_a, dim3(gp), dim3(PT), 0, st, A);
+        hipLaunchKernelGGL(k_gr_score, dim3(gp), dim3(PT), 0, st, A);
+        hipLaunchKernelGGL(k_gr_greedy, dim3(1), dim3(GT), 0, st, A);
+        hipLaunchKernelGGL(k_gr_fill_b, dim3(gp), dim3(PT), 0, st, A);
+        hipLaunchKernelGGL(k_gr_count, dim3(gp), dim3(PT), 0, st, A);
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return set_error(VNM_E_HIP, "capture of the N:M round kernels failed: %s", hipGetErrorString(e));
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return set_error(VNM_E_HIP, "instantiating the N:M round graph failed: %s", hipGetErrorString(e));
+    }
+    g->nm_graph = graph;
+    g->nm_exec = exec;
+    return VNM_OK;
+}
+
 // Grid of the record pass kernels: two workgroups per CU.
 static int pass_grid(vnm_graph *g) {
     int ncu = 0;
@@ -1352,20 +1383,13 @@ int nm_reorder_ex(vnm_graph *g, int N, int sched, int maxiter, int round_cap, in
                        round_cap > 0 ? round_cap : 0x7FFFFFFF, d_trace, d_trace ? trace_cap : (int64_t)0);
     g->ctx->launches++;
     VNM_LAUNCH_CHECK("k_gr_init");
-    // rounds are enqueued in growing batches; the kernels of rounds past the
-    // end return at entry
+    // rounds run in captured batches of ROUND_BATCH; the kernels of rounds
+    // past the end return at entry
+    if (!g->nm_exec && (rc = capture_rounds(g, A, gp))) return rc;
     int32_t done = 0;
-    for (int batch = 4; !done; batch = std::min(batch * 2, 64)) {
-        for (int b = 0; b < batch; ++b) {
-            hipLaunchKernelGGL(k_gr_select, dim3(1), dim3(GT), 0, st, A);
-            hipLaunchKernelGGL(k_gr_fill_a, dim3(gp), dim3(PT), 0, st, A);
-            hipLaunchKernelGGL(k_gr_score, dim3(gp), dim3(PT), 0, st, A);
-            hipLaunchKernelGGL(k_gr_greedy, dim3(1), dim3(GT), 0, st, A);
-            hipLaunchKernelGGL(k_gr_fill_b, dim3(gp), dim3(PT), 0, st, A);
-            hipLaunchKernelGGL(k_gr_count, dim3(gp), dim3(PT), 0, st, A);
-        }
-        g->ctx->launches += 6 * batch;
-        VNM_LAUNCH_CHECK("N:M round kernels");
+    while (!done) {
+        VNM_HIP_TRY(hipGraphLaunch(g->nm_exec, st));
+        g->ctx->launches += 6 * ROUND_BATCH;
         VNM_HIP_TRY(hipMemcpyAsync(&done, &A.st->done, 4, hipMemcpyDeviceToHost, st));
         VNM_HIP_TRY(hipStreamSynchronize(st));
     }
