@@ -43,7 +43,8 @@ template <int AT, int BT, int VEC>
 __global__ __launch_bounds__(256) void k_spmm_batch(const SpmmArgs *__restrict__ descs,
                                                     const int32_t *__restrict__ unit_off, int count, int T,
                                                     int total, int per_xcd, int waves_per_xcd) {
-    const int wv = threadIdx.x >> 6;
+    // wave-uniform; readfirstlane lets the unit's reads be scalar loads
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
     for (int u = slot * 4 + wv; u < per_xcd; u += waves_per_xcd) {
         const int unit = xcd * per_xcd + u;
@@ -151,6 +152,8 @@ static int batch_create(vnm_ctx *ctx, vnm_packed *const *ps, int count, const vo
         if ((ldb[g] * bsize_of(bt)) % 16 || ldc[g] % 4 || ((uintptr_t)B[g] & 15) || ((uintptr_t)C[g] & 15) ||
             (bias && bias[g] && ((uintptr_t)bias[g] & 15)))
             return set_error(VNM_E_ARG, "batch: matrix %d operands must be 16-byte aligned", g);
+        if ((uint64_t)p->ncols * (uint64_t)ldb[g] * (uint64_t)bsize_of(bt) > 0xFFFFFFFFull)
+            return set_error(VNM_E_ARG, "batch: B of matrix %d must be smaller than 4 GiB", g);
     }
     if (vec && !vec_ok(bt, n, vec)) return set_error(VNM_E_ARG, "batch: %d values per lane do not fit n", vec);
     hipStream_t st = ctx->stream;

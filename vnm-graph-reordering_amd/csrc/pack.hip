@@ -301,9 +301,10 @@ __global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int T, int items, int 
         tile = lwg % T;
         iwg = lwg / T;
     }
-    const int item = iwg * 4 + wv;
-    if (item >= items) return;  // wave-uniform
-    spmm_unit<AT, BT, VEC>(a, item, tile);
+    // wave-uniform; readfirstlane lets the entry reads be scalar loads
+    const int item = __builtin_amdgcn_readfirstlane(iwg * 4 + wv);
+    if (item >= items) return;
+    spmm_unit<AT, BT, VEC>(a, item, __builtin_amdgcn_readfirstlane(tile));
 }
 
 // Pre-fill the rows of a zero list (rows that keep nothing, rows cut by an
@@ -461,7 +462,8 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
     const int vs = dtype == VNM_DT_F32 ? 4 : dtype == VNM_DT_FP8 ? 1 : 2;
     if ((rc = p->rowptr.alloc(R + 1)) || (rc = p->ent_col.alloc(std::max(E, 1))) ||
         (rc = p->ent_blk.alloc(dtype == VNM_DT_FP8 ? std::max(E, 1) : 1)) ||
-        (rc = p->ent_row.alloc(std::max(E, 1))) || (rc = p->ent_idx.alloc(std::max(E, 1))) ||
+        (rc = p->ent_row.alloc(std::max(E, 1))) ||
+        (rc = p->ent_idx.alloc(std::max(E, 1))) ||
         (rc = p->ent_val.alloc((size_t)std::max(E, 1) * 2 * vs))) {
         delete p;
         return rc;
@@ -534,6 +536,7 @@ SpmmArgs packed_args(const vnm_packed *p, const void *B, int ldb, const float *b
     a.M = p->M;
     a.ldb = ldb;
     a.ldc = ldc;
+    a.b_rows = p->ncols;
     return a;
 }
 
@@ -644,6 +647,8 @@ int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, 
     if ((ldb * bsize_of(bt)) % 16 != 0 || (ldc % 4) != 0 || ((uintptr_t)B & 15) || ((uintptr_t)C & 15))
         return set_error(VNM_E_ARG, "spmm: B and C must be 16-byte aligned with rows of whole 16-byte "
                                     "units (ldb * sizeof(B value) and 4 * ldc multiples of 16)");
+    if ((uint64_t)p->ncols * (uint64_t)ldb * (uint64_t)bsize_of(bt) > 0xFFFFFFFFull)
+        return set_error(VNM_E_ARG, "spmm: B (ncols x ldb) must be smaller than 4 GiB");
     int v = 0;
     auto it = p->shapes.find(shape_key(bt, n));
     if (it != p->shapes.end()) {

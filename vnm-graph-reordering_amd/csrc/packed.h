@@ -88,14 +88,6 @@ __host__ __device__ inline float e4m3_to_f32(uint8_t c) {
 
 // ------------------------------------------------------------ SpMM device code
 
-template <int DT>
-__device__ __forceinline__ float load_aval(const uint8_t *p, int i) {
-    if constexpr (DT == VNM_DT_F32) return ((const float *)p)[i];
-    else if constexpr (DT == VNM_DT_BF16) return bf16_to_f32(((const uint16_t *)p)[i]);
-    else if constexpr (DT == VNM_DT_F16) return __half2float(((const __half *)p)[i]);
-    else return e4m3_to_f32(p[i]);
-}
-
 template <int BT>
 constexpr int bsize() { return BT == VNM_DT_F32 ? 4 : BT == VNM_DT_BF16 ? 2 : 1; }
 
@@ -108,24 +100,6 @@ struct BRaw {
     static constexpr int NW = BYTES < 4 ? 1 : BYTES / 4;
     uint32_t w[NW];
 };
-
-template <int BT, int VEC>
-__device__ __forceinline__ BRaw<BT, VEC> load_raw(const uint8_t *p) {
-    BRaw<BT, VEC> r;
-    constexpr int BY = BRaw<BT, VEC>::BYTES;
-    if constexpr (BY == 16) {
-        const uint4 t = *(const uint4 *)p;
-        r.w[0] = t.x; r.w[1] = t.y; r.w[2] = t.z; r.w[3] = t.w;
-    } else if constexpr (BY == 8) {
-        const uint2 t = *(const uint2 *)p;
-        r.w[0] = t.x; r.w[1] = t.y;
-    } else if constexpr (BY == 4) {
-        r.w[0] = *(const uint32_t *)p;
-    } else {
-        r.w[0] = *(const uint16_t *)p;
-    }
-    return r;
-}
 
 template <int BT, int VEC>
 __device__ __forceinline__ BRaw<BT, VEC> zero_raw() {
@@ -199,44 +173,90 @@ struct SpmmArgs {
     float *C;
     const float *bias;  // nullptr or n floats added to every output row
     int E, M, ldb, ldc;
+    int b_rows;         // rows of B (columns of A); b_rows * ldb * sizeof(value) < 2^32
 };
 
-// C = A_vnm B for work item `item` and column tile `tile` (64 * VEC
-// columns), one wave: lane k resolves entry k of the item (row, both B rows,
-// scaled values), then the wave walks the entries with the entry data
-// broadcast through scalar reads, each lane gathering VEC values of the B
-// rows four entries deep and owning VEC columns of the output row.
+// VEC values of one B row for this lane, by a buffer load whose row offset
+// (soff) is a scalar and whose column offset (voff) is fixed per lane.
+template <int BT, int VEC>
+__device__ __forceinline__ BRaw<BT, VEC> load_row(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff) {
+    BRaw<BT, VEC> r;
+    constexpr int BY = BRaw<BT, VEC>::BYTES;
+    if constexpr (BY == 16) {
+        const auto t = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)voff, (int)soff, 0);
+        r.w[0] = t[0]; r.w[1] = t[1]; r.w[2] = t[2]; r.w[3] = t[3];
+    } else if constexpr (BY == 8) {
+        const auto t = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, (int)soff, 0);
+        r.w[0] = t[0]; r.w[1] = t[1];
+    } else if constexpr (BY == 4) {
+        r.w[0] = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff, 0);
+    } else {
+        r.w[0] = __builtin_amdgcn_raw_buffer_load_b16(rs, (int)voff, (int)soff, 0);
+    }
+    return r;
+}
+
+// Value i of the value array as a float (per lane).
+template <int DT>
+__device__ __forceinline__ float load_aval(const uint8_t *__restrict__ p, int i) {
+    if constexpr (DT == VNM_DT_F32) return ((const float *)p)[i];
+    else if constexpr (DT == VNM_DT_BF16) return bf16_to_f32(((const uint16_t *)p)[i]);
+    else if constexpr (DT == VNM_DT_F16) return __half2float(((const __half *)p)[i]);
+    else return e4m3_to_f32(p[i]);
+}
+
+__device__ __forceinline__ int lane_bcast(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ __forceinline__ float lane_bcast(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
+}
+
+// C = A_vnm B for one work item (ITEM row entries starting at `beg`, `cnt`
+// of them) and column tile `tile` (64 * VEC columns), one wave.  Lane k
+// first resolves entry k: its row, its scaled values and the byte offsets
+// of its B rows (column * ldb * sizeof(B value); -1 for an absent second
+// value).  The wave then walks the entries, U at a time, with each entry's
+// data broadcast by readlane: every lane gathers VEC values of the entry's
+// B rows by buffer loads whose row offset is that broadcast scalar, and owns
+// VEC columns of the output row.  A row is flushed when the row changes;
+// atomically when an item boundary cuts it (such rows are pre-filled by the
+// zero list).
 template <int AT, int BT, int VEC>
-__device__ __forceinline__ void spmm_unit(const SpmmArgs &a, int item, int tile) {
-    constexpr int SB = bsize<BT>(), U = 4;
+__device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ erow,
+                                          const int32_t *__restrict__ ecol, const uint8_t *__restrict__ eidx,
+                                          const uint8_t *__restrict__ evals, const int32_t *__restrict__ eblk,
+                                          const float *__restrict__ blk_scale, const float *__restrict__ bsc,
+                                          const uint8_t *__restrict__ B, float *__restrict__ C,
+                                          const float *__restrict__ bias, int beg, int cnt, int M, int ldb,
+                                          int ldc, int b_rows, int tile) {
+    constexpr int SB = bsize<BT>();
+    constexpr int U = BRaw<BT, VEC>::BYTES >= 16 ? 4 : 8;
     const int lane = threadIdx.x & 63;
-    const int beg = item * ITEM;
-    const int cnt = min(ITEM, a.E - beg);
-    const int32_t *__restrict__ rowptr = a.rowptr;
-    int rr = 0, c0 = 0, c1 = -1;
+    const int rowb = ldb * SB;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)B, 0, (int)((uint32_t)b_rows * (uint32_t)rowb), 0x00020000);
+    int rr = 0, off0 = 0, off1 = -1;
     float a0 = 0.f, a1 = 0.f;
     if (lane < cnt) {
         const int e = beg + lane;
-        rr = a.ent_row[e];
-        c0 = a.ent_col[e];
-        const uint32_t code = a.ent_idx[e];
+        rr = erow[e];
+        const int c0 = ecol[e];
+        const uint32_t code = eidx[e];
         float sc = 1.f;
-        if constexpr (AT == VNM_DT_FP8) sc = a.blk_scale[a.ent_blk[e]];
-        a0 = load_aval<AT>(a.ent_val, 2 * e) * sc;
+        if constexpr (AT == VNM_DT_FP8) sc = blk_scale[eblk[e]];
+        a0 = load_aval<AT>(evals, 2 * e) * sc;
+        if constexpr (BT == VNM_DT_FP8) a0 *= bsc[c0 >> 7];
+        off0 = c0 * rowb;
         if (code & 0x10u) {
-            c1 = (c0 & ~(a.M - 1)) | (int)(code & 15u);
-            a1 = load_aval<AT>(a.ent_val, 2 * e + 1) * sc;
-        }
-        if constexpr (BT == VNM_DT_FP8) {
-            a0 *= a.b_scales[c0 >> 7];
-            if (c1 >= 0) a1 *= a.b_scales[c1 >> 7];
+            const int c1 = (c0 & ~(M - 1)) | (int)(code & 15u);
+            a1 = load_aval<AT>(evals, 2 * e + 1) * sc;
+            if constexpr (BT == VNM_DT_FP8) a1 *= bsc[c1 >> 7];
+            off1 = c1 * rowb;
         }
     }
-    const int first_row = __builtin_amdgcn_readlane(rr, 0);
-    const int last_row = __builtin_amdgcn_readlane(rr, (cnt - 1) & 63);
+    const int first_row = lane_bcast(rr, 0);
+    const int last_row = lane_bcast(rr, cnt - 1);
     const int col0 = tile * (64 * VEC) + lane * VEC;
-    const uint8_t *bcol = a.B + (size_t)col0 * SB;
-    const int ldb = a.ldb;
+    const int voff = col0 * SB;
     float acc[VEC];
 #pragma unroll
     for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
@@ -244,37 +264,44 @@ __device__ __forceinline__ void spmm_unit(const SpmmArgs &a, int item, int tile)
     for (int k = 0; k < cnt; k += U) {
         BRaw<BT, VEC> raw[U][2];
         float w[U][2];
-        int rowk[U];
+        int rowk[U], o1[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int kk = k + u;  // wave-uniform
             const bool ok = kk < cnt;
-            const int ck0 = __builtin_amdgcn_readlane(c0, kk & 63);
-            const int ck1 = __builtin_amdgcn_readlane(c1, kk & 63);
-            // slots past the item's end repeat its last row, so they never
-            // look like a row change
-            rowk[u] = ok ? __builtin_amdgcn_readlane(rr, kk & 63) : last_row;
-            w[u][0] = ok ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a0), kk & 63)) : 0.f;
-            w[u][1] = (ok && ck1 >= 0) ? __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a1), kk & 63)) : 0.f;
-            raw[u][0] = ok ? load_raw<BT, VEC>(bcol + (size_t)ck0 * ldb * SB) : zero_raw<BT, VEC>();
-            raw[u][1] = (ok && ck1 >= 0) ? load_raw<BT, VEC>(bcol + (size_t)ck1 * ldb * SB) : zero_raw<BT, VEC>();
+            const int ks = kk & 63;
+            rowk[u] = ok ? lane_bcast(rr, ks) : last_row;  // past the end: no row change
+            w[u][0] = ok ? lane_bcast(a0, ks) : 0.f;
+            raw[u][0] = ok ? load_row<BT, VEC>(rs, voff, lane_bcast(off0, ks)) : zero_raw<BT, VEC>();
+            o1[u] = ok ? lane_bcast(off1, ks) : -1;
+            w[u][1] = o1[u] >= 0 ? lane_bcast(a1, ks) : 0.f;
+            raw[u][1] = o1[u] >= 0 ? load_row<BT, VEC>(rs, voff, o1[u]) : zero_raw<BT, VEC>();
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (rowk[u] != cur) {  // wave-uniform: the row changed
                 const bool cut = (cur == first_row && rowptr[cur] < beg) ||
                                  (cur == last_row && rowptr[cur + 1] > beg + cnt);
-                flush_vec<VEC>(a.C, a.ldc, cur, col0, acc, cut, a.bias);
+                flush_vec<VEC>(C, ldc, cur, col0, acc, cut, bias);
 #pragma unroll
                 for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
                 cur = rowk[u];
             }
             fma_raw<BT, VEC>(acc, w[u][0], raw[u][0]);
-            fma_raw<BT, VEC>(acc, w[u][1], raw[u][1]);
+            if (o1[u] >= 0) fma_raw<BT, VEC>(acc, w[u][1], raw[u][1]);
         }
     }
     const bool cut = (cur == first_row && rowptr[cur] < beg) || (cur == last_row && rowptr[cur + 1] > beg + cnt);
-    flush_vec<VEC>(a.C, a.ldc, cur, col0, acc, cut, a.bias);
+    flush_vec<VEC>(C, ldc, cur, col0, acc, cut, bias);
+}
+
+// Work item `item` (wave-uniform), column tile `tile` of one packed matrix.
+template <int AT, int BT, int VEC>
+__device__ __forceinline__ void spmm_unit(const SpmmArgs &a, int item, int tile) {
+    const int beg = item * ITEM;
+    spmm_item<AT, BT, VEC>(a.rowptr, a.ent_row, a.ent_col, a.ent_idx, a.ent_val, a.ent_blk,
+                           a.blk_scale, a.b_scales, a.B, a.C, a.bias, beg, min(ITEM, a.E - beg), a.M, a.ldb, a.ldc,
+                           a.b_rows, tile);
 }
 
 // ------------------------------------------------------------ host helpers (pack.hip)
