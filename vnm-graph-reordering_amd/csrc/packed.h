@@ -7,6 +7,7 @@
 #include <hip/hip_fp16.h>
 
 #include <cstring>
+#include <type_traits>
 #include <map>
 #include <memory>
 
@@ -261,14 +262,17 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
 #pragma unroll
     for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
     int cur = first_row;
-    for (int k = 0; k < cnt; k += U) {
+    // U entries: broadcast, gather, then fold in (flushing on a row change);
+    // CHECK only for the item's last, partial group
+    auto group = [&](int k, auto check) {
+        constexpr bool CHECK = decltype(check)::value;
         BRaw<BT, VEC> raw[U][2];
         float w[U][2];
         int rowk[U], o1[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int kk = k + u;  // wave-uniform
-            const bool ok = kk < cnt;
+            const bool ok = !CHECK || kk < cnt;
             const int ks = kk & 63;
             rowk[u] = ok ? lane_bcast(rr, ks) : last_row;  // past the end: no row change
             w[u][0] = ok ? lane_bcast(a0, ks) : 0.f;
@@ -290,7 +294,10 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
             fma_raw<BT, VEC>(acc, w[u][0], raw[u][0]);
             if (o1[u] >= 0) fma_raw<BT, VEC>(acc, w[u][1], raw[u][1]);
         }
-    }
+    };
+    const int full = cnt & ~(U - 1);
+    for (int k = 0; k < full; k += U) group(k, std::false_type{});
+    if (full < cnt) group(full, std::true_type{});
     const bool cut = (cur == first_row && rowptr[cur] < beg) || (cur == last_row && rowptr[cur + 1] > beg + cnt);
     flush_vec<VEC>(C, ldc, cur, col0, acc, cut, bias);
 }
