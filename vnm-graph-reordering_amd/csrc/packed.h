@@ -256,6 +256,9 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
     }
     const int first_row = lane_bcast(rr, 0);
     const int last_row = lane_bcast(rr, cnt - 1);
+    // entries that start a new row (entry 0 does not: cur starts there)
+    const int prev_rr = __shfl_up(rr, 1, 64);
+    const uint64_t starts = __ballot(lane > 0 && lane < cnt && rr != prev_rr);
     const int col0 = tile * (64 * VEC) + lane * VEC;
     const int voff = col0 * SB;
     float acc[VEC];
@@ -268,13 +271,12 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
         constexpr bool CHECK = decltype(check)::value;
         BRaw<BT, VEC> raw[U][2];
         float w[U][2];
-        int rowk[U], o1[U];
+        int o1[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int kk = k + u;  // wave-uniform
             const bool ok = !CHECK || kk < cnt;
             const int ks = kk & 63;
-            rowk[u] = ok ? lane_bcast(rr, ks) : last_row;  // past the end: no row change
             w[u][0] = ok ? lane_bcast(a0, ks) : 0.f;
             raw[u][0] = ok ? load_row<BT, VEC>(rs, voff, lane_bcast(off0, ks)) : zero_raw<BT, VEC>();
             o1[u] = ok ? lane_bcast(off1, ks) : -1;
@@ -283,13 +285,14 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (rowk[u] != cur) {  // wave-uniform: the row changed
+            const int kk = k + u;
+            if ((starts >> (kk & 63)) & 1ull) {  // wave-uniform: a new row starts here
                 const bool cut = (cur == first_row && rowptr[cur] < beg) ||
                                  (cur == last_row && rowptr[cur + 1] > beg + cnt);
                 flush_vec<VEC>(C, ldc, cur, col0, acc, cut, bias);
 #pragma unroll
                 for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
-                cur = rowk[u];
+                cur = lane_bcast(rr, kk & 63);
             }
             fma_raw<BT, VEC>(acc, w[u][0], raw[u][0]);
             if (o1[u] >= 0) fma_raw<BT, VEC>(acc, w[u][1], raw[u][1]);
