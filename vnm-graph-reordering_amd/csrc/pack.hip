@@ -131,13 +131,28 @@ __global__ void k_blk_select(const uint64_t *__restrict__ bk, const int32_t *__r
     const int first = blk_first[b];
     int last = U;
     if (b + 1 < nb) last = blk_first[b + 1];
-    // score of subsets lane and lane + 64
+    // scores of 128 subsets at a time (lane and lane + 64; one round for
+    // M <= 8): the block's run bitmaps are loaded 64 at a time, one per
+    // lane, and broadcast
     int best_score = -1, best_idx = 0x7FFFFFFF;
-    for (int s = lane; s < nsub; s += 64) {
-        const uint32_t S = subs[s];
-        int sc = 0;
-        for (int q = first; q < last; ++q) sc += min(2, popc32(bits[rid_sorted[q]] & S));
-        if (sc > best_score) { best_score = sc; best_idx = s; }
+    for (int s0 = 0; s0 < nsub; s0 += 128) {
+        const int i0 = s0 + lane, i1 = s0 + 64 + lane;
+        const uint32_t S0 = i0 < nsub ? subs[i0] : 0u;
+        const uint32_t S1 = i1 < nsub ? subs[i1] : 0u;
+        int sc0 = 0, sc1 = 0;
+        for (int base = first; base < last; base += 64) {
+            const int q = base + lane;
+            const int mine = q < last ? (int)bits[rid_sorted[q]] : 0;
+            const int n = min(64, last - base);
+            for (int j = 0; j < n; ++j) {
+                const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane(mine, j);
+                sc0 += min(2, popc32(bj & S0));
+                sc1 += min(2, popc32(bj & S1));
+            }
+        }
+        // ascending subset index per lane: strict > keeps the smaller on ties
+        if (i0 < nsub && sc0 > best_score) { best_score = sc0; best_idx = i0; }
+        if (i1 < nsub && sc1 > best_score) { best_score = sc1; best_idx = i1; }
     }
     // wave argmax (score desc, index asc)
     for (int d = 32; d >= 1; d >>= 1) {
