@@ -122,13 +122,20 @@ __global__ void k_lp_adj_keys(const int32_t *__restrict__ row, const int32_t *__
     if (local) atomicAdd(nvalid, local);
 }
 
-__global__ void k_lp_adj_unpack(const uint64_t *__restrict__ keys, int64_t n, int32_t *__restrict__ idx,
-                                int32_t *__restrict__ cnt) {
+// Sorted keys (node << 32 | neighbour), n of them valid: idx[i] = neighbour,
+// ptr[u] = the first i whose node is >= u (ptr[R] = n), written at the
+// boundaries where the node changes (no per-entry counting).
+__global__ void k_lp_adj_unpack(const uint64_t *__restrict__ keys, int64_t n, int R,
+                                int32_t *__restrict__ idx, int32_t *__restrict__ ptr) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t k = keys[i];
         idx[i] = (int32_t)(uint32_t)k;
-        atomicAdd(&cnt[k >> 32], 1);
+        const int u = (int)(k >> 32);
+        const int prev = i == 0 ? -1 : (int)(keys[i - 1] >> 32);
+        for (int v = prev + 1; v <= u; ++v) ptr[v] = (int32_t)i;
+        if (i == n - 1)
+            for (int v = u + 1; v <= R; ++v) ptr[v] = (int32_t)n;
     }
 }
 
@@ -945,14 +952,11 @@ static int build_lists(vnm_graph *g, uint64_t *keys, uint64_t *tmp, int64_t n, i
     while ((1ll << bits) <= R) ++bits;
     int rc = sort_keys_u64(keys, tmp, n, 32 + bits, st);
     if (rc) return rc;
-    DBuf<int32_t> cnt;
-    if ((rc = ptr.alloc(R + 1)) || (rc = idx.alloc(std::max<int64_t>(nvalid, 1))) ||
-        (rc = cnt.alloc(R + 1)))
-        return rc;
-    VNM_HIP_TRY(hipMemsetAsync(cnt.p, 0, (R + 1) * 4, st));
-    hipLaunchKernelGGL(k_lp_adj_unpack, dim3(grid_of(nvalid, 256)), dim3(256), 0, st, tmp, nvalid, idx.p,
-                       cnt.p);
-    if ((rc = scan_excl_i32(cnt.p, ptr.p, R + 1, st))) return rc;
+    if ((rc = ptr.alloc(R + 1)) || (rc = idx.alloc(std::max<int64_t>(nvalid, 1)))) return rc;
+    VNM_HIP_TRY(hipMemsetAsync(ptr.p, 0, (R + 1) * 4, st));  // stays all zero when nothing is valid
+    if (nvalid > 0)
+        hipLaunchKernelGGL(k_lp_adj_unpack, dim3(grid_of(nvalid, 256)), dim3(256), 0, st, tmp, nvalid, R,
+                           idx.p, ptr.p);
     g->ctx->launches += 1;
     VNM_LAUNCH_CHECK("k_lp_adj_unpack");
     hptr.resize(R + 1);
