@@ -904,6 +904,7 @@ struct LPCache {
     int R = 0, nseg = 0, M = 0;
     int64_t nadj = 0;
     DBuf<int32_t> ptr, idx, tptr, tidx;
+    bool has_bwd = false;  // tptr/tidx and the backward plan are built
     DBuf<float> inv_deg;
     AggPlan fwd, bwd;
     DBuf<int32_t> ccls, segcls, segnnz, tot;
@@ -999,12 +1000,11 @@ static int lp_prepare(vnm_graph *g, LPCache **out) {
     VNM_HIP_TRY(hipMemcpyAsync(&nvalid, nv.p, 4, hipMemcpyDeviceToHost, st));
     VNM_HIP_TRY(hipStreamSynchronize(st));
     c->nadj = nvalid;
-    std::vector<int32_t> hp, htp;
+    std::vector<int32_t> hp;
     if ((rc = build_lists(g, kc.p, tmp.p, nnz, nvalid, c->ptr, c->idx, hp))) return rc;
-    if ((rc = build_lists(g, kt.p, tmp.p, nnz, nvalid, c->tptr, c->tidx, htp))) return rc;
     if ((rc = c->inv_deg.alloc(R))) return rc;
     hipLaunchKernelGGL(k_lp_inv_deg, dim3(grid_of(R, 256)), dim3(256), 0, st, R, c->ptr.p, c->inv_deg.p);
-    if ((rc = agg_plan(hp, c->fwd)) || (rc = agg_plan(htp, c->bwd))) return rc;
+    if ((rc = agg_plan(hp, c->fwd))) return rc;
     const int ns = c->nseg;
     if ((rc = c->ccls.alloc((size_t)R * 4)) || (rc = c->segcls.alloc((size_t)ns * 4)) ||
         (rc = c->segnnz.alloc(ns)) || (rc = c->tot.alloc(4)))
@@ -1035,10 +1035,39 @@ static int lp_stats(vnm_graph *g, LPCache *c, int N, const int32_t *segs, int ns
     return VNM_OK;
 }
 
+// The transposed adjacency and its aggregation plan, which only backward
+// passes read: built on first use (once per labelling, like the rest).
+static int lp_prepare_bwd(vnm_graph *g, LPCache *c) {
+    if (c->has_bwd) return VNM_OK;
+    hipStream_t st = g->ctx->stream;
+    const int64_t nnz = g->nnz;
+    DBuf<uint64_t> kc, kt, tmp;
+    DBuf<int32_t> nv;
+    int rc;
+    if ((rc = kc.alloc(std::max<int64_t>(nnz, 1))) || (rc = kt.alloc(std::max<int64_t>(nnz, 1))) ||
+        (rc = tmp.alloc(std::max<int64_t>(nnz, 1))) || (rc = nv.alloc(1)))
+        return rc;
+    VNM_HIP_TRY(hipMemsetAsync(nv.p, 0, 4, st));
+    if (nnz)
+        hipLaunchKernelGGL(k_lp_adj_keys, dim3(grid_of(nnz, 256, 4096)), dim3(256), 0, st, g->row.p,
+                           g->col.p, g->val.p, nnz, c->R, kc.p, kt.p, nv.p);
+    g->ctx->launches++;
+    VNM_LAUNCH_CHECK("k_lp_adj_keys");
+    std::vector<int32_t> htp;
+    if ((rc = build_lists(g, kt.p, tmp.p, nnz, c->nadj, c->tptr, c->tidx, htp))) return rc;
+    if ((rc = agg_plan(htp, c->bwd))) return rc;
+    c->has_bwd = true;
+    return VNM_OK;
+}
+
 template <typename TI, typename TO, int F, bool BWD>
 static int lp_aggregate(vnm_graph *g, LPCache *c, const TI *H, TO *out) {
     hipStream_t st = g->ctx->stream;
     const int R = c->R;
+    if (BWD) {
+        const int rc = lp_prepare_bwd(g, c);
+        if (rc) return rc;
+    }
     const AggPlan &P = BWD ? c->bwd : c->fwd;
     const int32_t *ptr = BWD ? c->tptr.p : c->ptr.p;
     const int32_t *idx = BWD ? c->tidx.p : c->idx.p;
