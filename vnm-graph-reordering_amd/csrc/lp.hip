@@ -271,7 +271,27 @@ __device__ __forceinline__ float4 agg_range(int b, int e, const int32_t *__restr
     const int lane = threadIdx.x & 63;
     const int gq = lane % LPR, slot = lane / LPR;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int j = b + slot; j < e; j += NS) {
+    // four rows per slot in flight; each slot still sums its rows in list
+    // order, so the result does not depend on the unrolling
+    constexpr int UN = 4;
+    int j = b + slot;
+    for (; j + (UN - 1) * NS < e; j += UN * NS) {
+        int v[UN];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) v[u] = idx[j + u * NS];
+        float4 h[UN];
+#pragma unroll
+        for (int u = 0; u < UN; ++u) h[u] = Act<TI>::load4(H + (size_t)v[u] * F + gq * 4);
+#pragma unroll
+        for (int u = 0; u < UN; ++u) {
+            if (BWD) {
+                const float f = inv_deg[v[u]];
+                h[u].x *= f; h[u].y *= f; h[u].z *= f; h[u].w *= f;
+            }
+            acc.x += h[u].x; acc.y += h[u].y; acc.z += h[u].z; acc.w += h[u].w;
+        }
+    }
+    for (; j < e; j += NS) {
         const int v = idx[j];
         float4 h = Act<TI>::load4(H + (size_t)v * F + gq * 4);
         if (BWD) {
