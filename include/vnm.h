@@ -152,6 +152,16 @@ int vnm_nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *st
  * touching a swapped node and applies the move that was scored. */
 int vnm_nm_reorder_allpairs(vnm_graph *g, int N, int sched, int maxiter, int clean, vnm_nm_stats *st);
 
+/* vnm_nm_reorder on count graphs of one context at once: one chain of round
+ * kernels runs every graph's greedy (grid y = graph), so the batch takes
+ * about as long as its longest run rather than the sum.  sched[i] is graph
+ * i's schedule (0..3 or >= 8), st[i] its stats; each result is that of
+ * vnm_nm_reorder on the graph alone.  No counterpart in the reference, which
+ * reorders one matrix per process (main.cu:339-352); find_best_2toX's four
+ * runs (main.cu:72-114) and batches of small graphs are its callers. */
+int vnm_nm_reorder_batch(vnm_graph *const *graphs, int count, int N, const int32_t *sched, int maxiter,
+                         vnm_nm_stats *st);
+
 /* vnm_nm_reorder with the reference's swap decisions recorded (what the
  * verbose swap print of run_onepair shows, reorder/reorder.h:1913-1918, as
  * data): 9 ints
@@ -189,7 +199,8 @@ typedef struct {
 int vnm_hybrid(vnm_graph *g, int N, int V, int rounds, int sched, vnm_hybrid_stats *st);
 
 /* Replaces find_best_2toX (main.cu:72-114) over schedules 0..nsched-1, each on
- * a fresh copy of the graph; improve-rate ties go to the lowest index (the
+ * a fresh copy of the graph, the copies reordered together as one
+ * vnm_nm_reorder_batch; improve-rate ties go to the lowest index (the
  * reference compares wall times). */
 int vnm_find_best_sched(vnm_graph *g, int N, int nsched, int32_t *best);
 

@@ -129,6 +129,67 @@ def test_greedy_rmat_medium(ctx):
     _nm_case(ctx, vnm.gen_rmat(11, 16384, 12, M=4), 4, 2, 0)
 
 
+def _batch_cases(rng):
+    cases = []
+    for t in range(10):
+        M = (4, 8)[t % 2]
+        R = int(rng.integers(2, 40)) * M
+        cases.append((_rand(rng, R, rng.uniform(0.03, 0.5), M, sym=t % 3 == 0), M))
+    for f in ("GD01_c", "lund_b", "temp"):  # temp.mtx is already valid: init_size 0
+        cases.append((vnm.mtx_read(os.path.join(GOLD, "mtx", f + ".mtx"), 8), 8))
+    cases.append((vnm.coo_from_tuples(16, 16, [], [], [], 8), 8))  # no entries
+    cases.append((vnm.gen_er(2048, 41943, 1), 8))
+    cases.append((vnm.gen_rmat(12, 32768, 11), 8))
+    return cases
+
+
+def test_greedy_batch_matches_oracle(ctx):
+    # uneven graphs, mixed M and schedules in one launch chain; each result
+    # must be the oracle's for the graph alone, and a second batch and a
+    # single run on the same graphs continue from their reordered state
+    rng = np.random.default_rng(40)
+    cases = _batch_cases(rng)
+    scheds = [(0, 1, 2, 3, 8)[i % 5] for i in range(len(cases))]
+    graphs = [vnm.Graph(ctx, m, M) for m, M in cases]
+    oms = [_omat(m).copy() for m, _ in cases]
+    for rep in range(2):
+        got = vnm.Graph.nm_reorder_batch(graphs, 2, scheds)
+        for i, ((m, M), g, om) in enumerate(zip(cases, graphs, oms)):
+            ost, _ = oracle.nm_reorder(om, 2, M, scheds[i], 10)
+            assert got[i] == ost.tolist(), (rep, i, scheds[i], got[i], ost.tolist())
+            out = g.download()
+            assert np.array_equal(out.row, om.row) and np.array_equal(out.col, om.col), (rep, i)
+    m, M = cases[-1]
+    st = graphs[-1].nm_reorder(2, 0)
+    ost, _ = oracle.nm_reorder(oms[-1], 2, M, 0, 10)
+    assert st == ost.tolist()
+    out = graphs[-1].download()
+    assert np.array_equal(out.row, oms[-1].row) and np.array_equal(out.col, oms[-1].col)
+
+
+def test_greedy_batch_rejects_bad_batches(ctx):
+    m = vnm.gen_er(256, 2000, 3)
+    a, b = vnm.Graph(ctx, m, 8), vnm.Graph(ctx, m, 8)
+    with pytest.raises(vnm.VnmError):
+        vnm.Graph.nm_reorder_batch([a, a], 2, 0)  # the same graph twice
+    with pytest.raises(vnm.VnmError):
+        vnm.Graph.nm_reorder_batch([a, b], 2, [0, 5])  # allpairs schedules are not batched
+    with pytest.raises(vnm.VnmError):
+        vnm.Graph.nm_reorder_batch([a, b], 8, 0)  # N >= M
+    other = vnm.Context(0)
+    try:
+        c = vnm.Graph(other, m, 8)
+        with pytest.raises(vnm.VnmError):
+            vnm.Graph.nm_reorder_batch([a, c], 2, 0)  # graphs of two contexts
+        c.close()
+    finally:
+        other.close()
+    # the rejected calls left the graphs usable
+    got = vnm.Graph.nm_reorder_batch([a, b], 2, [0, 3])
+    want = [oracle.nm_reorder(_omat(m).copy(), 2, 8, s, 10)[0].tolist() for s in (0, 3)]
+    assert got == want
+
+
 def test_kat_temp_mtx(ctx):
     m = vnm.mtx_read(os.path.join(GOLD, "mtx", "temp.mtx"), 8)
     g = vnm.Graph(ctx, m, 8)

@@ -101,6 +101,8 @@ int build_csc(vnm_graph *g);
 int locate_all(vnm_graph *g, int N, int32_t *d_cnt, int32_t *d_nzr);
 int relabel(vnm_graph *g, const int32_t *d_perm);
 int nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *st);
+int nm_reorder_batch(vnm_graph *const *gs, int count, int N, const int *scheds, int maxiter,
+                     vnm_nm_stats *stats);
 int nm_reorder_allpairs(vnm_graph *g, int N, int sched, int maxiter, int clean, vnm_nm_stats *stats);
 int nm_reorder_ex(vnm_graph *g, int N, int sched, int maxiter, int round_cap, int32_t *d_trace,
                   int64_t trace_cap, int64_t *trace_len, vnm_nm_stats *stats);

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cstring>
 #include <functional>
+#include <vector>
 
 namespace vnm {
 
@@ -89,21 +90,31 @@ int hybrid_learned(vnm_graph *g, const vnm_lp_model *m, int N, int V, int rounds
 }
 
 int find_best_sched(vnm_graph *g, int N, int nsched, int32_t *best) {
+    *best = 0;
+    if (g->nnz == 0) return VNM_OK;  // main.cu:85
+    if (g->M != 4 && g->M != 8)
+        return set_error(VNM_E_ARG, "the GPU N:M engine supports M = 4 or 8 (got %d)", g->M);
+    // the schedules run together on fresh copies
+    std::vector<vnm_graph *> c(nsched, nullptr);
+    struct Free {
+        std::vector<vnm_graph *> &c;
+        ~Free() {
+            for (vnm_graph *x : c) vnm_graph_destroy(x);
+        }
+    } free_copies{c};
+    int rc;
+    for (int i = 0; i < nsched; ++i)
+        if ((rc = graph_clone(g, &c[i]))) return rc;
+    std::vector<int> sched(nsched);
+    std::vector<vnm_nm_stats> s(nsched);
+    for (int i = 0; i < nsched; ++i) sched[i] = i;
+    if ((rc = nm_reorder_batch(c.data(), nsched, N, sched.data(), 10, s.data()))) return rc;
+    if (s[0].init_size == 0) return VNM_OK;  // main.cu:85
     int best_id = 0;
     float max_imprv = 0.0f;
     for (int i = 0; i < nsched; ++i) {
-        vnm_graph *c = nullptr;
-        int rc = graph_clone(g, &c);
-        if (rc) return rc;
-        vnm_nm_stats s;
-        rc = nm_reorder(c, N, i, 10, &s);
-        vnm_graph_destroy(c);
-        if (rc) return rc;
-        if (g->nnz == 0 || s.init_size == 0) {  // main.cu:85
-            *best = 0;
-            return VNM_OK;
-        }
-        const float imp = s.init_cnt != 0 ? (float)(s.init_cnt - s.final_cnt) * 1.0f / (float)s.init_cnt : 0.0f;
+        const float imp =
+            s[i].init_cnt != 0 ? (float)(s[i].init_cnt - s[i].final_cnt) * 1.0f / (float)s[i].init_cnt : 0.0f;
         if (imp > max_imprv) {  // ties keep the lower index (the reference compares wall time)
             best_id = i;
             max_imprv = imp;

@@ -343,7 +343,9 @@ struct GreedyState {
     uint64_t clk[16];
 };
 
-// Kernel arguments of every round kernel: pointers fixed for the graph.
+// Arguments of the round kernels for one graph: pointers fixed for the
+// graph.  The kernels take an array of them in device memory and serve graph
+// blockIdx.y, so one launch chain runs the greedy of many graphs at once.
 struct NMArgs {
     GreedyState *st;
     const int32_t *cptr, *crow;
@@ -573,7 +575,8 @@ __device__ __forceinline__ void clock_add(GreedyState *st, int i, uint64_t t0) {
 
 // Records B of the last round's recount are cleared, records A of this
 // round's segments filled.
-__global__ __launch_bounds__(PT) void k_gr_fill_a(NMArgs A) {
+__global__ __launch_bounds__(PT) void k_gr_fill_a(const NMArgs *__restrict__ args) {
+    const NMArgs A = args[blockIdx.y];
     __shared__ ColSet sa, sb;
     __shared__ int done;
     GreedyState *st = A.st;
@@ -592,7 +595,8 @@ __global__ __launch_bounds__(PT) void k_gr_fill_a(NMArgs A) {
 
 // Records A of this round are cleared, records B filled with the columns
 // after the round's swaps.
-__global__ __launch_bounds__(PT) void k_gr_fill_b(NMArgs A) {
+__global__ __launch_bounds__(PT) void k_gr_fill_b(const NMArgs *__restrict__ args) {
+    const NMArgs A = args[blockIdx.y];
     __shared__ ColSet sa, sb;
     __shared__ int done;
     GreedyState *st = A.st;
@@ -611,7 +615,8 @@ __global__ __launch_bounds__(PT) void k_gr_fill_b(NMArgs A) {
 
 // After a run: both record sets back to zero (zeroing a clean record is
 // harmless, so this needs no knowledge of where the run stopped).
-__global__ __launch_bounds__(PT) void k_gr_cleanup(NMArgs A) {
+__global__ __launch_bounds__(PT) void k_gr_cleanup(const NMArgs *__restrict__ args) {
+    const NMArgs A = args[blockIdx.y];
     __shared__ ColSet sa, sb;
     GreedyState *st = A.st;
     copy_words(&sa, &st->S, sizeof(ColSet));
@@ -628,7 +633,8 @@ __global__ __launch_bounds__(PT) void k_gr_cleanup(NMArgs A) {
 // the row's leading bit, one per row) are folded in with ballots, so each
 // (b1, b2) counter takes one LDS atomic per chunk; each workgroup adds its
 // nonzero sums to the device accumulators.
-__global__ __launch_bounds__(PT) void k_gr_score(NMArgs A) {
+__global__ __launch_bounds__(PT) void k_gr_score(const NMArgs *__restrict__ args) {
+    const NMArgs A = args[blockIdx.y];
     __shared__ ColSet sa;
     __shared__ int E[KMAX][MMAX];
     __shared__ int both[NPAIR_T][2][MMAX * MMAX];
@@ -713,7 +719,8 @@ __global__ __launch_bounds__(PT) void k_gr_score(NMArgs A) {
 
 // cnt / nzr of the round's segments after the swaps, from records B
 // (locate_invalid_segment on the touched segments).
-__global__ __launch_bounds__(PT) void k_gr_count(NMArgs A) {
+__global__ __launch_bounds__(PT) void k_gr_count(const NMArgs *__restrict__ args) {
+    const NMArgs A = args[blockIdx.y];
     __shared__ ColSet sb;
     __shared__ int cnt[KMAX], nzr[KMAX];
     __shared__ int done;
@@ -888,7 +895,8 @@ __device__ __forceinline__ int greedy_apply(GreedyShared &S, int M, bool rev, bo
 // the run goes on (reorder/reorder.h:1819-1826: rounds <= max(maxiter, list
 // size), a list of at least one (two for schedules >= 8) segments), choose
 // the round's segments and their columns.
-__global__ __launch_bounds__(GT) void k_gr_select(NMArgs A) {
+__global__ __launch_bounds__(GT) void k_gr_select(const NMArgs *__restrict__ args) {
+    const NMArgs A = args[blockIdx.y];
     __shared__ GreedyShared S;
     GreedyState *st = A.st;
     const int tid = threadIdx.x;
@@ -1031,7 +1039,8 @@ __global__ __launch_bounds__(GT) void k_gr_select(NMArgs A) {
 // The round's sequential part: priority queues of every ordered pair of the
 // round's segments, the greedy (reorder/reorder.h:1837-1932), the commit to
 // perm/inv, and the columns of the segments after the commit.
-__global__ __launch_bounds__(GT) void k_gr_greedy(NMArgs A) {
+__global__ __launch_bounds__(GT) void k_gr_greedy(const NMArgs *__restrict__ args) {
+    const NMArgs A = args[blockIdx.y];
     __shared__ GreedyShared S;
     GreedyState *st = A.st;
     const int tid = threadIdx.x;
@@ -1315,19 +1324,31 @@ static int check_nm_args(vnm_graph *g, int N) {
 }
 
 constexpr int ROUND_BATCH = 16;  // greedy rounds per captured graph launch
+constexpr int MAX_BATCH = 4096;  // graphs per batched run
 
-// Capture ROUND_BATCH rounds (six kernels each) on the context stream.
-static int capture_rounds(vnm_graph *g, const NMArgs &A, int gp) {
-    hipStream_t st = g->ctx->stream;
+// 1 in *flag when every graph of the batch has finished.
+__global__ __launch_bounds__(64) void k_gr_alldone(const NMArgs *__restrict__ args, int G,
+                                                   int32_t *__restrict__ flag) {
+    int open = 0;
+    for (int i = threadIdx.x; i < G; i += 64) open |= args[i].st->done == 0;
+    const uint64_t b = __ballot(open);
+    if (threadIdx.x == 0) *flag = b == 0;
+}
+
+// Capture ROUND_BATCH rounds (six kernels each, grid y = the G graphs) and
+// the all-done test on the context stream.
+static int capture_rounds(hipStream_t st, const NMArgs *d_args, int G, int gpx, int32_t *d_flag,
+                          hipGraph_t *graph_out, hipGraphExec_t *exec_out) {
     VNM_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int b = 0; b < ROUND_BATCH; ++b) {
-        hipLaunchKernelGGL(k_gr_select, dim3(1), dim3(GT), 0, st, A);
-        hipLaunchKernelGGL(k_gr_fill_a, dim3(gp), dim3(PT), 0, st, A);
-        hipLaunchKernelGGL(k_gr_score, dim3(gp), dim3(PT), 0, st, A);
-        hipLaunchKernelGGL(k_gr_greedy, dim3(1), dim3(GT), 0, st, A);
-        hipLaunchKernelGGL(k_gr_fill_b, dim3(gp), dim3(PT), 0, st, A);
-        hipLaunchKernelGGL(k_gr_count, dim3(gp), dim3(PT), 0, st, A);
+        hipLaunchKernelGGL(k_gr_select, dim3(1, G), dim3(GT), 0, st, d_args);
+        hipLaunchKernelGGL(k_gr_fill_a, dim3(gpx, G), dim3(PT), 0, st, d_args);
+        hipLaunchKernelGGL(k_gr_score, dim3(gpx, G), dim3(PT), 0, st, d_args);
+        hipLaunchKernelGGL(k_gr_greedy, dim3(1, G), dim3(GT), 0, st, d_args);
+        hipLaunchKernelGGL(k_gr_fill_b, dim3(gpx, G), dim3(PT), 0, st, d_args);
+        hipLaunchKernelGGL(k_gr_count, dim3(gpx, G), dim3(PT), 0, st, d_args);
     }
+    hipLaunchKernelGGL(k_gr_alldone, dim3(1), dim3(64), 0, st, d_args, G, d_flag);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (e != hipSuccess) {
@@ -1340,17 +1361,151 @@ static int capture_rounds(vnm_graph *g, const NMArgs &A, int gp) {
         (void)hipGraphDestroy(graph);
         return set_error(VNM_E_HIP, "instantiating the N:M round graph failed: %s", hipGetErrorString(e));
     }
-    g->nm_graph = graph;
-    g->nm_exec = exec;
+    *graph_out = graph;
+    *exec_out = exec;
     return VNM_OK;
 }
 
-// Grid of the record pass kernels: two workgroups per CU.
-static int pass_grid(vnm_graph *g) {
+// Workgroups per graph of the record pass kernels: two per CU for one graph,
+// fewer per graph in a batch (the grid-stride loops cover any count).
+static int pass_grid(vnm_ctx *ctx, int G) {
     int ncu = 0;
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g->ctx->device) != hipSuccess || ncu < 1)
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || ncu < 1)
         ncu = 1;
-    return 2 * ncu;
+    return std::max(16, 2 * ncu / G);
+}
+
+// Bytes of the greedy state of a graph, then its one-graph argument block
+// and all-done flag.
+constexpr size_t GST_STATE = (sizeof(GreedyState) + 255) / 256 * 256;
+constexpr size_t GST_BYTES = GST_STATE + (sizeof(NMArgs) + 63) / 64 * 64 + 64;
+
+// Per-graph state of a run: CSC, counts, records, greedy state and the
+// kernel arguments.
+static int greedy_prepare(vnm_graph *g, int N, NMArgs *A) {
+    int rc;
+    if ((rc = build_csc(g))) return rc;
+    if ((rc = locate_all(g, N, g->cnt.p, g->nzr.p))) return rc;
+    hipStream_t st = g->ctx->stream;
+    if (!g->gst.p && (rc = g->gst.alloc(GST_BYTES / 4))) return rc;
+    if (!g->rt.p) {
+        // records A and B, 16 bytes per row each, zero between runs
+        if ((rc = g->rt.alloc((size_t)g->nrows * 8))) return rc;
+        VNM_HIP_TRY(hipMemsetAsync(g->rt.p, 0, (size_t)g->nrows * 32, st));
+    }
+    A->st = (GreedyState *)g->gst.p;
+    A->cptr = g->cptr.p; A->crow = g->crow.p; A->perm = g->perm.p; A->inv = g->inv.p;
+    A->cnt = g->cnt.p; A->nzr = g->nzr.p;
+    A->rta = g->rt.p; A->rtb = g->rt.p + (size_t)g->nrows * 4;
+    return VNM_OK;
+}
+
+static void debug_print(int i, const GreedyState &h) {
+    fprintf(stderr, "[vnm] greedy%s%d rounds=%d entries=%lld kernel_us select=%llu fill_a=%llu score=%llu "
+                    "greedy=%llu (heaps %llu, sequential %llu) fill_b=%llu count=%llu\n",
+            i < 0 ? "" : " graph ", i < 0 ? 0 : i,
+            h.round - 1, (long long)h.work, (unsigned long long)(h.clk[0] / 100),
+            (unsigned long long)(h.clk[1] / 100), (unsigned long long)(h.clk[2] / 100),
+            (unsigned long long)(h.clk[3] / 100), (unsigned long long)(h.clk[6] / 100),
+            (unsigned long long)(h.clk[7] / 100), (unsigned long long)(h.clk[4] / 100),
+            (unsigned long long)(h.clk[5] / 100));
+    fprintf(stderr, "[vnm]   select: commit %llu, segments %llu, columns %llu; heaps: ranks %llu\n",
+            (unsigned long long)(h.clk[8] / 100), (unsigned long long)(h.clk[9] / 100),
+            (unsigned long long)(h.clk[10] / 100), (unsigned long long)(h.clk[11] / 100));
+}
+
+// The greedy of G graphs (schedules 0-3 or >= 8 each) as one chain of round
+// kernels on the context stream.  Graphs that finish early return at entry
+// of the later rounds' kernels.  The trace is recorded only for G == 1.
+static int greedy_run(vnm_graph *const *gs, const int *scheds, int G, int N, int maxiter, int round_cap,
+                      int32_t *d_trace, int64_t trace_cap, int64_t *trace_len, vnm_nm_stats *stats) {
+    vnm_ctx *ctx = gs[0]->ctx;
+    hipStream_t st = ctx->stream;
+    int rc;
+    std::vector<NMArgs> hargs(G);
+    for (int i = 0; i < G; ++i)
+        if ((rc = greedy_prepare(gs[i], N, &hargs[i]))) return rc;
+    // the argument array: inside the graph's state block for one graph (its
+    // captured rounds are kept with the graph), a buffer of the call for many
+    DBuf<int32_t> batch_buf;
+    NMArgs *d_args;
+    int32_t *d_flag;
+    if (G == 1) {
+        d_args = (NMArgs *)((char *)gs[0]->gst.p + GST_STATE);
+        d_flag = (int32_t *)((char *)gs[0]->gst.p + GST_BYTES - 64);
+    } else {
+        const size_t abytes = ((size_t)G * sizeof(NMArgs) + 63) / 64 * 64;
+        if ((rc = batch_buf.alloc((abytes + 64) / 4))) return rc;
+        d_args = (NMArgs *)batch_buf.p;
+        d_flag = (int32_t *)((char *)batch_buf.p + abytes);
+    }
+    VNM_HIP_TRY(hipMemcpyAsync(d_args, hargs.data(), (size_t)G * sizeof(NMArgs), hipMemcpyHostToDevice, st));
+    const int gpx = pass_grid(ctx, G);
+    KernelTimer kt(ctx, "nm_greedy");
+    for (int i = 0; i < G; ++i) {
+        vnm_graph *g = gs[i];
+        hipLaunchKernelGGL(k_gr_init, dim3(1), dim3(GT), 0, st, hargs[i], g->nrows, g->M, N, g->nseg(),
+                           scheds[i], maxiter, round_cap > 0 ? round_cap : 0x7FFFFFFF, d_trace,
+                           d_trace ? trace_cap : (int64_t)0);
+    }
+    ctx->launches += G;
+    VNM_LAUNCH_CHECK("k_gr_init");
+    // rounds run in captured batches of ROUND_BATCH; the kernels of rounds
+    // past a graph's end return at entry
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (G == 1) {
+        if (!gs[0]->nm_exec &&
+            (rc = capture_rounds(st, d_args, 1, gpx, d_flag, &gs[0]->nm_graph, &gs[0]->nm_exec)))
+            return rc;
+        exec = gs[0]->nm_exec;
+    } else if ((rc = capture_rounds(st, d_args, G, gpx, d_flag, &graph, &exec))) {
+        return rc;
+    }
+    struct Drop {
+        hipGraph_t g;
+        hipGraphExec_t e;
+        ~Drop() {
+            if (e) (void)hipGraphExecDestroy(e);
+            if (g) (void)hipGraphDestroy(g);
+        }
+    } drop{graph, G == 1 ? nullptr : exec};
+    int32_t done = 0;
+    while (!done) {
+        VNM_HIP_TRY(hipGraphLaunch(exec, st));
+        ctx->launches += 6 * ROUND_BATCH + 1;
+        VNM_HIP_TRY(hipMemcpyAsync(&done, d_flag, 4, hipMemcpyDeviceToHost, st));
+        VNM_HIP_TRY(hipStreamSynchronize(st));
+    }
+    for (int i = 0; i < G; ++i)
+        hipLaunchKernelGGL(k_gr_cleanup, dim3(gpx), dim3(PT), 0, st, d_args + i);
+    ctx->launches += G;
+    VNM_LAUNCH_CHECK("k_gr_cleanup");
+    kt.stop();
+    std::vector<GreedyState> h(G);
+    for (int i = 0; i < G; ++i)
+        VNM_HIP_TRY(hipMemcpyAsync(&h[i], hargs[i].st, sizeof(GreedyState), hipMemcpyDeviceToHost, st));
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    const bool debug = getenv("VNM_DEBUG") != nullptr;
+    int ub = -1;
+    for (int i = 0; i < G; ++i) {
+        if (debug) debug_print(G == 1 ? -1 : i, h[i]);
+        if (h[i].status == VNM_E_UB_PATH) {
+            if (ub < 0) ub = i;
+            continue;
+        }
+        if ((rc = relabel(gs[i], gs[i]->perm.p))) return rc;
+        stats[i].init_size = h[i].init_size;
+        stats[i].final_size = h[i].L;
+        stats[i].init_cnt = h[i].init_cnt;
+        stats[i].final_cnt = h[i].totcnt;
+        stats[i].rounds = h[i].round - 1;
+    }
+    if (ub >= 0)
+        return set_error(VNM_E_UB_PATH, "greedy of graph %d reached top() of an empty swap queue "
+                                        "(undefined in the reference)", ub);
+    if (trace_len) *trace_len = h[0].trace_len;
+    return VNM_OK;
 }
 
 int nm_reorder_ex(vnm_graph *g, int N, int sched, int maxiter, int round_cap, int32_t *d_trace,
@@ -1362,67 +1517,25 @@ int nm_reorder_ex(vnm_graph *g, int N, int sched, int maxiter, int round_cap, in
         return nm_reorder_allpairs(g, N, sched, maxiter, 0, stats);
     }
     if (sched < 0) return set_error(VNM_E_ARG, "schedule %d is negative", sched);
-    if ((rc = build_csc(g))) return rc;
-    if ((rc = locate_all(g, N, g->cnt.p, g->nzr.p))) return rc;
-    hipStream_t st = g->ctx->stream;
-    const size_t st_words = (sizeof(GreedyState) + 3) / 4;
-    if (!g->gst.p && (rc = g->gst.alloc(st_words))) return rc;
-    if (!g->rt.p) {
-        // records A and B, 16 bytes per row each, zero between runs
-        if ((rc = g->rt.alloc((size_t)g->nrows * 8))) return rc;
-        VNM_HIP_TRY(hipMemsetAsync(g->rt.p, 0, (size_t)g->nrows * 32, st));
+    return greedy_run(&g, &sched, 1, N, maxiter, round_cap, d_trace, trace_cap, trace_len, stats);
+}
+
+// The greedy of many graphs in one launch chain.  Each graph keeps its own
+// schedule, stats and result; they are those of nm_reorder on the graph alone.
+int nm_reorder_batch(vnm_graph *const *gs, int count, int N, const int *scheds, int maxiter,
+                     vnm_nm_stats *stats) {
+    if (count < 1 || count > MAX_BATCH) return set_error(VNM_E_ARG, "batch of %d graphs (1..%d)", count, MAX_BATCH);
+    for (int i = 0; i < count; ++i) {
+        if (!gs[i]) return set_error(VNM_E_ARG, "graph %d is null", i);
+        if (gs[i]->ctx != gs[0]->ctx) return set_error(VNM_E_ARG, "graphs of a batch share one context");
+        int rc = check_nm_args(gs[i], N);
+        if (rc) return rc;
+        if (scheds[i] < 0 || (scheds[i] >= 4 && scheds[i] <= 7))
+            return set_error(VNM_E_ARG, "batched schedules are 0-3 or >= 8 (graph %d: %d)", i, scheds[i]);
+        for (int j = 0; j < i; ++j)
+            if (gs[j] == gs[i]) return set_error(VNM_E_ARG, "graph %d appears twice in the batch", i);
     }
-    NMArgs A;
-    A.st = (GreedyState *)g->gst.p;
-    A.cptr = g->cptr.p; A.crow = g->crow.p; A.perm = g->perm.p; A.inv = g->inv.p;
-    A.cnt = g->cnt.p; A.nzr = g->nzr.p;
-    A.rta = g->rt.p; A.rtb = g->rt.p + (size_t)g->nrows * 4;
-    const int gp = pass_grid(g);
-    KernelTimer kt(g->ctx, "nm_greedy");
-    hipLaunchKernelGGL(k_gr_init, dim3(1), dim3(GT), 0, st, A, g->nrows, g->M, N, g->nseg(), sched, maxiter,
-                       round_cap > 0 ? round_cap : 0x7FFFFFFF, d_trace, d_trace ? trace_cap : (int64_t)0);
-    g->ctx->launches++;
-    VNM_LAUNCH_CHECK("k_gr_init");
-    // rounds run in captured batches of ROUND_BATCH; the kernels of rounds
-    // past the end return at entry
-    if (!g->nm_exec && (rc = capture_rounds(g, A, gp))) return rc;
-    int32_t done = 0;
-    while (!done) {
-        VNM_HIP_TRY(hipGraphLaunch(g->nm_exec, st));
-        g->ctx->launches += 6 * ROUND_BATCH;
-        VNM_HIP_TRY(hipMemcpyAsync(&done, &A.st->done, 4, hipMemcpyDeviceToHost, st));
-        VNM_HIP_TRY(hipStreamSynchronize(st));
-    }
-    hipLaunchKernelGGL(k_gr_cleanup, dim3(gp), dim3(PT), 0, st, A);
-    g->ctx->launches++;
-    VNM_LAUNCH_CHECK("k_gr_cleanup");
-    kt.stop();
-    GreedyState h;
-    VNM_HIP_TRY(hipMemcpyAsync(&h, A.st, sizeof h, hipMemcpyDeviceToHost, st));
-    VNM_HIP_TRY(hipStreamSynchronize(st));
-    if (getenv("VNM_DEBUG"))
-        fprintf(stderr, "[vnm] greedy rounds=%d entries=%lld kernel_us select=%llu fill_a=%llu score=%llu "
-                        "greedy=%llu (heaps %llu, sequential %llu) fill_b=%llu count=%llu\n",
-                h.round - 1, (long long)h.work, (unsigned long long)(h.clk[0] / 100),
-                (unsigned long long)(h.clk[1] / 100), (unsigned long long)(h.clk[2] / 100),
-                (unsigned long long)(h.clk[3] / 100), (unsigned long long)(h.clk[6] / 100),
-                (unsigned long long)(h.clk[7] / 100), (unsigned long long)(h.clk[4] / 100),
-                (unsigned long long)(h.clk[5] / 100));
-    if (getenv("VNM_DEBUG"))
-        fprintf(stderr, "[vnm]   select: commit %llu, segments %llu, columns %llu; heaps: ranks %llu\n",
-                (unsigned long long)(h.clk[8] / 100), (unsigned long long)(h.clk[9] / 100),
-                (unsigned long long)(h.clk[10] / 100), (unsigned long long)(h.clk[11] / 100));
-    if (h.status == VNM_E_UB_PATH)
-        return set_error(VNM_E_UB_PATH, "greedy reached top() of an empty swap queue "
-                                        "(undefined in the reference)");
-    if ((rc = relabel(g, g->perm.p))) return rc;
-    if (trace_len) *trace_len = h.trace_len;
-    stats->init_size = h.init_size;
-    stats->final_size = h.L;
-    stats->init_cnt = h.init_cnt;
-    stats->final_cnt = h.totcnt;
-    stats->rounds = h.round - 1;
-    return VNM_OK;
+    return greedy_run(gs, scheds, count, N, maxiter, 0, nullptr, 0, nullptr, stats);
 }
 
 int nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *stats) {
@@ -1514,6 +1627,12 @@ int vnm_pq_pop_order(vnm_ctx *ctx, const int32_t *scores, int count, int M, int3
 int vnm_nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *st) {
     if (!g || !st) return set_error(VNM_E_ARG, "bad arguments");
     return nm_reorder(g, N, sched, maxiter, st);
+}
+
+int vnm_nm_reorder_batch(vnm_graph *const *graphs, int count, int N, const int32_t *sched, int maxiter,
+                         vnm_nm_stats *st) {
+    if (!graphs || !sched || !st) return set_error(VNM_E_ARG, "bad arguments");
+    return nm_reorder_batch(graphs, count, N, sched, maxiter, st);
 }
 
 int vnm_nm_trace(vnm_graph *g, int N, int sched, int maxiter, int round_cap, int32_t *rec,

@@ -38,6 +38,7 @@ EXPORTS = [
     "vnm_pair_score", "vnm_lp_reorder", "vnm_hybrid_learned", "vnm_graph_clone",
     "vnm_spmm_set_shape", "vnm_spmm_get_shape", "vnm_gcn_layer", "vnm_batch_create",
     "vnm_batch_run", "vnm_batch_info", "vnm_batch_destroy", "vnm_nm_reorder_allpairs",
+    "vnm_nm_reorder_batch",
 ]
 
 LP_NPARAMS = 22784
@@ -154,6 +155,7 @@ def lib():
     L.vnm_swap_score.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, i32p]
     L.vnm_pq_pop_order.argtypes = [vp, i32p, ctypes.c_int, ctypes.c_int, i32p]
     L.vnm_nm_reorder.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(NMStats)]
+    L.vnm_nm_reorder_batch.argtypes = [P(vp), ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, P(NMStats)]
     L.vnm_venom_eval.argtypes = [vp, ctypes.c_int, i32p, i32p]
     L.vnm_venom_reorder.argtypes = [vp, ctypes.c_int, ctypes.c_int, P(VenomStats)]
     L.vnm_hybrid.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -413,6 +415,20 @@ class Graph:
         st = NMStats()
         _check(lib().vnm_nm_reorder(self.h, N, sched, maxiter, ctypes.byref(st)))
         return st.as_list()
+
+    @staticmethod
+    def nm_reorder_batch(graphs, N, sched, maxiter=10):
+        """nm_reorder of several graphs of one context in one launch chain;
+        sched is one schedule for all or one per graph.  Returns the list of
+        per-graph stats."""
+        n = len(graphs)
+        if isinstance(sched, int):
+            sched = [sched] * n
+        hs = (ctypes.c_void_p * n)(*[g.h for g in graphs])
+        sc = np.ascontiguousarray(sched, dtype=np.int32)
+        st = (NMStats * n)()
+        _check(lib().vnm_nm_reorder_batch(hs, n, N, _p(sc, ctypes.c_int32), maxiter, st))
+        return [s.as_list() for s in st]
 
     def nm_reorder_allpairs(self, N, sched, maxiter=10, clean=False):
         st = NMStats()
