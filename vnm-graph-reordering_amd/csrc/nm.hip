@@ -1366,13 +1366,14 @@ static int capture_rounds(hipStream_t st, const NMArgs *d_args, int G, int gpx, 
     return VNM_OK;
 }
 
-// Workgroups per graph of the record pass kernels: two per CU for one graph,
-// fewer per graph in a batch (the grid-stride loops cover any count).
+// Workgroups per graph of the record pass kernels: two per CU for up to four
+// graphs (one graph's round can touch millions of entries), fewer per graph
+// in larger batches (the grid-stride loops cover any count).
 static int pass_grid(vnm_ctx *ctx, int G) {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || ncu < 1)
         ncu = 1;
-    return std::max(16, 2 * ncu / G);
+    return std::max(16, std::min(2 * ncu, 8 * ncu / G));
 }
 
 // Bytes of the greedy state of a graph, then its one-graph argument block
