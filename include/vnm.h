@@ -54,10 +54,21 @@ const char *vnm_last_error(void);
  * The arrays are allocated by the library; release with vnm_coo_free. */
 int vnm_mtx_read(const char *path, int M, vnm_coo *out);
 
+/* vnm_mtx_read with the host worker threads given (0 = the machine's, at
+ * most 16; 1 = one thread, the reference's sequential fscanf loop and
+ * std::sort, mtx/readMtx.hpp:84-147,56-82).  Any thread count gives the
+ * same COO and the same first error. */
+int vnm_mtx_read_ex(const char *path, int M, int threads, vnm_coo *out);
+
 /* Replaces Mtx::write_to_file (mtx/Mtx.h:412-435): "%%MatrixMarket matrix
  * coordinate real general", "nrows ncols nnz", then "%d %d %.2f" per entry
  * in COO order (1-based). */
 int vnm_mtx_write(const char *path, const vnm_coo *coo);
+
+/* vnm_mtx_write with the formatting threads given (0 = the machine's, at
+ * most 16; 1 = the reference's single fprintf loop); the bytes written do
+ * not depend on it. */
+int vnm_mtx_write_ex(const char *path, const vnm_coo *coo, int threads);
 
 /* The loader's ordering applied to raw tuples (mtx/readMtx.hpp:152-219,419,
  * mtx/Mtx.h:105-112). */

@@ -27,7 +27,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-I" + INC,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-munsafe-fp-atomics"]
 CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v3", "-I" + INC, "-Wall",
-             "-Wno-unused-result"]
+             "-Wno-unused-result", "-pthread"]
 
 
 def _run(cmd):
@@ -58,7 +58,7 @@ def build(jobs=8, verbose=False):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    _run([HIPCC, "-shared", "-fPIC", "-pthread", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     cli = os.path.join(SRC, "cli_main.cpp")
     if os.path.exists(cli):

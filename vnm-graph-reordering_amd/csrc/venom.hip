@@ -432,7 +432,7 @@ int reload_sort(vnm_graph *g, bool quantize) {
         t.nrows = c.nrows; t.ncols = c.ncols; t.symmetric = false;
         t.row = c.row; t.col = c.col; t.val = c.val;
         HostCoo o;
-        loader_order(t, 0, o);
+        loader_order(t, 0, o, 0);
         g->nnz = (int64_t)o.row.size();
         VNM_HIP_TRY(hipMemcpyAsync(g->row.p, o.row.data(), g->nnz * 4, hipMemcpyHostToDevice, st));
         VNM_HIP_TRY(hipMemcpyAsync(g->col.p, o.col.data(), g->nnz * 4, hipMemcpyHostToDevice, st));

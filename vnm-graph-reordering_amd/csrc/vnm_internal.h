@@ -29,9 +29,11 @@ int set_error(int code, const char *fmt, ...);
 const char *last_error();
 
 // host I/O (mtx.cpp)
-int read_mtx_tuples(const char *path, Tuples &t);
-void loader_order(Tuples &t, int M, HostCoo &out);
-int write_mtx(const char *path, const HostCoo &c);
+// threads: worker threads of the host passes (0 = the machine's, at most 16)
+int host_threads(int threads);
+int read_mtx_tuples(const char *path, Tuples &t, int threads);
+void loader_order(Tuples &t, int M, HostCoo &out, int threads);
+int write_mtx(const char *path, const HostCoo &c, int threads);
 int coo_export(const HostCoo &c, vnm_coo *out);
 void coo_import(const vnm_coo *in, HostCoo &c);
 float quant_cpp_write(float v);

@@ -38,7 +38,7 @@ EXPORTS = [
     "vnm_pair_score", "vnm_lp_reorder", "vnm_hybrid_learned", "vnm_graph_clone",
     "vnm_spmm_set_shape", "vnm_spmm_get_shape", "vnm_gcn_layer", "vnm_batch_create",
     "vnm_batch_run", "vnm_batch_info", "vnm_batch_destroy", "vnm_nm_reorder_allpairs",
-    "vnm_nm_reorder_batch",
+    "vnm_nm_reorder_batch", "vnm_mtx_read_ex", "vnm_mtx_write_ex",
 ]
 
 LP_NPARAMS = 22784
@@ -125,6 +125,8 @@ def lib():
     L.vnm_last_error.restype = ctypes.c_char_p
     L.vnm_mtx_read.argtypes = [ctypes.c_char_p, ctypes.c_int, P(Coo)]
     L.vnm_mtx_write.argtypes = [ctypes.c_char_p, P(Coo)]
+    L.vnm_mtx_read_ex.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P(Coo)]
+    L.vnm_mtx_write_ex.argtypes = [ctypes.c_char_p, P(Coo), ctypes.c_int]
     L.vnm_coo_from_tuples.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int, ctypes.c_int64,
                                       i32p, i32p, P(ctypes.c_float), ctypes.c_int, P(Coo)]
     L.vnm_coo_free.argtypes = [P(Coo)]
@@ -243,14 +245,16 @@ def _take(c):
     return out
 
 
-def mtx_read(path, M):
+def mtx_read(path, M, threads=0):
+    """Load a MatrixMarket file as the reference loader does; threads = host
+    worker threads (0: the machine's, 1: sequential), same result for any."""
     c = Coo()
-    _check(lib().vnm_mtx_read(str(path).encode(), M, ctypes.byref(c)))
+    _check(lib().vnm_mtx_read_ex(str(path).encode(), M, threads, ctypes.byref(c)))
     return _take(c)
 
 
-def mtx_write(path, coo):
-    _check(lib().vnm_mtx_write(str(path).encode(), ctypes.byref(coo.c())))
+def mtx_write(path, coo, threads=0):
+    _check(lib().vnm_mtx_write_ex(str(path).encode(), ctypes.byref(coo.c()), threads))
 
 
 def coo_from_tuples(nrows, ncols, row, col, val, M, symmetric=False):
