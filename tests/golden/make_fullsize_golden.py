@@ -2,9 +2,10 @@
 
     python tests/golden/make_fullsize_golden.py
 
-Runs the CPU oracle (minutes) on the seeded R-MAT 65,536-node workload and
-records the N:M statistics and a SHA-256 of the relabelled COO, so the GPU
-tests can check the full-size run bit-for-bit without rerunning the oracle.
+Runs the CPU oracle (minutes) on the seeded R-MAT 65,536-node workload,
+schedules 0-3, and records the N:M statistics and a SHA-256 of each
+relabelled COO, so the GPU tests can check the full-size runs bit-for-bit
+without rerunning the oracle.
 Output: tests/golden/fullsize.json.
 """
 import hashlib
@@ -32,9 +33,12 @@ def digest(m):
 def main():
     out = {}
     m = vnm.gen_rmat(16, 524288, 2)  # 65,536 nodes, nnz 1,048,576, seed 2
-    om = oracle.Matrix(m.nrows, m.ncols, m.row, m.col, m.val)
-    st, _ = oracle.nm_reorder(om, 2, 8, 0, 10)
-    out["rmat16_seed2_nm_sched0"] = {"stats": st.tolist(), "sha256": digest(om)}
+    for sched in (0, 1, 2, 3):  # find_best_2toX's four schedules
+        # the oracle relabels in place: every schedule starts from a copy
+        om = oracle.Matrix(m.nrows, m.ncols, m.row.copy(), m.col.copy(), m.val.copy())
+        st, _ = oracle.nm_reorder(om, 2, 8, sched, 10)
+        out["rmat16_seed2_nm_sched%d" % sched] = {"stats": st.tolist(), "sha256": digest(om)}
+        print(sched, out["rmat16_seed2_nm_sched%d" % sched], flush=True)
     print(out, flush=True)
     json.dump(out, open(os.path.join(HERE, "fullsize.json"), "w"), indent=1)
 

@@ -243,3 +243,42 @@ def test_trained_proposer_beats_random_and_untrained(ctx):
     assert fin["trained"] <= fin["exhaustive"], fin
     assert pairs["trained"] <= 1.2 * pairs["random"], pairs
     assert pairs["trained"] <= 0.2 * pairs["exhaustive"], pairs
+
+
+@pytest.mark.parametrize("mode", [vnm.LP_LEARNED, vnm.LP_RANDOM, vnm.LP_EXHAUSTIVE])
+def test_lp_reorder_batch_matches_single_runs(ctx, mode):
+    """Uneven graphs reordered by the batched entry point on four worker
+    streams: stats and relabelled COO equal each graph's own lp_reorder with
+    the same seed, and a second batch continues from the reordered graphs."""
+    model = vnm.LPModel.load(ctx, WEIGHTS) if mode == vnm.LP_LEARNED else None
+    rng = np.random.default_rng(77)
+    coos = [vnm.gen_chung_lu(int(n), 2.1, 12.0, 3000 + i, M=M)
+            for i, n in enumerate(rng.integers(500, 20000, 9))]
+    coos.append(vnm.gen_rmat(12, 30000, 21))
+    coos.append(vnm.coo_from_tuples(64, 64, [], [], [], M))  # no entries
+    kw = dict(topk=512, window=8, maxiter=12, seed=17)
+    batch = [vnm.Graph(ctx, m, M) for m in coos]
+    single = [vnm.Graph(ctx, m, M) for m in coos]
+    for rep in range(2):
+        got = vnm.Graph.lp_reorder_batch(batch, model, N, mode=mode, workers=4, **kw)
+        for i, (gb, gs) in enumerate(zip(batch, single)):
+            want = gs.lp_reorder(model, N, mode=mode, **kw)
+            a, b = dict(got[i].as_dict()), dict(want.as_dict())
+            a.pop("ms"), b.pop("ms")
+            assert a == b, (rep, i, a, b)
+            ob, os_ = gb.download(), gs.download()
+            assert np.array_equal(ob.row, os_.row) and np.array_equal(ob.col, os_.col), (rep, i)
+            assert np.array_equal(gb.perm(), gs.perm()), (rep, i)
+
+
+def test_lp_reorder_batch_rejects_bad_batches(ctx):
+    m = vnm.gen_rmat(10, 4000, 3)
+    a, b = vnm.Graph(ctx, m, M), vnm.Graph(ctx, m, M)
+    with pytest.raises(vnm.VnmError):
+        vnm.Graph.lp_reorder_batch([a, a], None, N, mode=vnm.LP_RANDOM)  # the same graph twice
+    with pytest.raises(vnm.VnmError):
+        vnm.Graph.lp_reorder_batch([a, b], None, N, mode=vnm.LP_LEARNED)  # learned needs a model
+    with pytest.raises(vnm.VnmError):
+        vnm.Graph.lp_reorder_batch([a, b], None, N, mode=vnm.LP_RANDOM, workers=0)
+    st = vnm.Graph.lp_reorder_batch([a, b], None, N, mode=vnm.LP_RANDOM, maxiter=4, seed=3)
+    assert st[0].final_cnt == st[1].final_cnt  # same graph, same seed

@@ -190,6 +190,27 @@ def test_greedy_batch_rejects_bad_batches(ctx):
     assert got == want
 
 
+def test_greedy_batch_fullsize_golden(ctx):
+    # find_best_2toX's four schedules on R-MAT 65,536 (seed 2) in one batch:
+    # each graph's stats and relabelled COO equal the oracle's own run
+    import hashlib
+    gold = json.load(open(os.path.join(GOLD, "fullsize.json")))
+    m = vnm.gen_rmat(16, 524288, 2)
+    base = vnm.Graph(ctx, m, 8)
+    gs = [base.clone() for _ in range(4)]
+    got = vnm.Graph.nm_reorder_batch(gs, 2, [0, 1, 2, 3], 10)
+    for s, (g, st) in enumerate(zip(gs, got)):
+        ref = gold["rmat16_seed2_nm_sched%d" % s]
+        assert st == ref["stats"], (s, st)
+        out = g.download()
+        h = hashlib.sha256()
+        for a in (out.row, out.col, out.val):
+            h.update(np.ascontiguousarray(a).tobytes())
+        assert h.hexdigest() == ref["sha256"], s
+    imp = [(r[2] - r[3]) / r[2] for r in (gold["rmat16_seed2_nm_sched%d" % s]["stats"] for s in range(4))]
+    assert base.find_best_sched(2, 4) == int(np.argmax(imp))
+
+
 def test_kat_temp_mtx(ctx):
     m = vnm.mtx_read(os.path.join(GOLD, "mtx", "temp.mtx"), 8)
     g = vnm.Graph(ctx, m, 8)

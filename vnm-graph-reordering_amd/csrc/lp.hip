@@ -42,10 +42,13 @@
 #include "tables.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
+#include <thread>
 #include <vector>
 
 struct vnm_lp_model {
@@ -1304,13 +1307,18 @@ static int lp_score_cands(vnm_graph *g, LPCache *c, int N, Cand *cand, int4 *bes
     return VNM_OK;
 }
 
-int lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window, int maxiter,
-                      uint64_t seed, vnm_lp_stats *stats) {
+static int lp_reorder_args(vnm_graph *g, const vnm_lp_model *m, int mode, int topk, int window, int maxiter) {
     if (mode < 0 || mode > 2) return set_error(VNM_E_ARG, "mode must be 0 (learned), 1 (random) or 2 (exhaustive)");
     if (mode == 0 && !m) return set_error(VNM_E_ARG, "learned mode needs a model");
     if (m && m->ctx != g->ctx) return set_error(VNM_E_ARG, "model and graph belong to different contexts");
     if (topk < 1 || window < 1 || maxiter < 0) return set_error(VNM_E_ARG, "topk, window >= 1, maxiter >= 0");
     if (g->nseg() > (1 << 18)) return set_error(VNM_E_ARG, "more than 2^18 segments");
+    return VNM_OK;
+}
+
+// The reorder itself, on the stream of g->ctx (arguments checked).
+static int lp_reorder_run(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window,
+                          int maxiter, uint64_t seed, vnm_lp_stats *stats) {
     auto t0 = std::chrono::steady_clock::now();
     LPCache *c;
     int rc = lp_begin(g, N, &c);
@@ -1399,6 +1407,94 @@ int lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, i
     VNM_HIP_TRY(hipStreamSynchronize(st));
     S.ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *stats = S;
+    return VNM_OK;
+}
+
+int lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window, int maxiter,
+               uint64_t seed, vnm_lp_stats *stats) {
+    int rc = lp_reorder_args(g, m, mode, topk, window, maxiter);
+    if (rc) return rc;
+    return lp_reorder_run(g, m, N, mode, topk, window, maxiter, seed, stats);
+}
+
+// The learned reorder of many graphs of one context: `workers` host threads,
+// each with a stream of its own, take the graphs in turn (a graph's run
+// launches dozens of small kernels and waits on the host three times per
+// iteration, so runs on separate streams overlap).  A graph's run, and so
+// its result, is the one lp_reorder makes alone with the same seed.
+int lp_reorder_batch(vnm_graph *const *gs, int count, const vnm_lp_model *m, int N, int mode, int topk,
+                     int window, int maxiter, uint64_t seed, int workers, vnm_lp_stats *stats) {
+    if (count < 1) return set_error(VNM_E_ARG, "batch of %d graphs", count);
+    if (workers < 1 || workers > 16) return set_error(VNM_E_ARG, "workers must be 1..16 (got %d)", workers);
+    vnm_ctx *ctx = gs[0] ? gs[0]->ctx : nullptr;
+    for (int i = 0; i < count; ++i) {
+        if (!gs[i]) return set_error(VNM_E_ARG, "graph %d is null", i);
+        if (gs[i]->ctx != ctx) return set_error(VNM_E_ARG, "graphs of a batch share one context");
+        int rc = lp_reorder_args(gs[i], m, mode, topk, window, maxiter);
+        if (rc) return rc;
+        rc = lp_check(gs[i], N);
+        if (rc) return rc;
+        for (int j = 0; j < i; ++j)
+            if (gs[j] == gs[i]) return set_error(VNM_E_ARG, "graph %d appears twice in the batch", i);
+    }
+    VNM_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int W = std::min(workers, count);
+    // one stream per worker, default priority, released on every path
+    struct Aux {
+        std::vector<vnm_ctx> c;
+        explicit Aux(int n) : c(n) {}
+        ~Aux() {
+            for (auto &x : c)
+                if (x.stream) {
+                    (void)hipStreamSynchronize(x.stream);
+                    (void)hipStreamDestroy(x.stream);
+                }
+        }
+    } aux(W);
+    for (int w = 0; w < W; ++w) {
+        aux.c[w].device = ctx->device;
+        hipError_t e = hipStreamCreateWithFlags(&aux.c[w].stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            aux.c[w].stream = nullptr;
+            return set_error(VNM_E_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+        }
+    }
+    std::atomic<int> next{0}, failed{0};
+    std::vector<int> wrc(W, VNM_OK), wat(W, -1);
+    std::vector<std::string> wmsg(W);
+    auto work = [&](int w) {
+        if (hipSetDevice(ctx->device) != hipSuccess) {
+            wrc[w] = VNM_E_HIP;
+            wmsg[w] = "hipSetDevice failed in a worker";
+            failed.store(1);
+            return;
+        }
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= count || failed.load()) return;
+            vnm_graph *g = gs[i];
+            g->ctx = &aux.c[w];
+            const int rc = lp_reorder_run(g, m, N, mode, topk, window, maxiter, seed, &stats[i]);
+            g->ctx = ctx;
+            if (rc) {
+                wrc[w] = rc;
+                wat[w] = i;
+                wmsg[w] = last_error();
+                failed.store(1);
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < W; ++w) th.emplace_back(work, w);
+    work(0);
+    for (auto &t : th) t.join();
+    for (auto &x : aux.c) ctx->launches += x.launches;
+    // the failure of the lowest graph index is reported
+    int bad = -1;
+    for (int w = 0; w < W; ++w)
+        if (wrc[w] != VNM_OK && (bad < 0 || wat[w] < wat[bad] || wat[bad] < 0)) bad = w;
+    if (bad >= 0) return set_error(wrc[bad], "graph %d: %s", wat[bad], wmsg[bad].c_str());
     return VNM_OK;
 }
 
@@ -1567,6 +1663,12 @@ int vnm_lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int top
                    uint64_t seed, vnm_lp_stats *st) {
     if (!g || !st) return set_error(VNM_E_ARG, "bad arguments");
     return lp_reorder(g, m, N, mode, topk, window, maxiter, seed, st);
+}
+
+int vnm_lp_reorder_batch(vnm_graph *const *graphs, int count, const vnm_lp_model *m, int N, int mode, int topk,
+                         int window, int maxiter, uint64_t seed, int workers, vnm_lp_stats *st) {
+    if (!graphs || !st) return set_error(VNM_E_ARG, "bad arguments");
+    return lp_reorder_batch(graphs, count, m, N, mode, topk, window, maxiter, seed, workers, st);
 }
 
 }  // extern "C"

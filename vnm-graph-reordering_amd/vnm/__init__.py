@@ -38,7 +38,7 @@ EXPORTS = [
     "vnm_pair_score", "vnm_lp_reorder", "vnm_hybrid_learned", "vnm_graph_clone",
     "vnm_spmm_set_shape", "vnm_spmm_get_shape", "vnm_gcn_layer", "vnm_batch_create",
     "vnm_batch_run", "vnm_batch_info", "vnm_batch_destroy", "vnm_nm_reorder_allpairs",
-    "vnm_nm_reorder_batch", "vnm_mtx_read_ex", "vnm_mtx_write_ex",
+    "vnm_nm_reorder_batch", "vnm_mtx_read_ex", "vnm_mtx_write_ex", "vnm_lp_reorder_batch",
 ]
 
 LP_NPARAMS = 22784
@@ -185,6 +185,8 @@ def lib():
     L.vnm_pair_score.argtypes = [vp, ctypes.c_int, i32p, i32p, ctypes.c_int, i32p]
     L.vnm_lp_reorder.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_uint64, P(LPStats)]
+    L.vnm_lp_reorder_batch.argtypes = [P(vp), ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, P(LPStats)]
     L.vnm_hybrid_learned.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, P(HybridStats)]
     L.vnm_graph_clone.argtypes = [vp, P(vp)]
@@ -470,6 +472,18 @@ class Graph:
         _check(lib().vnm_lp_reorder(self.h, model.h if model is not None else None, N, mode, topk,
                                     window, maxiter, seed, ctypes.byref(st)))
         return st
+
+    @staticmethod
+    def lp_reorder_batch(graphs, model, N, mode=LP_LEARNED, topk=4096, window=16, maxiter=20, seed=1,
+                         workers=4):
+        """lp_reorder of several graphs of one context on `workers` streams;
+        each result is lp_reorder's on the graph alone.  Returns the stats."""
+        n = len(graphs)
+        hs = (ctypes.c_void_p * n)(*[g.h for g in graphs])
+        st = (LPStats * n)()
+        _check(lib().vnm_lp_reorder_batch(hs, n, model.h if model is not None else None, N, mode, topk,
+                                          window, maxiter, seed, workers, st))
+        return list(st)
 
     def venom_eval(self, V):
         a, b = ctypes.c_int32(), ctypes.c_int32()
