@@ -169,28 +169,42 @@ __global__ __launch_bounds__(SS_T) void k_lp_seg_stats(
     __shared__ int sel[1];
     __shared__ int colc[8][4];
     __shared__ int segc[4];
+    __shared__ RowHash H;
     uint32_t *slot = tab + (size_t)blockIdx.x * (R >> 2);
     const int n = segs ? nsel : nseg;
     const int tid = threadIdx.x;
+    hash_reset(H);
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         if (tid == 0) sel[0] = segs ? segs[i] : i;
         if (tid < 32) colc[tid >> 2][tid & 3] = 0;
         if (tid < 4) segc[tid] = 0;
         __syncthreads();
         const int total = table_columns(sel, 1, M, inv, ptr, cols, pre);
-        table_scatter(slot, M, R, cols, pre, M, total, idx);
+        // row bitmaps: the LDS hash for small segments, the global slot else
+        const bool small = total <= HASH_MAX;
+        if (small) hash_scatter(H, M, cols, pre, M, total, idx);
+        else table_scatter(slot, M, R, cols, pre, M, total, idx);
         __syncthreads();
-        for (int e = tid; e < total; e += SS_T) {
-            const int t = entry_col(pre, M, e);
-            const int r = idx[cols[t].start + (e - pre[t])];
-            const uint32_t bits = slot_byte(slot, 0, R, r);
-            const int pc = popc32(bits);
-            const int k = pc < N ? 0 : pc == N ? 1 : pc == N + 1 ? 2 : 3;
-            atomicAdd(&colc[t][k], 1);
-            if ((bits >> (M - 1 - t)) == 1u) atomicAdd(&segc[k], 1);
+        // TAB_U entries per thread resolved before their loads are issued
+        for (int e0 = tid; e0 < total; e0 += SS_T * TAB_U) {
+            int t[TAB_U], r[TAB_U];
+            uint32_t bits[TAB_U];
+            table_entries(cols, pre, M, total, idx, e0, SS_T, t, r);
+#pragma unroll
+            for (int u = 0; u < TAB_U; ++u)
+                bits[u] = t[u] < 0 ? 0u : small ? (hash_get(H, (uint32_t)r[u]) & 0xFFu) : slot_byte(slot, 0, R, r[u]);
+#pragma unroll
+            for (int u = 0; u < TAB_U; ++u) {
+                if (t[u] < 0) continue;
+                const int pc = popc32(bits[u]);
+                const int k = pc < N ? 0 : pc == N ? 1 : pc == N + 1 ? 2 : 3;
+                atomicAdd(&colc[t[u]][k], 1);
+                if ((bits[u] >> (M - 1 - t[u])) == 1u) atomicAdd(&segc[k], 1);
+            }
         }
         __syncthreads();
-        table_clear(slot, M, R, cols, pre, M, total, idx);
+        if (small) hash_reset(H);
+        else table_clear(slot, M, R, cols, pre, M, total, idx);
         const int s = sel[0];
         if (tid < M * 4) {
             const int node = ld_l2(&inv[s * M + (tid >> 2)]);
@@ -643,13 +657,13 @@ __global__ void k_lp_pick(int nseg, int M, int mode, uint64_t seed, const int32_
     }
     dbest[s] = (int8_t)bdi;
     abest[s] = (int8_t)bai;
-    if (bdi >= 0) {
-        dkey[s] = ((uint64_t)(~ord_f32(bd)) << 32) | (uint32_t)s;
-        atomicAdd(ndonor, 1);
-    } else {
-        dkey[s] = ~0ull;
-    }
+    dkey[s] = bdi >= 0 ? ((uint64_t)(~ord_f32(bd)) << 32) | (uint32_t)s : ~0ull;
     akey[s] = ((uint64_t)(~ord_f32(ba)) << 32) | (uint32_t)s;
+    // donors counted once per wave (lanes past nseg have returned)
+    const uint64_t dm = __ballot(bdi >= 0);
+    const uint64_t live = __ballot(1);
+    if ((int)(threadIdx.x & 63) == __ffsll((unsigned long long)live) - 1 && dm)
+        atomicAdd(ndonor, (int)__popcll(dm));
 }
 
 struct Cand { int p, q, bu, bv; };
@@ -714,10 +728,14 @@ __global__ __launch_bounds__(256) void k_lp_fill_pairs(int R, int M, const Cand 
                                                        int clear) {
     __shared__ ColSpan cols[16];
     __shared__ int pre[17];
-    __shared__ int sel[1];
+    __shared__ int sel[2];
     const Cand c = cand[blockIdx.x >> 1];
     const int seg = (blockIdx.x & 1) ? c.q : c.p;
     if (c.q < 0) return;  // uniform per workgroup
+    if (threadIdx.x == 0) { sel[0] = c.p; sel[1] = c.q; }
+    __syncthreads();
+    // pairs of at most HASH_MAX entries are scored from an LDS hash
+    if (table_columns(sel, 2, M, inv, ptr, cols, pre) <= HASH_MAX) return;
     if (threadIdx.x == 0) sel[0] = seg;
     __syncthreads();
     const int total = table_columns(sel, 1, M, inv, ptr, cols, pre);
@@ -746,6 +764,7 @@ __global__ __launch_bounds__(256) void k_lp_pair_gain(int R, int M, int N, const
     __shared__ int pre[17];
     __shared__ int sel[2];
     __shared__ int sBl[64], sBr[64], sEp[8], sEq[8];
+    __shared__ RowHash H;
     const int i = blockIdx.x;
     const Cand c = cand[i];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -758,49 +777,77 @@ __global__ __launch_bounds__(256) void k_lp_pair_gain(int R, int M, int N, const
     if (tid < 64) { sBl[tid] = 0; sBr[tid] = 0; }
     if (tid < 8) { sEp[tid] = 0; sEq[tid] = 0; }
     __syncthreads();
+    hash_reset(H);
     const int total = table_columns(sel, 2, M, inv, ptr, cols, pre);
     uint32_t *sp = tab + (size_t)(2 * i) * (R >> 2);
     uint32_t *sq = tab + (size_t)(2 * i + 1) * (R >> 2);
+    // small pairs: both segments' row bitmaps in the LDS hash (byte 0 = p,
+    // byte 1 = q); larger ones in the slots k_lp_fill_pairs filled
+    const bool small = total <= HASH_MAX;
+    if (small) {
+        hash_scatter(H, M, cols, pre, 2 * M, total, idx);
+        __syncthreads();
+    }
     const int b1 = lane / M, b2 = lane % M;
     int aBl = 0, aBr = 0, aEp = 0, aEq = 0;
-    for (int base = 0; base < total; base += 256) {
-        const int e = base + tid;
-        bool isP = false, isQ = false;
-        uint32_t l = 0, rr = 0;
-        if (e < total) {
-            const int t = entry_col(pre, 2 * M, e);
-            const int r = idx[cols[t].start + (e - pre[t])];
-            if (t < M) {
-                l = slot_byte(sp, 0, R, r);
-                isP = (l >> (M - 1 - t)) == 1u;  // the row's owner entry in p
-                if (isP) rr = slot_byte(sq, 0, R, r);
+    // TAB_U entries per thread resolved and their bitmaps loaded before any
+    // is folded in (the loads are independent, so they overlap)
+    for (int base = 0; base < total; base += 256 * TAB_U) {
+        int tt[TAB_U], rw[TAB_U];
+        uint32_t lv[TAB_U], rv[TAB_U];
+        table_entries(cols, pre, 2 * M, total, idx, base + tid, 256, tt, rw);
+#pragma unroll
+        for (int u = 0; u < TAB_U; ++u) {
+            lv[u] = 0u;
+            rv[u] = 0u;
+            if (tt[u] < 0) continue;
+            if (small) {
+                const uint32_t h = hash_get(H, (uint32_t)rw[u]);
+                lv[u] = h & 0xFFu;
+                rv[u] = (h >> 8) & 0xFFu;
             } else {
-                rr = slot_byte(sq, 0, R, r);
-                isQ = (rr >> (M - 1 - (t - M))) == 1u;
+                if (tt[u] < M) lv[u] = slot_byte(sp, 0, R, rw[u]);
+                rv[u] = slot_byte(sq, 0, R, rw[u]);
             }
         }
-        const int pl = popc32(l), pr = popc32(rr);
-        const bool both = isP && rr != 0;
-        const uint64_t Ap = __ballot(isP && pl == N + 1);
-        const uint64_t Dq = __ballot(isQ && pr == N + 1);
-        const uint64_t A = __ballot(both && pl == N + 1), B = __ballot(both && pl == N);
-        const uint64_t C = __ballot(both && pr == N), D = __ballot(both && pr == N + 1);
-        uint64_t Lb = 0, Rb = 0, Lown = 0, Rown = 0;
-        for (int b = 0; b < M; ++b) {
-            const uint64_t lm = __ballot(isP && ((l >> (M - 1 - b)) & 1u));
-            const uint64_t rb = __ballot(both && ((rr >> (M - 1 - b)) & 1u));
-            const uint64_t rq = __ballot(isQ && ((rr >> (M - 1 - b)) & 1u));
-            if (b == b1) Lb = lm;
-            if (b == b2) Rb = rb;
-            if (b == lane) { Lown = lm; Rown = rq; }
-        }
-        if (lane < MM && (A | B | C | D)) {
-            aBl += __popcll(Rb & ((Lb & A) | (~Lb & B)));
-            aBr += __popcll(Lb & ((Rb & D) | (~Rb & C)));
-        }
-        if (lane < M) {
-            aEp += __popcll(Lown & Ap);
-            aEq += __popcll(Rown & Dq);
+#pragma unroll
+        for (int u = 0; u < TAB_U; ++u) {
+            const int t = tt[u];
+            bool isP = false, isQ = false;
+            uint32_t l = 0, rr = 0;
+            if (t >= 0) {
+                if (t < M) {
+                    l = lv[u];
+                    isP = (l >> (M - 1 - t)) == 1u;  // the row's owner entry in p
+                    if (isP) rr = rv[u];
+                } else {
+                    rr = rv[u];
+                    isQ = (rr >> (M - 1 - (t - M))) == 1u;
+                }
+            }
+            const int pl = popc32(l), pr = popc32(rr);
+            const bool both = isP && rr != 0;
+            const uint64_t Ap = __ballot(isP && pl == N + 1);
+            const uint64_t Dq = __ballot(isQ && pr == N + 1);
+            const uint64_t A = __ballot(both && pl == N + 1), B = __ballot(both && pl == N);
+            const uint64_t C = __ballot(both && pr == N), D = __ballot(both && pr == N + 1);
+            uint64_t Lb = 0, Rb = 0, Lown = 0, Rown = 0;
+            for (int b = 0; b < M; ++b) {
+                const uint64_t lm = __ballot(isP && ((l >> (M - 1 - b)) & 1u));
+                const uint64_t rb = __ballot(both && ((rr >> (M - 1 - b)) & 1u));
+                const uint64_t rq = __ballot(isQ && ((rr >> (M - 1 - b)) & 1u));
+                if (b == b1) Lb = lm;
+                if (b == b2) Rb = rb;
+                if (b == lane) { Lown = lm; Rown = rq; }
+            }
+            if (lane < MM && (A | B | C | D)) {
+                aBl += __popcll(Rb & ((Lb & A) | (~Lb & B)));
+                aBr += __popcll(Lb & ((Rb & D) | (~Rb & C)));
+            }
+            if (lane < M) {
+                aEp += __popcll(Lown & Ap);
+                aEq += __popcll(Rown & Dq);
+            }
         }
     }
     if (lane < MM) {

@@ -140,27 +140,39 @@ __global__ __launch_bounds__(LOC_T) void k_locate_all(
     __shared__ int pre[17];
     __shared__ int sel[1];
     __shared__ int c_inv, c_nz;
+    __shared__ RowHash H;
     uint32_t *slot = tab + (size_t)blockIdx.x * (R >> 2);
+    hash_reset(H);
     for (int s = blockIdx.x; s < nseg; s += gridDim.x) {
         if (threadIdx.x == 0) { sel[0] = s; c_inv = 0; c_nz = 0; }
         __syncthreads();
         int total = table_columns(sel, 1, M, inv, cptr, cols, pre);
-        table_scatter(slot, M, R, cols, pre, M, total, crow);
+        // row bitmaps: the LDS hash for small segments, the global slot else
+        const bool small = total <= HASH_MAX;
+        if (small) hash_scatter(H, M, cols, pre, M, total, crow);
+        else table_scatter(slot, M, R, cols, pre, M, total, crow);
         __syncthreads();
         int my_inv = 0, my_nz = 0;
-        for (int e = threadIdx.x; e < total; e += blockDim.x) {
-            int t = entry_col(pre, M, e);
-            int r = crow[cols[t].start + (e - pre[t])];
-            uint32_t bits = slot_byte(slot, 0, R, r);
-            if ((bits >> (M - 1 - t)) == 1u) {  // leading bit: the row's owner
-                my_nz += 1;
-                my_inv += popc32(bits) > N;
-            }
+        // TAB_U entries per thread resolved before their loads are issued
+        for (int e0 = threadIdx.x; e0 < total; e0 += blockDim.x * TAB_U) {
+            int t[TAB_U], r[TAB_U];
+            uint32_t bits[TAB_U];
+            table_entries(cols, pre, M, total, crow, e0, blockDim.x, t, r);
+#pragma unroll
+            for (int u = 0; u < TAB_U; ++u)
+                bits[u] = t[u] < 0 ? 0u : small ? (hash_get(H, (uint32_t)r[u]) & 0xFFu) : slot_byte(slot, 0, R, r[u]);
+#pragma unroll
+            for (int u = 0; u < TAB_U; ++u)
+                if (t[u] >= 0 && (bits[u] >> (M - 1 - t[u])) == 1u) {  // leading bit: the row's owner
+                    my_nz += 1;
+                    my_inv += popc32(bits[u]) > N;
+                }
         }
         if (my_inv) atomicAdd(&c_inv, my_inv);
         if (my_nz) atomicAdd(&c_nz, my_nz);
         __syncthreads();
-        table_clear(slot, M, R, cols, pre, M, total, crow);
+        if (small) hash_reset(H);
+        else table_clear(slot, M, R, cols, pre, M, total, crow);
         if (threadIdx.x == 0) { cnt[s] = c_inv; nzr[s] = c_nz; }
         __syncthreads();
     }

@@ -126,53 +126,59 @@ __global__ void k_blk_select(const uint64_t *__restrict__ bk, const int32_t *__r
                              uint16_t *__restrict__ blk_sel, uint32_t *__restrict__ blk_mask,
                              int32_t *__restrict__ run_blk, unsigned long long *__restrict__ retained) {
     const int lane = threadIdx.x & 63;
-    const int b = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (b >= nb) return;
-    const int first = blk_first[b];
-    int last = U;
-    if (b + 1 < nb) last = blk_first[b + 1];
-    // scores of 128 subsets at a time (lane and lane + 64; one round for
-    // M <= 8): the block's run bitmaps are loaded 64 at a time, one per
-    // lane, and broadcast
-    int best_score = -1, best_idx = 0x7FFFFFFF;
-    for (int s0 = 0; s0 < nsub; s0 += 128) {
-        const int i0 = s0 + lane, i1 = s0 + 64 + lane;
-        const uint32_t S0 = i0 < nsub ? subs[i0] : 0u;
-        const uint32_t S1 = i1 < nsub ? subs[i1] : 0u;
-        int sc0 = 0, sc1 = 0;
-        for (int base = first; base < last; base += 64) {
-            const int q = base + lane;
-            const int mine = q < last ? (int)bits[rid_sorted[q]] : 0;
-            const int n = min(64, last - base);
-            for (int j = 0; j < n; ++j) {
-                const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane(mine, j);
-                sc0 += min(2, popc32(bj & S0));
-                sc1 += min(2, popc32(bj & S1));
+    const int wpb = blockDim.x / 64;
+    // blocks in a grid-stride loop per wave; the retained count is summed per
+    // wave and added once (one atomic per block on a single address
+    // serialises at the L2)
+    unsigned long long kept = 0;
+    for (int b = blockIdx.x * wpb + (threadIdx.x >> 6); b < nb; b += gridDim.x * wpb) {
+        const int first = blk_first[b];
+        int last = U;
+        if (b + 1 < nb) last = blk_first[b + 1];
+        // scores of 128 subsets at a time (lane and lane + 64; one round for
+        // M <= 8): the block's run bitmaps are loaded 64 at a time, one per
+        // lane, and broadcast
+        int best_score = -1, best_idx = 0x7FFFFFFF;
+        for (int s0 = 0; s0 < nsub; s0 += 128) {
+            const int i0 = s0 + lane, i1 = s0 + 64 + lane;
+            const uint32_t S0 = i0 < nsub ? subs[i0] : 0u;
+            const uint32_t S1 = i1 < nsub ? subs[i1] : 0u;
+            int sc0 = 0, sc1 = 0;
+            for (int base = first; base < last; base += 64) {
+                const int q = base + lane;
+                const int mine = q < last ? (int)bits[rid_sorted[q]] : 0;
+                const int n = min(64, last - base);
+                for (int j = 0; j < n; ++j) {
+                    const uint32_t bj = (uint32_t)__builtin_amdgcn_readlane(mine, j);
+                    sc0 += min(2, popc32(bj & S0));
+                    sc1 += min(2, popc32(bj & S1));
+                }
             }
+            // ascending subset index per lane: strict > keeps the smaller on ties
+            if (i0 < nsub && sc0 > best_score) { best_score = sc0; best_idx = i0; }
+            if (i1 < nsub && sc1 > best_score) { best_score = sc1; best_idx = i1; }
         }
-        // ascending subset index per lane: strict > keeps the smaller on ties
-        if (i0 < nsub && sc0 > best_score) { best_score = sc0; best_idx = i0; }
-        if (i1 < nsub && sc1 > best_score) { best_score = sc1; best_idx = i1; }
+        // wave argmax (score desc, index asc)
+        for (int d = 32; d >= 1; d >>= 1) {
+            int os = __shfl_xor(best_score, d, 64), oi = __shfl_xor(best_idx, d, 64);
+            if (os > best_score || (os == best_score && oi < best_idx)) { best_score = os; best_idx = oi; }
+        }
+        const uint32_t S = subs[best_idx];
+        if (lane == 0) {
+            const uint64_t key = bk[first] >> 32;
+            blk_seg[b] = (int32_t)(key / (uint64_t)nblk);
+            // 4 column offsets (ascending), 4 bits each
+            uint16_t sel = 0;
+            int k = 0;
+            for (int c = 0; c < M && k < 4; ++c)
+                if (S & (1u << (M - 1 - c))) sel |= (uint16_t)(c << (4 * k++));
+            blk_sel[b] = sel;
+            blk_mask[b] = S;
+            kept += (unsigned long long)best_score;
+        }
+        for (int q = first + lane; q < last; q += 64) run_blk[rid_sorted[q]] = b;
     }
-    // wave argmax (score desc, index asc)
-    for (int d = 32; d >= 1; d >>= 1) {
-        int os = __shfl_xor(best_score, d, 64), oi = __shfl_xor(best_idx, d, 64);
-        if (os > best_score || (os == best_score && oi < best_idx)) { best_score = os; best_idx = oi; }
-    }
-    const uint32_t S = subs[best_idx];
-    if (lane == 0) {
-        const uint64_t key = bk[first] >> 32;
-        blk_seg[b] = (int32_t)(key / (uint64_t)nblk);
-        // 4 column offsets (ascending), 4 bits each
-        uint16_t sel = 0;
-        int k = 0;
-        for (int c = 0; c < M && k < 4; ++c)
-            if (S & (1u << (M - 1 - c))) sel |= (uint16_t)(c << (4 * k++));
-        blk_sel[b] = sel;
-        blk_mask[b] = S;
-        atomicAdd(retained, (unsigned long long)best_score);
-    }
-    for (int q = first + lane; q < last; q += 64) run_blk[rid_sorted[q]] = b;
+    if (lane == 0 && kept) atomicAdd(retained, kept);
 }
 
 __global__ void k_blk_first(const int32_t *__restrict__ head, const int32_t *__restrict__ excl, int U,
@@ -442,7 +448,7 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
     if (nb > 0) {
         hipLaunchKernelGGL(k_blk_first, dim3(grid_of(U, 256)), dim3(256), 0, st, bhead.p, bexcl.p, U,
                            blk_first.p);
-        hipLaunchKernelGGL(k_blk_select, dim3((nb + 3) / 4), dim3(256), 0, st, bkey2.p, rid2.p,
+        hipLaunchKernelGGL(k_blk_select, dim3(std::min((nb + 3) / 4, 4096)), dim3(256), 0, st, bkey2.p, rid2.p,
                            bexcl.p, bhead.p, U, nb, M, nblk, bits.p, dsubs.p, (int)subs.size(),
                            blk_first.p, p->blk_seg.p, p->blk_sel.p, blk_mask.p, run_blk.p, dret.p);
         VNM_HIP_TRY(hipMemsetAsync(absmax.p, 0, (size_t)nb * 4, st));

@@ -109,6 +109,71 @@ __device__ inline void table_pass(uint32_t *tab, int slot0, int M, int R, const 
     }
 }
 
+// ---- row hash in LDS, for segment sets of at most HASH_MAX entries
+//
+// The global slots above cost R bytes per workgroup and one scattered
+// atomic, read and clear per entry in a table far larger than the L2; a set
+// of a few thousand entries touches at most that many rows, which an
+// open-addressing hash in LDS holds instead: key = row, value = the row's
+// bitmaps (byte j = slot j, as in the global slots).  Probes are bounded by
+// the capacity, so every loop ends.
+
+constexpr int HASH_BITS = 12;
+constexpr int HASH_CAP = 1 << HASH_BITS;  // slots (32 KiB of LDS with the values)
+constexpr int HASH_MAX = HASH_CAP / 2;    // entries of a set that uses the hash
+constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
+
+struct RowHash {
+    uint32_t key[HASH_CAP];
+    uint32_t val[HASH_CAP];
+};
+
+__device__ __forceinline__ uint32_t hash_home(uint32_t r) { return (r * 0x9E3779B1u) >> (32 - HASH_BITS); }
+
+// All slots empty; call with every thread, synchronise before use.
+__device__ inline void hash_reset(RowHash &H) {
+    for (int i = threadIdx.x; i < HASH_CAP; i += blockDim.x) {
+        H.key[i] = HASH_EMPTY;
+        H.val[i] = 0u;
+    }
+}
+
+__device__ __forceinline__ void hash_or(RowHash &H, uint32_t r, uint32_t bits) {
+    uint32_t h = hash_home(r);
+    for (int p = 0; p < HASH_CAP; ++p) {
+        const uint32_t k = atomicCAS(&H.key[h], HASH_EMPTY, r);
+        if (k == HASH_EMPTY || k == r) {
+            atomicOr(&H.val[h], bits);
+            return;
+        }
+        h = (h + 1) & (HASH_CAP - 1);
+    }
+}
+
+// Bitmaps of row r (inserted before the last barrier).
+__device__ __forceinline__ uint32_t hash_get(const RowHash &H, uint32_t r) {
+    uint32_t h = hash_home(r);
+    for (int p = 0; p < HASH_CAP; ++p) {
+        const uint32_t k = H.key[h];
+        if (k == r) return H.val[h];
+        if (k == HASH_EMPTY) return 0u;
+        h = (h + 1) & (HASH_CAP - 1);
+    }
+    return 0u;
+}
+
+// Insert the entries of nc columns (slot t / M, bit t % M) into the hash.
+__device__ inline void hash_scatter(RowHash &H, int M, const ColSpan *cols, const int *pre, int nc, int total,
+                                    const int32_t *crow) {
+    for (int e0 = threadIdx.x; e0 < total; e0 += blockDim.x * TAB_U) {
+        int t[TAB_U], r[TAB_U];
+        table_entries(cols, pre, nc, total, crow, e0, blockDim.x, t, r);
+#pragma unroll
+        for (int u = 0; u < TAB_U; ++u)
+            if (t[u] >= 0) hash_or(H, (uint32_t)r[u], (1u << (M - 1 - t[u] % M)) << (8 * (t[u] / M)));
+    }
+}
+
 __device__ inline void table_scatter(uint32_t *tab, int M, int R, const ColSpan *cols, const int *pre,
                                      int nc, int total, const int32_t *crow) {
     table_pass<false>(tab, 0, M, R, cols, pre, nc, total, crow, 0, 1);
