@@ -893,52 +893,121 @@ __global__ void k_lp_gain_keys(int nc, const Cand *__restrict__ cand, const int4
 }
 
 constexpr int SEL_T = 1024, SEL_B = 1024;
+constexpr int SEL_ROUNDS = 24;                 // parallel rounds before the sequential finish
+constexpr int32_t OWNER_NONE = 0x7F7F7F7F;    // empty owner (bytes 0x7F, set by a memset)
 
-// Walk the candidates in (gain desc, index asc) order and accept each whose
-// two segments are both still unused: one thread over an LDS copy of the
-// batch and an LDS bitmap of used segments.  out[0] = accepted, out[1] =
-// their total gain; acc[k] = candidate index.
+// Accept candidates in (gain desc, index asc) order when both segments are
+// still unused: the sequential greedy (reorder/reorder.h's segment-disjoint
+// rule applied to the learned batch), computed in rounds.  In a round every
+// live candidate k writes min(k) into the owner slot of its two segments;
+// k is accepted when it owns both (no live candidate before it shares a
+// segment, so the sequential walk would accept it too), and candidates
+// touching a segment used by now are dropped.  The first live candidate is
+// always accepted, so rounds make progress; after SEL_ROUNDS rounds thread 0
+// finishes the chunk sequentially.  Chunks of SEL_B candidates in order.
+// out[0] = accepted, out[1] = their total gain; acc = candidate indices in
+// order.  owner: nseg ints, OWNER_NONE on entry and on exit.
 __global__ __launch_bounds__(SEL_T) void k_lp_select(int nc, int nseg, const uint64_t *__restrict__ skeys,
                                                      const Cand *__restrict__ cand,
                                                      const int4 *__restrict__ best, int32_t *__restrict__ acc,
-                                                     int32_t *__restrict__ out) {
+                                                     int32_t *__restrict__ out, int32_t *owner) {
     extern __shared__ uint32_t used[];
-    __shared__ uint64_t bk[SEL_B];
-    __shared__ int bp[SEL_B], bq[SEL_B], bg[SEL_B];
+    __shared__ int bp[SEL_B], bq[SEL_B], bg[SEL_B], bc[SEL_B];
+    __shared__ uint8_t live[SEL_B], took[SEL_B];
+    __shared__ int wsum[SEL_T / 64], wg[SEL_T / 64];
     __shared__ int nacc, gsum, stop;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int words = (nseg + 31) / 32;
-    for (int i = threadIdx.x; i < words; i += SEL_T) used[i] = 0;
-    if (threadIdx.x == 0) { nacc = 0; gsum = 0; stop = 0; }
+    for (int i = tid; i < words; i += SEL_T) used[i] = 0;
+    if (tid == 0) { nacc = 0; gsum = 0; stop = 0; }
     __syncthreads();
     for (int base = 0; base < nc && !stop; base += SEL_B) {
         const int n = min(SEL_B, nc - base);
-        for (int k = threadIdx.x; k < n; k += SEL_T) {
+        // one candidate per thread (SEL_T == SEL_B)
+        const int k = tid;
+        bool alive = false;
+        int p = 0, q = 0;
+        if (k < n) {
             const uint64_t key = skeys[base + k];
-            bk[k] = key;
             if (key != ~0ull) {
                 const int ci = (int)(uint32_t)key;
-                bp[k] = cand[ci].p;
-                bq[k] = cand[ci].q;
+                p = cand[ci].p;
+                q = cand[ci].q;
                 const int4 b = best[ci];
+                bp[k] = p;
+                bq[k] = q;
                 bg[k] = b.z + b.w;
+                bc[k] = ci;
+                alive = !((used[p >> 5] >> (p & 31)) & 1u) && !((used[q >> 5] >> (q & 31)) & 1u);
+            } else {
+                stop = 1;  // sorted keys: the rest are not candidates
+            }
+        }
+        bool taken = false;
+        for (int r = 0; r < SEL_ROUNDS; ++r) {
+            if (!__syncthreads_or(alive)) break;
+            if (alive) {
+                atomicMin(&owner[p], k);
+                atomicMin(&owner[q], k);
+            }
+            __threadfence();
+            __syncthreads();
+            bool win = false;
+            if (alive) win = ld_l2(&owner[p]) == k && ld_l2(&owner[q]) == k;
+            __syncthreads();
+            if (alive) {
+                st_l2(&owner[p], OWNER_NONE);
+                st_l2(&owner[q], OWNER_NONE);
+                if (win) {
+                    taken = true;
+                    atomicOr(&used[p >> 5], 1u << (p & 31));
+                    atomicOr(&used[q >> 5], 1u << (q & 31));
+                }
+            }
+            __threadfence();
+            __syncthreads();
+            if (alive) alive = !taken && !((used[p >> 5] >> (p & 31)) & 1u) && !((used[q >> 5] >> (q & 31)) & 1u);
+        }
+        if (k < SEL_B) {
+            live[k] = alive;
+            took[k] = taken;
+        }
+        // candidates left after the rounds: the sequential walk, in order
+        if (__syncthreads_or(alive) && tid == 0) {
+            for (int j = 0; j < n; ++j) {
+                if (!live[j]) continue;
+                const int a = bp[j], b = bq[j];
+                if (((used[a >> 5] >> (a & 31)) & 1u) || ((used[b >> 5] >> (b & 31)) & 1u)) continue;
+                used[a >> 5] |= 1u << (a & 31);
+                used[b >> 5] |= 1u << (b & 31);
+                took[j] = 1;
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int k = 0; k < n; ++k) {
-                if (bk[k] == ~0ull) { stop = 1; break; }
-                const int p = bp[k], q = bq[k];
-                const uint32_t mp = 1u << (p & 31), mq = 1u << (q & 31);
-                if ((used[p >> 5] & mp) || (used[q >> 5] & mq)) continue;
-                used[p >> 5] |= mp;
-                used[q >> 5] |= mq;
-                acc[nacc++] = (int)(uint32_t)bk[k];
-                gsum += bg[k];
+        // accepted candidates in order: wave ballots, then wave offsets
+        const bool t = k < n && took[k];
+        const uint64_t bal = __ballot(t);
+        const int before = __popcll(bal & ((1ull << lane) - 1ull));
+        int g = t ? bg[k] : 0;
+        for (int d = 32; d >= 1; d >>= 1) g += __shfl_xor(g, d, 64);
+        if (lane == 0) {
+            wsum[w] = __popcll(bal);
+            wg[w] = g;
+        }
+        __syncthreads();
+        int off = nacc;
+        for (int v = 0; v < w; ++v) off += wsum[v];
+        if (t) acc[off + before] = bc[k];
+        __syncthreads();
+        if (tid == 0) {
+            for (int v = 0; v < SEL_T / 64; ++v) {
+                nacc += wsum[v];
+                gsum += wg[v];
             }
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { out[0] = nacc; out[1] = gsum; }
+    if (tid == 0) { out[0] = nacc; out[1] = gsum; }
 }
 
 // Apply the accepted swaps (segment-disjoint, so in parallel) and list the
@@ -1324,7 +1393,7 @@ struct LPRound {
     DBuf<int8_t> dbest, abest;
     DBuf<Cand> cand;
     DBuf<int4> best;
-    DBuf<int32_t> acc, out, touched;
+    DBuf<int32_t> acc, out, touched, owner;
 };
 
 // Score candidates [0, nc) exactly, in chunks that fit the table slots.
@@ -1378,8 +1447,9 @@ static int lp_reorder_run(vnm_graph *g, const vnm_lp_model *m, int N, int mode, 
         (rc = w.sakey.alloc(ns)) || (rc = w.dbest.alloc(ns)) || (rc = w.abest.alloc(ns)) ||
         (rc = w.cand.alloc(ncmax)) || (rc = w.best.alloc(ncmax)) || (rc = w.gkey.alloc(ncmax)) ||
         (rc = w.sgkey.alloc(ncmax)) || (rc = w.acc.alloc(ncmax)) || (rc = w.out.alloc(8)) ||
-        (rc = w.touched.alloc(2 * (size_t)ncmax)))
+        (rc = w.touched.alloc(2 * (size_t)ncmax)) || (rc = w.owner.alloc(ns)))
         return rc;
+    VNM_HIP_TRY(hipMemsetAsync(w.owner.p, 0x7F, (size_t)ns * 4, st));  // OWNER_NONE in every slot
     int32_t tot[2];
     VNM_HIP_TRY(hipMemcpyAsync(tot, c->tot.p, 8, hipMemcpyDeviceToHost, st));
     VNM_HIP_TRY(hipStreamSynchronize(st));
@@ -1420,7 +1490,7 @@ static int lp_reorder_run(vnm_graph *g, const vnm_lp_model *m, int N, int mode, 
         if ((rc = sort_keys_u64(w.gkey.p, w.sgkey.p, nc, 64, st))) return rc;
         const size_t shm = (size_t)((ns + 31) / 32) * 4;
         hipLaunchKernelGGL(k_lp_select, dim3(1), dim3(SEL_T), shm, st, nc, ns, w.sgkey.p, w.cand.p, w.best.p,
-                           w.acc.p, w.out.p);
+                           w.acc.p, w.out.p, w.owner.p);
         g->ctx->launches += 2;
         VNM_LAUNCH_CHECK("k_lp_select");
         int32_t o[2];
