@@ -70,7 +70,11 @@ def test_pair_score_rejects_self_pair(ctx):
 
 @pytest.mark.parametrize("NN", [1, 2, 3])
 def test_features_match_oracle(ctx, NN):
-    for m in (vnm.gen_rmat(11, 9000, 4), _rand(np.random.default_rng(NN), 64, 0.15, 8, sym=False)):
+    # R-MAT 65,536 has segments of up to 8,697 entries spanning two row
+    # tiles (the class-count pass's large-segment path); the others fit its
+    # LDS row hash
+    for m in (vnm.gen_rmat(11, 9000, 4), _rand(np.random.default_rng(NN), 64, 0.15, 8, sym=False),
+              vnm.gen_rmat(16, 200000, 4)):
         X = vnm.Graph(ctx, m, M).lp_features(NN)
         ref = oracle.lp_features(_omat(m), NN, M)
         np.testing.assert_allclose(X, ref, rtol=1e-5, atol=1e-5)

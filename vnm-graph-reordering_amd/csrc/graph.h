@@ -76,6 +76,7 @@ struct vnm_graph {
     vnm::DBuf<uint32_t> tab;              // per-slot row bitmaps (bytes in words)
     vnm::DBuf<uint32_t> rt;               // N:M greedy row records A and B (16 B per row each)
     vnm::DBuf<int32_t> gst;               // N:M greedy device state
+    vnm::DBuf<int32_t> queue;             // work counters of the per-segment passes
     // a batch of N:M greedy rounds captured once (its kernels read only the
     // buffers above, which live as long as the graph)
     hipGraph_t nm_graph = nullptr;

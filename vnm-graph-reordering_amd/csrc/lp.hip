@@ -155,6 +155,7 @@ __global__ void k_lp_inv_deg(int R, const int32_t *__restrict__ ptr, float *__re
 // ------------------------------------------------------------ segment statistics
 
 constexpr int SS_T = 256;
+constexpr int TB = SS_T / 8;  // row tiles whose ends one pass of binary searches finds
 
 // For each selected segment (all when segs == nullptr): class counts of the
 // rows of each of its M nodes (ccls[node][4]) and of the segment itself
@@ -162,49 +163,154 @@ constexpr int SS_T = 256;
 __global__ __launch_bounds__(SS_T) void k_lp_seg_stats(
     int R, int M, int N, int nseg, const int32_t *__restrict__ segs, int nsel,
     const int32_t *__restrict__ inv, const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
-    uint32_t *tab, int32_t *__restrict__ ccls, int32_t *__restrict__ segcls,
-    int32_t *__restrict__ segnnz) {
+    int32_t *__restrict__ ccls, int32_t *__restrict__ segcls, int32_t *__restrict__ segnnz,
+    int32_t *__restrict__ queue) {
     __shared__ ColSpan cols[16];
     __shared__ int pre[17];
-    __shared__ int sel[1];
+    __shared__ int sel[1], cur;
     __shared__ int colc[8][4];
     __shared__ int segc[4];
-    __shared__ RowHash H;
-    uint32_t *slot = tab + (size_t)blockIdx.x * (R >> 2);
+    __shared__ int tlo[8], thi[8], tpre[9];
+    __shared__ int tend[8][TB];
+    // the row bitmaps of one segment: an LDS hash (at most HASH_MAX entries)
+    // or, for larger segments, LDS byte tiles of TILE_ROWS rows
+    __shared__ union {
+        RowHash h;
+        uint32_t tile[TILE_ROWS / 4];
+    } L;
     const int n = segs ? nsel : nseg;
     const int tid = threadIdx.x;
-    hash_reset(H);
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    hash_reset(L.h);
+    // segments from a work counter (*queue zero at launch): segment sizes are
+    // heavy-tailed and cluster by index, so a fixed assignment leaves one
+    // workgroup with many of the largest
+    for (;;) {
+        if (tid == 0) cur = atomicAdd(queue, 1);
+        __syncthreads();
+        const int i = cur;
+        if (i >= n) break;
         if (tid == 0) sel[0] = segs ? segs[i] : i;
         if (tid < 32) colc[tid >> 2][tid & 3] = 0;
         if (tid < 4) segc[tid] = 0;
         __syncthreads();
         const int total = table_columns(sel, 1, M, inv, ptr, cols, pre);
-        // row bitmaps: the LDS hash for small segments, the global slot else
-        const bool small = total <= HASH_MAX;
-        if (small) hash_scatter(H, M, cols, pre, M, total, idx);
-        else table_scatter(slot, M, R, cols, pre, M, total, idx);
-        __syncthreads();
-        // TAB_U entries per thread resolved before their loads are issued
-        for (int e0 = tid; e0 < total; e0 += SS_T * TAB_U) {
-            int t[TAB_U], r[TAB_U];
-            uint32_t bits[TAB_U];
-            table_entries(cols, pre, M, total, idx, e0, SS_T, t, r);
+        auto count = [&](int t, uint32_t bits) {
+            const int pc = popc32(bits);
+            const int k = pc < N ? 0 : pc == N ? 1 : pc == N + 1 ? 2 : 3;
+            atomicAdd(&colc[t][k], 1);
+            if ((bits >> (M - 1 - t)) == 1u) atomicAdd(&segc[k], 1);
+        };
+        if (total <= HASH_MAX) {
+            hash_scatter(L.h, M, cols, pre, M, total, idx);
+            __syncthreads();
+            // TAB_U entries per thread resolved before their loads are issued
+            for (int e0 = tid; e0 < total; e0 += SS_T * TAB_U) {
+                int t[TAB_U], r[TAB_U];
+                table_entries(cols, pre, M, total, idx, e0, SS_T, t, r);
 #pragma unroll
-            for (int u = 0; u < TAB_U; ++u)
-                bits[u] = t[u] < 0 ? 0u : small ? (hash_get(H, (uint32_t)r[u]) & 0xFFu) : slot_byte(slot, 0, R, r[u]);
-#pragma unroll
-            for (int u = 0; u < TAB_U; ++u) {
-                if (t[u] < 0) continue;
-                const int pc = popc32(bits[u]);
-                const int k = pc < N ? 0 : pc == N ? 1 : pc == N + 1 ? 2 : 3;
-                atomicAdd(&colc[t[u]][k], 1);
-                if ((bits[u] >> (M - 1 - t[u])) == 1u) atomicAdd(&segc[k], 1);
+                for (int u = 0; u < TAB_U; ++u)
+                    if (t[u] >= 0) count(t[u], hash_get(L.h, (uint32_t)r[u]) & 0xFFu);
             }
+            __syncthreads();
+            hash_reset(L.h);
+        } else {
+            // rows [r0, r0 + TILE_ROWS) at a time in an LDS byte tile: the
+            // column lists are sorted by row, so each column's entries of a
+            // tile are one range, and the tile ends of TB tiles are found by
+            // M * TB binary searches at once
+            for (int w = tid; w < TILE_ROWS / 4; w += SS_T) L.tile[w] = 0u;
+            if (tid < M) tlo[tid] = cols[tid].start;
+            int cc[8][4], sc[4];  // this thread's class counts (columns, segment)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                sc[k] = 0;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) cc[t][k] = 0;
+            }
+            const int ntile = (R + TILE_ROWS - 1) / TILE_ROWS;
+            for (int g0 = 0; g0 < ntile; g0 += TB) {
+                if (tid < M * TB) {  // end of tile g0 + j in column t
+                    const int t = tid / TB, j = tid % TB;
+                    const int64_t r1 = min((int64_t)R, (int64_t)(g0 + j + 1) * TILE_ROWS);
+                    int lo = cols[t].start, hi = cols[t].start + cols[t].len;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (idx[mid] < r1) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    tend[t][j] = lo;
+                }
+                __syncthreads();
+                for (int j = 0; j < TB && g0 + j < ntile; ++j) {
+                    const int r0 = (g0 + j) * TILE_ROWS;
+                    if (tid == 0) {
+                        tpre[0] = 0;
+                        for (int t = 0; t < M; ++t) {
+                            thi[t] = tend[t][j];
+                            tpre[t + 1] = tpre[t] + thi[t] - tlo[t];
+                        }
+                    }
+                    __syncthreads();
+                    const int tt = tpre[M];
+                    if (tt > 0) {  // uniform
+                        for (int e0 = tid; e0 < tt; e0 += SS_T * TAB_U) {
+                            int t[TAB_U], r[TAB_U];
+#pragma unroll
+                            for (int u = 0; u < TAB_U; ++u) {
+                                const int e = e0 + u * SS_T;
+                                t[u] = e < tt ? entry_col(tpre, M, e) : -1;
+                            }
+#pragma unroll
+                            for (int u = 0; u < TAB_U; ++u)
+                                r[u] = t[u] >= 0 ? idx[tlo[t[u]] + (e0 + u * SS_T - tpre[t[u]])] - r0 : 0;
+#pragma unroll
+                            for (int u = 0; u < TAB_U; ++u)
+                                if (t[u] >= 0) atomicOr(&L.tile[r[u] >> 2], (1u << (M - 1 - t[u])) << (8 * (r[u] & 3)));
+                        }
+                        __syncthreads();
+                        // every row of the tile once: its bitmap's class,
+                        // counted for the segment and for each of its columns
+                        for (int w = tid; w < TILE_ROWS / 4; w += SS_T) {
+                            const uint32_t word = L.tile[w];
+                            if (!word) continue;
+                            L.tile[w] = 0u;
+#pragma unroll
+                            for (int y = 0; y < 4; ++y) {
+                                const uint32_t bits = (word >> (8 * y)) & 0xFFu;
+                                if (!bits) continue;
+                                const int pc = popc32(bits);
+                                const int kk = pc < N ? 0 : pc == N ? 1 : pc == N + 1 ? 2 : 3;
+#pragma unroll
+                                for (int k = 0; k < 4; ++k) {
+                                    const int is = kk == k;
+                                    sc[k] += is;
+#pragma unroll
+                                    for (int t = 0; t < 8; ++t)
+                                        if (t < M) cc[t][k] += is & (int)((bits >> (M - 1 - t)) & 1u);
+                                }
+                            }
+                        }
+                    }
+                    if (tid < M) tlo[tid] = thi[tid];
+                    __syncthreads();
+                }
+            }
+            // the threads' counts into the segment's counters
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                int v = sc[k];
+                for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+                if ((tid & 63) == 0 && v) atomicAdd(&segc[k], v);
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    int c = cc[t][k];
+                    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+                    if ((tid & 63) == 0 && c) atomicAdd(&colc[t][k], c);
+                }
+            }
+            __syncthreads();
+            hash_reset(L.h);
         }
-        __syncthreads();
-        if (small) hash_reset(H);
-        else table_clear(slot, M, R, cols, pre, M, total, idx);
         const int s = sel[0];
         if (tid < M * 4) {
             const int node = ld_l2(&inv[s * M + (tid >> 2)]);
@@ -1163,11 +1269,12 @@ static int lp_stats(vnm_graph *g, LPCache *c, int N, const int32_t *segs, int ns
     const int n = segs ? nsel : c->nseg;
     if (n == 0) return VNM_OK;
     const int grid = std::max(1, std::min(n, 1024));
-    int rc = ensure_tab(g, grid);
-    if (rc) return rc;
     hipStream_t st = g->ctx->stream;
+    int rc;
+    if (!g->queue.p && (rc = g->queue.alloc(4))) return rc;
+    VNM_HIP_TRY(hipMemsetAsync(g->queue.p, 0, 4, st));
     hipLaunchKernelGGL(k_lp_seg_stats, dim3(grid), dim3(SS_T), 0, st, g->nrows, g->M, N, c->nseg, segs, nsel,
-                       g->inv.p, c->ptr.p, c->idx.p, g->tab.p, c->ccls.p, c->segcls.p, c->segnnz.p);
+                       g->inv.p, c->ptr.p, c->idx.p, c->ccls.p, c->segcls.p, c->segnnz.p, g->queue.p);
     hipLaunchKernelGGL(k_lp_totals, dim3(1), dim3(1024), 0, st, c->nseg, c->segcls.p, c->tot.p);
     g->ctx->launches += 2;
     VNM_LAUNCH_CHECK("k_lp_seg_stats");

@@ -135,15 +135,21 @@ constexpr int LOC_T = 256;
 __global__ __launch_bounds__(LOC_T) void k_locate_all(
     int R, int M, int N, int nseg, const int32_t *__restrict__ inv,
     const int32_t *__restrict__ cptr, const int32_t *__restrict__ crow, uint32_t *tab,
-    int32_t *__restrict__ cnt, int32_t *__restrict__ nzr) {
+    int32_t *__restrict__ cnt, int32_t *__restrict__ nzr, int32_t *__restrict__ queue) {
     __shared__ ColSpan cols[16];
     __shared__ int pre[17];
-    __shared__ int sel[1];
+    __shared__ int sel[1], cur;
     __shared__ int c_inv, c_nz;
     __shared__ RowHash H;
     uint32_t *slot = tab + (size_t)blockIdx.x * (R >> 2);
     hash_reset(H);
-    for (int s = blockIdx.x; s < nseg; s += gridDim.x) {
+    // segments from a work counter (*queue zero at launch): sizes are
+    // heavy-tailed and cluster by index
+    for (;;) {
+        if (threadIdx.x == 0) cur = atomicAdd(queue, 1);
+        __syncthreads();
+        const int s = cur;
+        if (s >= nseg) break;
         if (threadIdx.x == 0) { sel[0] = s; c_inv = 0; c_nz = 0; }
         __syncthreads();
         int total = table_columns(sel, 1, M, inv, cptr, cols, pre);
@@ -1305,9 +1311,11 @@ int locate_all(vnm_graph *g, int N, int32_t *d_cnt, int32_t *d_nzr) {
     int grid = locate_grid(g);
     int rc = ensure_tab(g, std::max<size_t>(grid, KMAX + 1));
     if (rc) return rc;
+    if (!g->queue.p && (rc = g->queue.alloc(4))) return rc;
+    VNM_HIP_TRY(hipMemsetAsync(g->queue.p, 0, 4, g->ctx->stream));
     KernelTimer kt(g->ctx, "k_locate_all");
     hipLaunchKernelGGL(k_locate_all, dim3(grid), dim3(LOC_T), 0, g->ctx->stream, g->nrows, g->M, N,
-                       g->nseg(), g->inv.p, g->cptr.p, g->crow.p, g->tab.p, d_cnt, d_nzr);
+                       g->nseg(), g->inv.p, g->cptr.p, g->crow.p, g->tab.p, d_cnt, d_nzr, g->queue.p);
     kt.stop();
     g->ctx->launches++;
     VNM_LAUNCH_CHECK("k_locate_all");

@@ -122,6 +122,7 @@ constexpr int HASH_BITS = 12;
 constexpr int HASH_CAP = 1 << HASH_BITS;  // slots (32 KiB of LDS with the values)
 constexpr int HASH_MAX = HASH_CAP / 2;    // entries of a set that uses the hash
 constexpr uint32_t HASH_EMPTY = 0xFFFFFFFFu;
+constexpr int TILE_ROWS = HASH_CAP * 8;    // rows of an LDS byte tile in the hash's storage
 
 struct RowHash {
     uint32_t key[HASH_CAP];
