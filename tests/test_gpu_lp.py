@@ -48,11 +48,16 @@ def test_pair_score_matches_swap_score_and_oracle(ctx):
     rng = np.random.default_rng(3)
     cases = [(vnm.gen_rmat(12, 30000, 7), 8, 2), (vnm.gen_er(2048, 41943, 1), 8, 2),
              (_rand(rng, 96, 0.2, 8), 8, 3), (vnm.gen_rmat(11, 12000, 8, M=4), 4, 1)]
+    cases.append((vnm.gen_rmat(16, 200000, 9), 8, 2))
     for m, MM, NN in cases:
         g = vnm.Graph(ctx, m, MM)
         ns = g.nseg
         p = rng.integers(0, ns, 40).astype(np.int32)
+        # the largest segments too: pairs beyond the scorer's LDS hash
+        sizes = np.bincount(m.col // MM, minlength=ns)
+        p[:12] = np.argsort(-sizes, kind="stable")[:12]
         q = (p + rng.integers(1, ns, 40)) % ns
+        q[:4] = p[4:8]
         got = g.pair_score(NN, p, q)
         om = _omat(m)
         for k in range(40):

@@ -23,8 +23,9 @@
 //
 // Kernels:
 //   k_lp_adj_keys/unpack : deterministic (sorted) adjacency and its transpose
-//   k_lp_seg_stats       : per-segment table pass -> class counts per node
-//                          and per segment (whole graph or touched segments)
+//   k_lp_seg_stats       : class counts per node and per segment (whole graph
+//                          or touched segments) from an LDS row hash or LDS
+//                          row tiles, segments taken from a work counter
 //   k_lp_features        : 16 input features per node
 //   k_lp_agg_*           : neighbour mean (and its transpose for backward),
 //                          one wave per node, long lists split into chunks
@@ -36,6 +37,7 @@
 //   k_lp_pair_gain       : exact M x M gain matrix of a segment pair from the
 //                          entries of its two segments (ballot bit planes)
 //   k_lp_select/swap     : segment-disjoint greedy over positive exact gains
+//                          (parallel rounds, the sequential walk's choice)
 #include "graph.h"
 #include "hip_common.h"
 #include "sortlib.h"
@@ -156,6 +158,8 @@ __global__ void k_lp_inv_deg(int R, const int32_t *__restrict__ ptr, float *__re
 
 constexpr int SS_T = 256;
 constexpr int TB = SS_T / 8;  // row tiles whose ends one pass of binary searches finds
+constexpr int PB = 256 / 16;   // the same for the pair scorer (2M <= 16 columns)
+constexpr int PAIR_TILE_ROWS = TILE_ROWS / 2;  // rows of a pair tile (16 bits per row)
 
 // For each selected segment (all when segs == nullptr): class counts of the
 // rows of each of its M nodes (ccls[node][4]) and of the segment itself
@@ -826,32 +830,8 @@ __global__ void k_lp_match(int nd, int W, int nseg, int stride, int M, int mode,
 
 // ------------------------------------------------------------ exact pair scorer
 
-// Fill (or clear) two slots per candidate: slot 2i = segment p, 2i+1 = q.
-__global__ __launch_bounds__(256) void k_lp_fill_pairs(int R, int M, const Cand *__restrict__ cand,
-                                                       const int32_t *__restrict__ inv,
-                                                       const int32_t *__restrict__ ptr,
-                                                       const int32_t *__restrict__ idx, uint32_t *tab,
-                                                       int clear) {
-    __shared__ ColSpan cols[16];
-    __shared__ int pre[17];
-    __shared__ int sel[2];
-    const Cand c = cand[blockIdx.x >> 1];
-    const int seg = (blockIdx.x & 1) ? c.q : c.p;
-    if (c.q < 0) return;  // uniform per workgroup
-    if (threadIdx.x == 0) { sel[0] = c.p; sel[1] = c.q; }
-    __syncthreads();
-    // pairs of at most HASH_MAX entries are scored from an LDS hash
-    if (table_columns(sel, 2, M, inv, ptr, cols, pre) <= HASH_MAX) return;
-    if (threadIdx.x == 0) sel[0] = seg;
-    __syncthreads();
-    const int total = table_columns(sel, 1, M, inv, ptr, cols, pre);
-    uint32_t *slot = tab + (size_t)blockIdx.x * (R >> 2);
-    if (clear) table_clear(slot, M, R, cols, pre, M, total, idx);
-    else table_scatter(slot, M, R, cols, pre, M, total, idx);
-}
-
 // Exact M x M gains of candidate i's segment pair (p, q) from the entries of
-// the two segments, with the slots of k_lp_fill_pairs:
+// the two segments:
 //   lgain[b1][b2] = E_p[b1] - Bl[b1][b2],  rgain[b1][b2] = E_q[b2] - Br[b1][b2]
 //   E_s[b] = rows of s with popcount N+1 and bit b set
 //   Bl = sum over rows in both of y_b2 * (x_b1 ? [pl = N+1] : [pl = N])
@@ -864,13 +844,21 @@ __global__ __launch_bounds__(256) void k_lp_fill_pairs(int R, int M, const Cand 
 __global__ __launch_bounds__(256) void k_lp_pair_gain(int R, int M, int N, const Cand *__restrict__ cand,
                                                       const int32_t *__restrict__ inv,
                                                       const int32_t *__restrict__ ptr,
-                                                      const int32_t *__restrict__ idx, uint32_t *tab,
+                                                      const int32_t *__restrict__ idx,
                                                       int4 *__restrict__ best, int32_t *__restrict__ full) {
     __shared__ ColSpan cols[16];
     __shared__ int pre[17];
     __shared__ int sel[2];
     __shared__ int sBl[64], sBr[64], sEp[8], sEq[8];
-    __shared__ RowHash H;
+    __shared__ int tlo[16], thi[16], tpre[17];
+    __shared__ int tend[16][PB];
+    // row bitmaps of the pair: an LDS hash (byte 0 = p, byte 1 = q) for at
+    // most HASH_MAX entries, else LDS tiles of PAIR_TILE_ROWS rows, 16 bits
+    // per row
+    __shared__ union {
+        RowHash h;
+        uint32_t tile[PAIR_TILE_ROWS / 2];
+    } L;
     const int i = blockIdx.x;
     const Cand c = cand[i];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -883,76 +871,132 @@ __global__ __launch_bounds__(256) void k_lp_pair_gain(int R, int M, int N, const
     if (tid < 64) { sBl[tid] = 0; sBr[tid] = 0; }
     if (tid < 8) { sEp[tid] = 0; sEq[tid] = 0; }
     __syncthreads();
-    hash_reset(H);
     const int total = table_columns(sel, 2, M, inv, ptr, cols, pre);
-    uint32_t *sp = tab + (size_t)(2 * i) * (R >> 2);
-    uint32_t *sq = tab + (size_t)(2 * i + 1) * (R >> 2);
-    // small pairs: both segments' row bitmaps in the LDS hash (byte 0 = p,
-    // byte 1 = q); larger ones in the slots k_lp_fill_pairs filled
-    const bool small = total <= HASH_MAX;
-    if (small) {
-        hash_scatter(H, M, cols, pre, 2 * M, total, idx);
-        __syncthreads();
-    }
     const int b1 = lane / M, b2 = lane % M;
     int aBl = 0, aBr = 0, aEp = 0, aEq = 0;
-    // TAB_U entries per thread resolved and their bitmaps loaded before any
-    // is folded in (the loads are independent, so they overlap)
-    for (int base = 0; base < total; base += 256 * TAB_U) {
-        int tt[TAB_U], rw[TAB_U];
-        uint32_t lv[TAB_U], rv[TAB_U];
-        table_entries(cols, pre, 2 * M, total, idx, base + tid, 256, tt, rw);
-#pragma unroll
-        for (int u = 0; u < TAB_U; ++u) {
-            lv[u] = 0u;
-            rv[u] = 0u;
-            if (tt[u] < 0) continue;
-            if (small) {
-                const uint32_t h = hash_get(H, (uint32_t)rw[u]);
-                lv[u] = h & 0xFFu;
-                rv[u] = (h >> 8) & 0xFFu;
+    // one entry per lane (t = its column, -1 for none; lv / rv = its row's
+    // bitmaps in p and q), folded into the gain sums with wave ballots; the
+    // sums count each row once, through its owner (leading-bit) entries
+    auto fold = [&](int t, uint32_t lv, uint32_t rv) {
+        bool isP = false, isQ = false;
+        uint32_t l = 0, rr = 0;
+        if (t >= 0) {
+            if (t < M) {
+                l = lv;
+                isP = (l >> (M - 1 - t)) == 1u;  // the row's owner entry in p
+                if (isP) rr = rv;
             } else {
-                if (tt[u] < M) lv[u] = slot_byte(sp, 0, R, rw[u]);
-                rv[u] = slot_byte(sq, 0, R, rw[u]);
+                rr = rv;
+                isQ = (rr >> (M - 1 - (t - M))) == 1u;
             }
         }
+        const int pl = popc32(l), pr = popc32(rr);
+        const bool both = isP && rr != 0;
+        const uint64_t Ap = __ballot(isP && pl == N + 1);
+        const uint64_t Dq = __ballot(isQ && pr == N + 1);
+        const uint64_t A = __ballot(both && pl == N + 1), B = __ballot(both && pl == N);
+        const uint64_t C = __ballot(both && pr == N), D = __ballot(both && pr == N + 1);
+        uint64_t Lb = 0, Rb = 0, Lown = 0, Rown = 0;
+        for (int b = 0; b < M; ++b) {
+            const uint64_t lm = __ballot(isP && ((l >> (M - 1 - b)) & 1u));
+            const uint64_t rb = __ballot(both && ((rr >> (M - 1 - b)) & 1u));
+            const uint64_t rq = __ballot(isQ && ((rr >> (M - 1 - b)) & 1u));
+            if (b == b1) Lb = lm;
+            if (b == b2) Rb = rb;
+            if (b == lane) { Lown = lm; Rown = rq; }
+        }
+        if (lane < MM && (A | B | C | D)) {
+            aBl += __popcll(Rb & ((Lb & A) | (~Lb & B)));
+            aBr += __popcll(Lb & ((Rb & D) | (~Rb & C)));
+        }
+        if (lane < M) {
+            aEp += __popcll(Lown & Ap);
+            aEq += __popcll(Rown & Dq);
+        }
+    };
+    if (total <= HASH_MAX) {
+        hash_reset(L.h);
+        __syncthreads();
+        hash_scatter(L.h, M, cols, pre, 2 * M, total, idx);
+        __syncthreads();
+        // TAB_U entries per thread resolved before their lookups
+        for (int base = 0; base < total; base += 256 * TAB_U) {
+            int tt[TAB_U], rw[TAB_U];
+            table_entries(cols, pre, 2 * M, total, idx, base + tid, 256, tt, rw);
 #pragma unroll
-        for (int u = 0; u < TAB_U; ++u) {
-            const int t = tt[u];
-            bool isP = false, isQ = false;
-            uint32_t l = 0, rr = 0;
-            if (t >= 0) {
-                if (t < M) {
-                    l = lv[u];
-                    isP = (l >> (M - 1 - t)) == 1u;  // the row's owner entry in p
-                    if (isP) rr = rv[u];
-                } else {
-                    rr = rv[u];
-                    isQ = (rr >> (M - 1 - (t - M))) == 1u;
+            for (int u = 0; u < TAB_U; ++u) {
+                const uint32_t h = tt[u] >= 0 ? hash_get(L.h, (uint32_t)rw[u]) : 0u;
+                fold(tt[u], h & 0xFFu, (h >> 8) & 0xFFu);
+            }
+        }
+    } else {
+        // rows [r0, r0 + PAIR_TILE_ROWS) at a time: each column's entries of
+        // a tile are one range of its row-sorted list; the ends of PB tiles
+        // of all 2M columns come from one round of binary searches
+        const int nc = 2 * M;
+        for (int w = tid; w < PAIR_TILE_ROWS / 2; w += 256) L.tile[w] = 0u;
+        if (tid < nc) tlo[tid] = cols[tid].start;
+        const int ntile = (R + PAIR_TILE_ROWS - 1) / PAIR_TILE_ROWS;
+        for (int g0 = 0; g0 < ntile; g0 += PB) {
+            if (tid < nc * PB) {
+                const int t = tid / PB, j = tid % PB;
+                const int64_t r1 = min((int64_t)R, (int64_t)(g0 + j + 1) * PAIR_TILE_ROWS);
+                int lo = cols[t].start, hi = cols[t].start + cols[t].len;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (idx[mid] < r1) lo = mid + 1;
+                    else hi = mid;
                 }
+                tend[t][j] = lo;
             }
-            const int pl = popc32(l), pr = popc32(rr);
-            const bool both = isP && rr != 0;
-            const uint64_t Ap = __ballot(isP && pl == N + 1);
-            const uint64_t Dq = __ballot(isQ && pr == N + 1);
-            const uint64_t A = __ballot(both && pl == N + 1), B = __ballot(both && pl == N);
-            const uint64_t C = __ballot(both && pr == N), D = __ballot(both && pr == N + 1);
-            uint64_t Lb = 0, Rb = 0, Lown = 0, Rown = 0;
-            for (int b = 0; b < M; ++b) {
-                const uint64_t lm = __ballot(isP && ((l >> (M - 1 - b)) & 1u));
-                const uint64_t rb = __ballot(both && ((rr >> (M - 1 - b)) & 1u));
-                const uint64_t rq = __ballot(isQ && ((rr >> (M - 1 - b)) & 1u));
-                if (b == b1) Lb = lm;
-                if (b == b2) Rb = rb;
-                if (b == lane) { Lown = lm; Rown = rq; }
-            }
-            if (lane < MM && (A | B | C | D)) {
-                aBl += __popcll(Rb & ((Lb & A) | (~Lb & B)));
-                aBr += __popcll(Lb & ((Rb & D) | (~Rb & C)));
-            }
-            if (lane < M) {
-                aEp += __popcll(Lown & Ap);
-                aEq += __popcll(Rown & Dq);
+            __syncthreads();
+            for (int j = 0; j < PB && g0 + j < ntile; ++j) {
+                const int r0 = (g0 + j) * PAIR_TILE_ROWS;
+                if (tid == 0) {
+                    tpre[0] = 0;
+                    for (int t = 0; t < nc; ++t) {
+                        thi[t] = tend[t][j];
+                        tpre[t + 1] = tpre[t] + thi[t] - tlo[t];
+                    }
+                }
+                __syncthreads();
+                const int tt = tpre[nc];
+                if (tt > 0) {  // uniform
+                    // TAB_U entries per thread resolved before their loads
+                    auto entries = [&](int e0, int (&t)[TAB_U], int (&r)[TAB_U]) {
+#pragma unroll
+                        for (int u = 0; u < TAB_U; ++u) {
+                            const int e = e0 + u * 256;
+                            t[u] = e < tt ? entry_col(tpre, nc, e) : -1;
+                        }
+#pragma unroll
+                        for (int u = 0; u < TAB_U; ++u)
+                            r[u] = t[u] >= 0 ? idx[tlo[t[u]] + (e0 + u * 256 - tpre[t[u]])] - r0 : 0;
+                    };
+                    for (int e0 = tid; e0 < tt; e0 += 256 * TAB_U) {
+                        int t[TAB_U], r[TAB_U];
+                        entries(e0, t, r);
+#pragma unroll
+                        for (int u = 0; u < TAB_U; ++u)
+                            if (t[u] >= 0)
+                                atomicOr(&L.tile[r[u] >> 1],
+                                         ((1u << (M - 1 - t[u] % M)) << (8 * (t[u] / M))) << (16 * (r[u] & 1)));
+                    }
+                    __syncthreads();
+                    for (int base = 0; base < tt; base += 256 * TAB_U) {
+                        int t[TAB_U], r[TAB_U];
+                        entries(base + tid, t, r);
+#pragma unroll
+                        for (int u = 0; u < TAB_U; ++u) {
+                            const uint32_t h = t[u] >= 0 ? (L.tile[r[u] >> 1] >> (16 * (r[u] & 1))) & 0xFFFFu : 0u;
+                            fold(t[u], h & 0xFFu, h >> 8);
+                        }
+                    }
+                    __syncthreads();
+                    for (int w = tid; w < PAIR_TILE_ROWS / 2; w += 256) L.tile[w] = 0u;
+                }
+                if (tid < nc) tlo[tid] = thi[tid];
+                __syncthreads();
             }
         }
     }
@@ -1503,30 +1547,15 @@ struct LPRound {
     DBuf<int32_t> acc, out, touched, owner;
 };
 
-// Score candidates [0, nc) exactly, in chunks that fit the table slots.
+// Score candidates [0, nc) exactly: a workgroup per candidate.
 static int lp_score_cands(vnm_graph *g, LPCache *c, int N, Cand *cand, int4 *best, int nc, int32_t *full) {
-    hipStream_t st = g->ctx->stream;
-    const int R = g->nrows;
-    // two slots per candidate, at most ~1 GiB of slots at a time
-    const size_t slot_bytes = (size_t)R;
-    int chunk = (int)std::max<size_t>(1, std::min<size_t>(4096, ((size_t)1 << 30) / (2 * slot_bytes)));
-    int rc = ensure_tab(g, (size_t)2 * std::min(chunk, std::max(nc, 1)));
-    if (rc) return rc;
-    chunk = std::min<int>(chunk, (int)(g->tab_slots / 2));
-    for (int base = 0; base < nc; base += chunk) {
-        const int n = std::min(chunk, nc - base);
-        hipLaunchKernelGGL(k_lp_fill_pairs, dim3(2 * n), dim3(256), 0, st, R, g->M, cand + base, g->inv.p,
-                           c->ptr.p, c->idx.p, g->tab.p, 0);
-        KernelTimer kt(g->ctx, "k_lp_pair_gain");
-        hipLaunchKernelGGL(k_lp_pair_gain, dim3(n), dim3(256), 0, st, R, g->M, N, cand + base, g->inv.p,
-                           c->ptr.p, c->idx.p, g->tab.p, best + base,
-                           full ? full + (size_t)base * 2 * g->M * g->M : (int32_t *)nullptr);
-        kt.stop();
-        hipLaunchKernelGGL(k_lp_fill_pairs, dim3(2 * n), dim3(256), 0, st, R, g->M, cand + base, g->inv.p,
-                           c->ptr.p, c->idx.p, g->tab.p, 1);
-        g->ctx->launches += 3;
-        VNM_LAUNCH_CHECK("k_lp_pair_gain");
-    }
+    if (nc <= 0) return VNM_OK;
+    KernelTimer kt(g->ctx, "k_lp_pair_gain");
+    hipLaunchKernelGGL(k_lp_pair_gain, dim3(nc), dim3(256), 0, g->ctx->stream, g->nrows, g->M, N, cand,
+                       g->inv.p, c->ptr.p, c->idx.p, best, full);
+    kt.stop();
+    g->ctx->launches++;
+    VNM_LAUNCH_CHECK("k_lp_pair_gain");
     return VNM_OK;
 }
 
