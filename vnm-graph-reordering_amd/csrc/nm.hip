@@ -362,8 +362,9 @@ struct GreedyState {
 };
 
 // Arguments of the round kernels for one graph: pointers fixed for the
-// graph.  The kernels take an array of them in device memory and serve graph
-// blockIdx.y, so one launch chain runs the greedy of many graphs at once.
+// graph.  The kernels take graph 0's by value and an array of all of them in
+// device memory, and serve graph blockIdx.y, so one launch chain runs the
+// greedy of many graphs at once (a single graph reads no argument memory).
 struct NMArgs {
     GreedyState *st;
     const int32_t *cptr, *crow;
@@ -593,8 +594,8 @@ __device__ __forceinline__ void clock_add(GreedyState *st, int i, uint64_t t0) {
 
 // Records B of the last round's recount are cleared, records A of this
 // round's segments filled.
-__global__ __launch_bounds__(PT) void k_gr_fill_a(const NMArgs *__restrict__ args) {
-    const NMArgs A = args[blockIdx.y];
+__global__ __launch_bounds__(PT) void k_gr_fill_a(NMArgs a0, const NMArgs *__restrict__ args) {
+    const NMArgs A = blockIdx.y == 0 ? a0 : args[blockIdx.y];
     __shared__ ColSet sa, sb;
     __shared__ int done;
     GreedyState *st = A.st;
@@ -613,8 +614,8 @@ __global__ __launch_bounds__(PT) void k_gr_fill_a(const NMArgs *__restrict__ arg
 
 // Records A of this round are cleared, records B filled with the columns
 // after the round's swaps.
-__global__ __launch_bounds__(PT) void k_gr_fill_b(const NMArgs *__restrict__ args) {
-    const NMArgs A = args[blockIdx.y];
+__global__ __launch_bounds__(PT) void k_gr_fill_b(NMArgs a0, const NMArgs *__restrict__ args) {
+    const NMArgs A = blockIdx.y == 0 ? a0 : args[blockIdx.y];
     __shared__ ColSet sa, sb;
     __shared__ int done;
     GreedyState *st = A.st;
@@ -633,8 +634,8 @@ __global__ __launch_bounds__(PT) void k_gr_fill_b(const NMArgs *__restrict__ arg
 
 // After a run: both record sets back to zero (zeroing a clean record is
 // harmless, so this needs no knowledge of where the run stopped).
-__global__ __launch_bounds__(PT) void k_gr_cleanup(const NMArgs *__restrict__ args) {
-    const NMArgs A = args[blockIdx.y];
+__global__ __launch_bounds__(PT) void k_gr_cleanup(NMArgs a0, const NMArgs *__restrict__ args) {
+    const NMArgs A = blockIdx.y == 0 ? a0 : args[blockIdx.y];
     __shared__ ColSet sa, sb;
     GreedyState *st = A.st;
     copy_words(&sa, &st->S, sizeof(ColSet));
@@ -651,8 +652,8 @@ __global__ __launch_bounds__(PT) void k_gr_cleanup(const NMArgs *__restrict__ ar
 // the row's leading bit, one per row) are folded in with ballots, so each
 // (b1, b2) counter takes one LDS atomic per chunk; each workgroup adds its
 // nonzero sums to the device accumulators.
-__global__ __launch_bounds__(PT) void k_gr_score(const NMArgs *__restrict__ args) {
-    const NMArgs A = args[blockIdx.y];
+__global__ __launch_bounds__(PT) void k_gr_score(NMArgs a0, const NMArgs *__restrict__ args) {
+    const NMArgs A = blockIdx.y == 0 ? a0 : args[blockIdx.y];
     __shared__ ColSet sa;
     __shared__ int E[KMAX][MMAX];
     __shared__ int both[NPAIR_T][2][MMAX * MMAX];
@@ -737,8 +738,8 @@ __global__ __launch_bounds__(PT) void k_gr_score(const NMArgs *__restrict__ args
 
 // cnt / nzr of the round's segments after the swaps, from records B
 // (locate_invalid_segment on the touched segments).
-__global__ __launch_bounds__(PT) void k_gr_count(const NMArgs *__restrict__ args) {
-    const NMArgs A = args[blockIdx.y];
+__global__ __launch_bounds__(PT) void k_gr_count(NMArgs a0, const NMArgs *__restrict__ args) {
+    const NMArgs A = blockIdx.y == 0 ? a0 : args[blockIdx.y];
     __shared__ ColSet sb;
     __shared__ int cnt[KMAX], nzr[KMAX];
     __shared__ int done;
@@ -913,8 +914,8 @@ __device__ __forceinline__ int greedy_apply(GreedyShared &S, int M, bool rev, bo
 // the run goes on (reorder/reorder.h:1819-1826: rounds <= max(maxiter, list
 // size), a list of at least one (two for schedules >= 8) segments), choose
 // the round's segments and their columns.
-__global__ __launch_bounds__(GT) void k_gr_select(const NMArgs *__restrict__ args) {
-    const NMArgs A = args[blockIdx.y];
+__global__ __launch_bounds__(GT) void k_gr_select(NMArgs a0, const NMArgs *__restrict__ args) {
+    const NMArgs A = blockIdx.y == 0 ? a0 : args[blockIdx.y];
     __shared__ GreedyShared S;
     GreedyState *st = A.st;
     const int tid = threadIdx.x;
@@ -1057,8 +1058,8 @@ __global__ __launch_bounds__(GT) void k_gr_select(const NMArgs *__restrict__ arg
 // The round's sequential part: priority queues of every ordered pair of the
 // round's segments, the greedy (reorder/reorder.h:1837-1932), the commit to
 // perm/inv, and the columns of the segments after the commit.
-__global__ __launch_bounds__(GT) void k_gr_greedy(const NMArgs *__restrict__ args) {
-    const NMArgs A = args[blockIdx.y];
+__global__ __launch_bounds__(GT) void k_gr_greedy(NMArgs a0, const NMArgs *__restrict__ args) {
+    const NMArgs A = blockIdx.y == 0 ? a0 : args[blockIdx.y];
     __shared__ GreedyShared S;
     GreedyState *st = A.st;
     const int tid = threadIdx.x;
@@ -1357,16 +1358,16 @@ __global__ __launch_bounds__(64) void k_gr_alldone(const NMArgs *__restrict__ ar
 
 // Capture ROUND_BATCH rounds (six kernels each, grid y = the G graphs) and
 // the all-done test on the context stream.
-static int capture_rounds(hipStream_t st, const NMArgs *d_args, int G, int gpx, int32_t *d_flag,
-                          hipGraph_t *graph_out, hipGraphExec_t *exec_out) {
+static int capture_rounds(hipStream_t st, const NMArgs &a0, const NMArgs *d_args, int G, int gpx,
+                          int32_t *d_flag, hipGraph_t *graph_out, hipGraphExec_t *exec_out) {
     VNM_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int b = 0; b < ROUND_BATCH; ++b) {
-        hipLaunchKernelGGL(k_gr_select, dim3(1, G), dim3(GT), 0, st, d_args);
-        hipLaunchKernelGGL(k_gr_fill_a, dim3(gpx, G), dim3(PT), 0, st, d_args);
-        hipLaunchKernelGGL(k_gr_score, dim3(gpx, G), dim3(PT), 0, st, d_args);
-        hipLaunchKernelGGL(k_gr_greedy, dim3(1, G), dim3(GT), 0, st, d_args);
-        hipLaunchKernelGGL(k_gr_fill_b, dim3(gpx, G), dim3(PT), 0, st, d_args);
-        hipLaunchKernelGGL(k_gr_count, dim3(gpx, G), dim3(PT), 0, st, d_args);
+        hipLaunchKernelGGL(k_gr_select, dim3(1, G), dim3(GT), 0, st, a0, d_args);
+        hipLaunchKernelGGL(k_gr_fill_a, dim3(gpx, G), dim3(PT), 0, st, a0, d_args);
+        hipLaunchKernelGGL(k_gr_score, dim3(gpx, G), dim3(PT), 0, st, a0, d_args);
+        hipLaunchKernelGGL(k_gr_greedy, dim3(1, G), dim3(GT), 0, st, a0, d_args);
+        hipLaunchKernelGGL(k_gr_fill_b, dim3(gpx, G), dim3(PT), 0, st, a0, d_args);
+        hipLaunchKernelGGL(k_gr_count, dim3(gpx, G), dim3(PT), 0, st, a0, d_args);
     }
     hipLaunchKernelGGL(k_gr_alldone, dim3(1), dim3(64), 0, st, d_args, G, d_flag);
     hipGraph_t graph = nullptr;
@@ -1477,10 +1478,10 @@ static int greedy_run(vnm_graph *const *gs, const int *scheds, int G, int N, int
     hipGraphExec_t exec = nullptr;
     if (G == 1) {
         if (!gs[0]->nm_exec &&
-            (rc = capture_rounds(st, d_args, 1, gpx, d_flag, &gs[0]->nm_graph, &gs[0]->nm_exec)))
+            (rc = capture_rounds(st, hargs[0], d_args, 1, gpx, d_flag, &gs[0]->nm_graph, &gs[0]->nm_exec)))
             return rc;
         exec = gs[0]->nm_exec;
-    } else if ((rc = capture_rounds(st, d_args, G, gpx, d_flag, &graph, &exec))) {
+    } else if ((rc = capture_rounds(st, hargs[0], d_args, G, gpx, d_flag, &graph, &exec))) {
         return rc;
     }
     struct Drop {
@@ -1499,7 +1500,7 @@ static int greedy_run(vnm_graph *const *gs, const int *scheds, int G, int N, int
         VNM_HIP_TRY(hipStreamSynchronize(st));
     }
     for (int i = 0; i < G; ++i)
-        hipLaunchKernelGGL(k_gr_cleanup, dim3(gpx), dim3(PT), 0, st, d_args + i);
+        hipLaunchKernelGGL(k_gr_cleanup, dim3(gpx), dim3(PT), 0, st, hargs[i], d_args + i);
     ctx->launches += G;
     VNM_LAUNCH_CHECK("k_gr_cleanup");
     kt.stop();
