@@ -50,6 +50,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -1739,7 +1740,10 @@ int lp_reorder_batch(vnm_graph *const *gs, int count, const vnm_lp_model *m, int
         }
     };
     std::vector<std::thread> th;
-    for (int w = 1; w < W; ++w) th.emplace_back(work, w);
+    try {  // workers that cannot be started leave their graphs to the others
+        for (int w = 1; w < W; ++w) th.emplace_back(work, w);
+    } catch (const std::system_error &) {
+    }
     work(0);
     for (auto &t : th) t.join();
     for (auto &x : aux.c) ctx->launches += x.launches;

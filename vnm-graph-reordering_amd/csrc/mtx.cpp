@@ -27,6 +27,7 @@
 #include <fcntl.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
+#include <system_error>
 #include <thread>
 #include <unistd.h>
 
@@ -76,7 +77,8 @@ int host_threads(int threads) {
     return (int)std::max(1u, std::min(h, 16u));
 }
 
-// Run f(k) for k in [0, T) on T threads (the caller's thread takes k = 0).
+// Run f(k) for k in [0, T) on up to T threads (the caller's thread takes
+// k = 0, and any k whose thread could not be started).
 template <typename F>
 void parallel_for(int T, F f) {
     if (T <= 1) {
@@ -85,8 +87,13 @@ void parallel_for(int T, F f) {
     }
     std::vector<std::thread> th;
     th.reserve(T - 1);
-    for (int k = 1; k < T; ++k) th.emplace_back(f, k);
+    int started = 1;
+    try {
+        for (; started < T; ++started) th.emplace_back(f, started);
+    } catch (const std::system_error &) {
+    }
     f(0);
+    for (int k = started; k < T; ++k) f(k);
     for (auto &x : th) x.join();
 }
 
