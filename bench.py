@@ -165,13 +165,18 @@ def bench_precision_131k(a):
         wall, kms = timed_spmm(ctx, p, Bt, dt, C, NC, a.steps, a.warmup, sc)
         info = p.info()
         alg = info.bytes + coo.ncols * NC * sb + coo.nrows * NC * 4
-        res_dt[name] = {"ms_per_step": round(wall, 5), "kernel_ms": round(kms, 5),
+        res_dt[name] = {"cols_per_lane": p.get_shape(dt, NC), "ms_per_step": round(wall, 5),
+                        "kernel_ms": round(kms, 5),
                         "tflops": round(2.0 * info.retained_nnz * NC / (wall * 1e-3) / 1e12, 4),
                         "alg_bytes": int(alg), "a_packed_bytes": int(info.bytes),
                         "achieved_gbs": round(alg / (kms * 1e-3) / 1e9, 1),
                         "retained_nnz": int(info.retained_nnz)}
         outs[name] = C.cpu().numpy()
     head = res_dt["bf16"]
+    if a.copy_calib:
+        copy_calibration(dev)
+    traffic = measured_traffic("k_spmm_rmat131072_deg24_V64_M4_traffic.json", "k_spmm_bf16",
+                               head["cols_per_lane"])
     res = {
         "metric": "V:N:M SpMM TFLOP/s (after GPU V:N:M reorder)",
         "value": head["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": a.steps,
@@ -182,7 +187,7 @@ def bench_precision_131k(a):
                    "nnz": coo.nnz, "V": V, "N": N, "M": M, "ncols": NC, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_spmm", "kernel_ms": head["kernel_ms"],
+                     "traffic": traffic, "kernel": "k_spmm", "kernel_ms": head["kernel_ms"],
                      "alg_bytes": head["alg_bytes"], "a_packed_bytes": head["a_packed_bytes"]},
         "reorder": {"method": "learned proposer + exact gains", "gpu_ms": round(st.ms, 2),
                     "init_cnt": st.init_cnt, "final_cnt": st.final_cnt,
@@ -198,6 +203,28 @@ def bench_precision_131k(a):
             res["variants"][name]["rel_err_fro"] = float(np.linalg.norm(d) / np.linalg.norm(ref))
             res["variants"][name]["rel_err_max"] = float(np.abs(d).max() / np.abs(ref).max())
     print(json.dumps(res), flush=True)
+
+
+def copy_calibration(dev):
+    """A plain streaming kernel of known traffic (reads 96 MiB, writes 96 MiB)
+    for calibrating rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    src = torch.ones(24 * 1024 * 1024, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.add(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+
+
+def measured_traffic(fname, group, vec):
+    """HBM bytes per launch of a kernel variant (values per lane `vec`) from a
+    committed counter summary under profiles/ (rocprofv3 FETCH_SIZE +
+    WRITE_SIZE per dispatch, calibrated), or None."""
+    tpath = os.path.join(ROOT, "profiles", fname)
+    if not os.path.exists(tpath):
+        return None
+    t = json.load(open(tpath)).get(group, {}).get(f"cols_per_lane_{vec}")
+    return int(t["read_bytes"] + t["write_bytes"]) if t else None
 
 
 def bench_batch64(a):
@@ -263,6 +290,10 @@ def bench_batch64(a):
     seq_ms = (time.perf_counter() - t) / a.steps * 1e3
     batch.run()
     torch.cuda.synchronize()
+    if a.copy_calib:
+        copy_calibration(dev)
+    traffic = measured_traffic("k_spmm_batch_chunglu64_traffic.json", "k_spmm_batch_bf16",
+                               batch.info()["cols_per_lane"])
     retained = sum(i.retained_nnz for i in infos)
     flops = 2.0 * retained * NC
     alg = sum(i.bytes + m.ncols * NC * 2 + m.nrows * NC * 4 for i, m in zip(infos, coos))
@@ -277,7 +308,7 @@ def bench_batch64(a):
                    "nodes": int(sum(m.nrows for m in coos)), "nnz": int(sum(m.nnz for m in coos)),
                    "V": V, "N": N, "M": M, "ncols": NC, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_spmm_batch",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_spmm_batch",
                      "kernel_ms": round(k_ms, 5), "alg_bytes": int(alg)},
         "batch": {**batch.info(), "setup_ms_incl_tuning_and_capture": round(setup_ms, 2),
                   "separate_calls_ms_per_step": round(seq_ms, 5)},
@@ -407,21 +438,9 @@ def main():
     C8 = C.cpu().numpy()
 
     if a.copy_calib:
-        # plain streaming kernel of known traffic for counter calibration:
-        # reads 96 MiB, writes 96 MiB
-        src = torch.ones(24 * 1024 * 1024, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.add(src, 1.0, out=dst)
-        torch.cuda.synchronize()
-
-    traffic = None
+        copy_calibration(dev)
     vec32 = p32.get_shape(vnm.DT_F32, NCOLS)  # the launch shape the tuner chose
-    tpath = os.path.join(ROOT, "profiles", "k_spmm_rmat65536_V128_traffic.json")
-    if os.path.exists(tpath):
-        t = json.load(open(tpath)).get("k_spmm_f32", {}).get(f"cols_per_lane_{vec32}")
-        if t:  # rocprofv3 FETCH_SIZE / WRITE_SIZE per dispatch, calibrated
-            traffic = int(t["read_bytes"] + t["write_bytes"])
+    traffic = measured_traffic("k_spmm_rmat65536_V128_traffic.json", "k_spmm_f32", vec32)
 
     res = {
         "metric": "V:N:M SpMM TFLOP/s (after GPU V:N:M reorder)",
