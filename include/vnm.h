@@ -108,7 +108,10 @@ int vnm_ctx_timer_read(vnm_ctx *ctx, const char *kernel, double *total_ms, int64
 /* ---------------------------------------------------------------- graph */
 
 /* Replaces Mtx::convert_to_bsr_and_sync_device (mtx/Mtx.h:308-379): uploads
- * the COO and builds the device segment stores.  M in {4, 8, 16}. */
+ * the COO and builds the device segment stores.  M in {4, 8, 16}.  Positions
+ * (row, col) must be distinct, as vnm_mtx_read and vnm_coo_from_tuples give
+ * them (the reference's loader removes repeats before csr2bsr,
+ * mtx/readMtx.hpp:197-218). */
 int  vnm_graph_create(vnm_ctx *ctx, const vnm_coo *coo, int M, vnm_graph **out);
 void vnm_graph_destroy(vnm_graph *g);
 /* Current COO, in the order the reference writes it (mtx/Mtx.h:429-431):
