@@ -83,6 +83,8 @@ def lib():
                                        P(ctypes.c_int64)]
         L.orc_spmm_vnm_f64.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, P(ctypes.c_float),
                                        ctypes.c_int, P(ctypes.c_double)]
+        L.orc_spmm_vnm_f64_mt.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, P(ctypes.c_float),
+                                          ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
         L.orc_quant_cpp_write.argtypes = [ctypes.c_float]
         L.orc_quant_cpp_write.restype = ctypes.c_float
         L.orc_quant_py_roundtrip.argtypes = [ctypes.c_float]
@@ -253,11 +255,12 @@ def retained_nnz(m, V, N, M):
     return out.value
 
 
-def spmm_vnm_f64(m, V, M, B):
+def spmm_vnm_f64(m, V, M, B, threads=1):
+    """FP64 V:N:M product; threads > 1 uses OpenMP (same result)."""
     B = np.ascontiguousarray(B, dtype=np.float32)
     C = np.zeros((m.nrows, B.shape[1]), np.float64)
-    _check(lib().orc_spmm_vnm_f64(ctypes.byref(m.c()), V, M, _ptr(B, ctypes.c_float), B.shape[1],
-                                  _ptr(C, ctypes.c_double)), "spmm_vnm_f64")
+    _check(lib().orc_spmm_vnm_f64_mt(ctypes.byref(m.c()), V, M, _ptr(B, ctypes.c_float), B.shape[1],
+                                     _ptr(C, ctypes.c_double), threads), "spmm_vnm_f64")
     return C
 
 

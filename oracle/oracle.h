@@ -123,6 +123,10 @@ int orc_retained_nnz(const orc_coo *coo, int V, int N, int M, int64_t *out);
  * inside them: the 2 largest |v|, ties to the lower column). */
 int orc_spmm_vnm_f64(const orc_coo *coo, int V, int M, const float *B,
                      int ncolsB, double *C);
+/* The same on `threads` OpenMP threads (the all-cores CPU baseline); the
+ * result is bit-identical to orc_spmm_vnm_f64. */
+int orc_spmm_vnm_f64_mt(const orc_coo *coo, int V, int M, const float *B, int ncolsB, double *C,
+                        int threads);
 
 /* Value quantisation of the reference's text round trips. */
 float orc_quant_cpp_write(float v);   /* "%.2f" then fscanf %f */

@@ -322,3 +322,32 @@ def test_allpairs_matches_literal_restatement(sched, clean):
         assert list(m.row) == rows and list(m.col) == cols
         if clean:
             assert ow == 0
+
+
+def test_spmm_oracle_dense_and_threads():
+    """The FP64 V:N:M product on a matrix the format keeps whole (each V x M
+    block uses at most 4 columns, each row at most 2 of them) equals the dense
+    product, and the OpenMP form is bit-identical to one thread."""
+    rng = np.random.default_rng(11)
+    R, M, V = 64, 8, 4
+    A = np.zeros((R, R), np.float32)
+    for blk in range(R // V):
+        for seg in range(R // M):
+            S = rng.choice(M, 4, replace=False)
+            for r in range(blk * V, blk * V + V):
+                for c in rng.choice(S, int(rng.integers(0, 3)), replace=False):
+                    A[r, seg * M + c] = rng.uniform(0.5, 2.0) * rng.choice([-1.0, 1.0])
+    r, c = np.nonzero(A)
+    m = oracle.Matrix(R, R, r.astype(np.int32), c.astype(np.int32), A[r, c].astype(np.float32))
+    B = rng.standard_normal((R, 48)).astype(np.float32)
+    C1 = oracle.spmm_vnm_f64(m, V, M, B)
+    np.testing.assert_allclose(C1, A.astype(np.float64) @ B.astype(np.float64), rtol=1e-12, atol=1e-12)
+    C4 = oracle.spmm_vnm_f64(m, V, M, B, threads=4)
+    assert np.array_equal(C1, C4)
+    # a pruned case: the threaded form still matches one thread exactly
+    big = (rng.random((256, 256)) < 0.2)
+    r, c = np.nonzero(big)
+    m2 = oracle.Matrix(256, 256, r.astype(np.int32), c.astype(np.int32),
+                       rng.uniform(-1, 1, r.size).astype(np.float32))
+    B2 = rng.standard_normal((256, 32)).astype(np.float32)
+    assert np.array_equal(oracle.spmm_vnm_f64(m2, 16, 8, B2), oracle.spmm_vnm_f64(m2, 16, 8, B2, threads=4))
