@@ -60,6 +60,14 @@ int vnm_mtx_read(const char *path, int M, vnm_coo *out);
  * same COO and the same first error. */
 int vnm_mtx_read_ex(const char *path, int M, int threads, vnm_coo *out);
 
+/* vnm_mtx_read through a binary cache next to the file, named as the
+ * reference's convert() names it (mtx/readMtx.hpp:230-247: ".<file>.<ud|d>.
+ * <nosl|sl>.bin"; the reference never writes one): the loader's COO before
+ * the M-padding, used while the .mtx keeps its size and modification time,
+ * written after a parse when the directory allows.  *from_cache (may be
+ * NULL) = 1 when the cache was used.  Same COO as vnm_mtx_read. */
+int vnm_mtx_read_cached(const char *path, int M, int threads, int *from_cache, vnm_coo *out);
+
 /* Replaces Mtx::write_to_file (mtx/Mtx.h:412-435): "%%MatrixMarket matrix
  * coordinate real general", "nrows ncols nnz", then "%d %d %.2f" per entry
  * in COO order (1-based). */

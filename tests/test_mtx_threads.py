@@ -200,3 +200,35 @@ def test_read_write_round_trip_threads(tmp_path):
     a, err = _same(str(p8))
     assert err is None and a.nnz == m.nnz
     assert np.array_equal(a.row, m.row) and np.array_equal(a.col, m.col)
+
+
+def test_binary_cache(tmp_path):
+    """The cache file is named as the reference names it, serves the same COO
+    for any M, and is ignored once the .mtx changes or the cache is damaged."""
+    import shutil
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mtx", "lund_b.mtx")
+    path = str(tmp_path / "lund_b.mtx")
+    shutil.copy(src, path)
+    sym = "symmetric" in open(path).readline().lower()
+    cache = tmp_path / (".lund_b.mtx.%s.nosl.bin" % ("ud" if sym else "d"))
+    for M, want_cache in ((8, False), (8, True), (4, True)):
+        got, used = vnm.mtx_read_cached(path, M)
+        ref = vnm.mtx_read(path, M)
+        assert used == want_cache and cache.exists()
+        assert (got.nrows, got.ncols) == (ref.nrows, ref.ncols)
+        assert np.array_equal(got.row, ref.row) and np.array_equal(got.col, ref.col)
+        assert np.array_equal(got.val.view(np.uint32), ref.val.view(np.uint32))
+    # the source changes: the cache is rebuilt from the new text
+    with open(path, "a") as f:
+        f.write("\n")
+    os.utime(path, ns=(0, 1_000_000_000))
+    got, used = vnm.mtx_read_cached(path, 8)
+    assert not used
+    got, used = vnm.mtx_read_cached(path, 8)
+    assert used
+    # a damaged cache is ignored and replaced
+    data = bytearray(cache.read_bytes())
+    cache.write_bytes(bytes(data[: len(data) // 2]))
+    got, used = vnm.mtx_read_cached(path, 8)
+    ref = vnm.mtx_read(path, 8)
+    assert not used and np.array_equal(got.row, ref.row) and np.array_equal(got.col, ref.col)

@@ -696,6 +696,106 @@ int write_mtx(const char *path, const HostCoo &c, int threads) {
     return VNM_OK;
 }
 
+// ---- binary cache of loaded matrices
+//
+// The reference names a binary cache next to the matrix,
+// "<dir>/.<file>.<ud|d>.<nosl|sl>.bin" (mtx/readMtx.hpp:230-247,388-405: ud
+// for symmetric input, nosl unless GRB_UTIL_REMOVE_SELFLOOP=0), and never
+// writes or reads it.  Here the file holds the loader's COO before the
+// M-padding (sorted, repeats removed), valid while the source file keeps its
+// size and modification time: a 64-byte header, then row[], col[], val[].
+
+namespace {
+
+struct CacheHeader {
+    char magic[8];  // "VNMCOO1"
+    uint64_t src_size;
+    int64_t src_mtime_ns;
+    int64_t nnz;
+    int32_t nrows, ncols;
+    int32_t symmetric;
+    char pad[20];
+};
+static_assert(sizeof(CacheHeader) == 64, "cache header layout");
+
+int64_t mtime_ns(const struct stat &st) {
+    return (int64_t)st.st_mtim.tv_sec * 1000000000ll + (int64_t)st.st_mtim.tv_nsec;
+}
+
+}  // namespace
+
+// The banner's storage field (first line), for the cache name.
+int banner_symmetric(const char *path, bool *sym) {
+    FILE *f = fopen(path, "r");
+    if (!f) return set_error(VNM_E_IO, "cannot open %s", path);
+    char line[1100] = {0};
+    const bool ok = fgets(line, sizeof line, f) != nullptr;
+    fclose(f);
+    if (!ok) return set_error(VNM_E_FORMAT, "%s: empty file", path);
+    for (char *q = line; *q; ++q) *q = (char)tolower(*q);
+    *sym = strstr(line, "symmetric") != nullptr;
+    return VNM_OK;
+}
+
+std::string cache_name(const char *path, bool symmetric) {
+    const std::string p(path);
+    const size_t sl = p.find_last_of('/');
+    const std::string dir = sl == std::string::npos ? "." : (sl == 0 ? "/" : p.substr(0, sl));
+    const std::string base = sl == std::string::npos ? p : p.substr(sl + 1);
+    const char *e = getenv("GRB_UTIL_REMOVE_SELFLOOP");
+    const bool nosl = e == nullptr || atoi(e) != 0;
+    return dir + (dir == "/" ? "" : "/") + "." + base + "." + (symmetric ? "ud" : "d") + "." +
+           (nosl ? "nosl" : "sl") + ".bin";
+}
+
+bool cache_load(const std::string &name, const struct stat &src, HostCoo &c) {
+    FILE *f = fopen(name.c_str(), "rb");
+    if (!f) return false;
+    CacheHeader h;
+    bool ok = fread(&h, sizeof h, 1, f) == 1 && memcmp(h.magic, "VNMCOO1", 8) == 0 &&
+              h.src_size == (uint64_t)src.st_size && h.src_mtime_ns == mtime_ns(src) && h.nnz >= 0 &&
+              h.nrows >= 0 && h.ncols >= 0;
+    if (ok) {
+        struct stat cs;
+        ok = fstat(fileno(f), &cs) == 0 && (uint64_t)cs.st_size == sizeof h + (uint64_t)h.nnz * 12u;
+    }
+    if (ok) {
+        c.nrows = h.nrows;
+        c.ncols = h.ncols;
+        c.row.resize(h.nnz);
+        c.col.resize(h.nnz);
+        c.val.resize(h.nnz);
+        const size_t n = (size_t)h.nnz;
+        ok = fread(c.row.data(), 4, n, f) == n && fread(c.col.data(), 4, n, f) == n &&
+             fread(c.val.data(), 4, n, f) == n;
+        for (size_t i = 0; ok && i < n; ++i)
+            ok = c.row[i] >= 0 && c.row[i] < c.nrows && c.col[i] >= 0 && c.col[i] < c.ncols;
+    }
+    fclose(f);
+    return ok;
+}
+
+// Best effort: a directory that cannot be written leaves no cache.
+void cache_store(const std::string &name, const struct stat &src, bool symmetric, const HostCoo &c) {
+    const std::string tmp = name + ".tmp" + std::to_string((long)getpid());
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return;
+    CacheHeader h;
+    memset(&h, 0, sizeof h);
+    memcpy(h.magic, "VNMCOO1", 8);
+    h.src_size = (uint64_t)src.st_size;
+    h.src_mtime_ns = mtime_ns(src);
+    h.nnz = (int64_t)c.row.size();
+    h.nrows = c.nrows;
+    h.ncols = c.ncols;
+    h.symmetric = symmetric;
+    const size_t n = c.row.size();
+    bool ok = fwrite(&h, sizeof h, 1, f) == 1 && fwrite(c.row.data(), 4, n, f) == n &&
+              fwrite(c.col.data(), 4, n, f) == n && fwrite(c.val.data(), 4, n, f) == n;
+    ok = (fclose(f) == 0) && ok;
+    if (!ok || rename(tmp.c_str(), name.c_str()) != 0) unlink(tmp.c_str());
+}
+
 int coo_export(const HostCoo &c, vnm_coo *out) {
     size_t n = c.row.size();
     out->nrows = c.nrows;
@@ -763,6 +863,31 @@ int vnm_mtx_read_ex(const char *path, int M, int threads, vnm_coo *out) {
 }
 
 int vnm_mtx_read(const char *path, int M, vnm_coo *out) { return vnm_mtx_read_ex(path, M, 0, out); }
+
+int vnm_mtx_read_cached(const char *path, int M, int threads, int *from_cache, vnm_coo *out) {
+    if (!path || !out || M < 1 || threads < 0) return set_error(VNM_E_ARG, "vnm_mtx_read_cached: bad arguments");
+    if (from_cache) *from_cache = 0;
+    struct stat st;
+    if (stat(path, &st) != 0) return set_error(VNM_E_IO, "cannot open %s", path);
+    bool sym = false;
+    int rc = banner_symmetric(path, &sym);
+    if (rc != VNM_OK) return rc;
+    const std::string cname = cache_name(path, sym);
+    HostCoo c;
+    if (cache_load(cname, st, c)) {
+        if (from_cache) *from_cache = 1;
+    } else {
+        Tuples t;
+        if ((rc = read_mtx_tuples(path, t, threads)) != VNM_OK) return rc;
+        loader_order(t, 0, c, threads);  // unpadded: the cache serves any M
+        cache_store(cname, st, sym, c);
+    }
+    if (c.nrows % M != 0) {  // mtx/Mtx.h:106-109, as loader_order pads
+        c.nrows = ((c.nrows + M - 1) / M) * M;
+        c.ncols = ((c.ncols + M - 1) / M) * M;
+    }
+    return coo_export(c, out);
+}
 
 int vnm_mtx_write_ex(const char *path, const vnm_coo *coo, int threads) {
     if (!path || !coo || threads < 0) return set_error(VNM_E_ARG, "vnm_mtx_write: bad arguments");

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <sys/stat.h>
 #include <vector>
 
 namespace vnm {
@@ -34,6 +35,10 @@ int host_threads(int threads);
 int read_mtx_tuples(const char *path, Tuples &t, int threads);
 void loader_order(Tuples &t, int M, HostCoo &out, int threads);
 int write_mtx(const char *path, const HostCoo &c, int threads);
+int banner_symmetric(const char *path, bool *sym);
+std::string cache_name(const char *path, bool symmetric);
+bool cache_load(const std::string &name, const struct stat &src, HostCoo &c);
+void cache_store(const std::string &name, const struct stat &src, bool symmetric, const HostCoo &c);
 int coo_export(const HostCoo &c, vnm_coo *out);
 void coo_import(const vnm_coo *in, HostCoo &c);
 float quant_cpp_write(float v);

@@ -39,6 +39,7 @@ EXPORTS = [
     "vnm_spmm_set_shape", "vnm_spmm_get_shape", "vnm_gcn_layer", "vnm_batch_create",
     "vnm_batch_run", "vnm_batch_info", "vnm_batch_destroy", "vnm_nm_reorder_allpairs",
     "vnm_nm_reorder_batch", "vnm_mtx_read_ex", "vnm_mtx_write_ex", "vnm_lp_reorder_batch",
+    "vnm_mtx_read_cached",
 ]
 
 LP_NPARAMS = 22784
@@ -126,6 +127,7 @@ def lib():
     L.vnm_mtx_read.argtypes = [ctypes.c_char_p, ctypes.c_int, P(Coo)]
     L.vnm_mtx_write.argtypes = [ctypes.c_char_p, P(Coo)]
     L.vnm_mtx_read_ex.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P(Coo)]
+    L.vnm_mtx_read_cached.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, P(ctypes.c_int), P(Coo)]
     L.vnm_mtx_write_ex.argtypes = [ctypes.c_char_p, P(Coo), ctypes.c_int]
     L.vnm_coo_from_tuples.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int, ctypes.c_int64,
                                       i32p, i32p, P(ctypes.c_float), ctypes.c_int, P(Coo)]
@@ -253,6 +255,15 @@ def mtx_read(path, M, threads=0):
     c = Coo()
     _check(lib().vnm_mtx_read_ex(str(path).encode(), M, threads, ctypes.byref(c)))
     return _take(c)
+
+
+def mtx_read_cached(path, M, threads=0):
+    """mtx_read through the binary cache next to the file; returns (coo,
+    whether the cache was used)."""
+    c = Coo()
+    used = ctypes.c_int()
+    _check(lib().vnm_mtx_read_cached(str(path).encode(), M, threads, ctypes.byref(used), ctypes.byref(c)))
+    return _take(c), bool(used.value)
 
 
 def mtx_write(path, coo, threads=0):
