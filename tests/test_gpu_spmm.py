@@ -7,6 +7,8 @@ max|dC| / max|C|):
   A FP32, B FP32 : 1e-5   (FP32 accumulation only)
   A BF16, B BF16 : 1e-2   (bf16 rounding of both operands)
   A FP8,  B FP8  : 1e-1   (e4m3, per-block / per-128-row scales)
+  A FP16, B FP32 : 2e-3   (fp16 rounding of the values: 2^-11 relative)
+  A FP16, B BF16 : 1e-2   (bf16 rounding of B dominates)
 """
 import os
 
@@ -123,6 +125,17 @@ def test_spmm_rmat_bf16_and_fp8(ctx):
     C, ref, _ = _run(ctx, m, 8, 128, 256, vnm.DT_FP8, vnm.DT_FP8)
     fro, mx = _errs(C, ref)
     assert fro < 1e-1 and mx < 1e-1, (fro, mx)
+
+
+def test_spmm_fp16_values(ctx):
+    # FP16 matrix values with FP32 and BF16 B, M = 8 and 4
+    for mm, M, V in ((vnm.gen_rmat(13, 60000, 6), 8, 128), (vnm.gen_rmat(12, 40000, 7, M=4), 4, 64)):
+        C, ref, _ = _run(ctx, mm, M, V, 128, vnm.DT_F16, vnm.DT_F32)
+        fro, mx = _errs(C, ref)
+        assert fro < 2e-3 and mx < 2e-3, (M, fro, mx)
+        C, ref, _ = _run(ctx, mm, M, V, 256, vnm.DT_F16, vnm.DT_BF16)
+        fro, mx = _errs(C, ref)
+        assert fro < 1e-2 and mx < 1e-2, (M, fro, mx)
 
 
 def test_spmm_m4_bf16(ctx):
