@@ -127,8 +127,14 @@ void vnm_graph_destroy(vnm_graph *g);
 int  vnm_graph_download(vnm_graph *g, vnm_coo *out);
 /* A device copy of g (current COO and composed relabel) on the same context. */
 int  vnm_graph_clone(vnm_graph *g, vnm_graph **out);
-/* Map from the ids at vnm_graph_create to current ids (int32[nrows]). */
+/* Map from the ids at vnm_graph_create to current ids (int32[nrows given to
+ * vnm_graph_create]).  A reload to the (n, n) shape with n the padded column
+ * count (reordering.py:180) can cut off empty rows: their ids stay at or past
+ * the current row count. */
 int  vnm_graph_perm(vnm_graph *g, int32_t *perm);
+/* Current shape and entry count: Mtx's nrows, ncols, nnz (mtx/Mtx.h:54-56),
+ * which a VENOM reload resets to the padded column count. */
+int  vnm_graph_shape(vnm_graph *g, int32_t *nrows, int32_t *ncols, int64_t *nnz);
 
 /* ---------------------------------------------------------------- N:M primitives */
 

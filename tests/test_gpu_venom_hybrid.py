@@ -141,3 +141,68 @@ def test_kat_temp_mtx_venom(ctx):
     g = vnm.Graph(ctx, m, 8)
     for V in (1, 8, 16, 32):
         assert g.venom_eval(V) == (0, 0)
+
+
+def _rect(rng, M):
+    """More padded rows than columns, entries in the leading square part or
+    anywhere."""
+    C = int(rng.integers(2, 8)) * M
+    R = C + int(rng.integers(1, 5)) * M
+    lim = C if rng.random() < 0.6 else R
+    mask = rng.random((lim, C)) < rng.uniform(0.05, 0.4)
+    r, c = np.nonzero(mask)
+    return vnm.coo_from_tuples(R, C, r, c, rng.uniform(-1, 1, r.size).astype(np.float32), M)
+
+
+def test_rectangular_hybrid_venom_and_pack(ctx):
+    """The row permutation renames columns too, so on a matrix with more
+    padded rows than columns VENOM, the reload to the (n, n) shape and the
+    packer can meet entries past the padded column count, where the
+    reference's Python raises.  The device refuses exactly where the oracle
+    does; elsewhere its results, the shape the reload leaves and the greedy
+    rounds captured before the rows shrank are the oracle's."""
+    rng = np.random.default_rng(44)
+    hyb, packs = set(), set()
+    for t in range(48):
+        M = (4, 8)[t % 2]
+        m = _rect(rng, M)
+        V = int(rng.choice([1, 4, 16]))
+        sched = t % 4
+        print("case", t, M, m.nrows, m.ncols, V, sched, flush=True)
+        g = vnm.Graph(ctx, m, M)
+        try:
+            st, err = g.hybrid(2, V, 2, sched).as_tuple(), None
+        except vnm.VnmError as e:
+            st, err = None, e
+        try:
+            (ost, oout), oerr = oracle.hybrid(_om(m), 2, M, V, 2, sched), None
+        except RuntimeError as e:
+            oerr = e
+        assert (err is None) == (oerr is None), (t, err, oerr)
+        if err is None:
+            out = g.download()
+            assert st[:15] == ost, (t, st, ost)
+            assert (g.nrows, g.ncols) == (oout.nrows, oout.ncols)
+            assert np.array_equal(out.row, oout.row) and np.array_equal(out.col, oout.col)
+            assert np.array_equal(out.val, oout.val)
+            assert g.perm().shape == (m.nrows,)
+        hyb.add(err is None)
+        g.close()
+        # one N:M run, then the packer
+        g2 = vnm.Graph(ctx, m, M)
+        g2.nm_reorder(2, sched)
+        om = _om(m).copy()
+        oracle.nm_reorder(om, 2, M, sched)
+        out2 = g2.download()
+        assert np.array_equal(out2.row, om.row) and np.array_equal(out2.col, om.col)
+        over = om.nnz > 0 and int(om.col.max()) >= (m.ncols + M - 1) // M * M
+        try:
+            got = g2.retained_nnz(V)
+        except vnm.VnmError:
+            got = None
+        assert (got is None) == over, t
+        if got is not None:
+            assert got == oracle.retained_nnz(om, V, 2, M)
+        packs.add(over)
+        g2.close()
+    assert hyb == {True, False} and packs == {True, False}

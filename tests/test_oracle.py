@@ -351,3 +351,88 @@ def test_spmm_oracle_dense_and_threads():
                        rng.uniform(-1, 1, r.size).astype(np.float32))
     B2 = rng.standard_normal((256, 32)).astype(np.float32)
     assert np.array_equal(oracle.spmm_vnm_f64(m2, 16, 8, B2), oracle.spmm_vnm_f64(m2, 16, 8, B2, threads=4))
+
+
+def _rect(rng, M):
+    """A matrix with more padded rows than columns, its entries in the
+    leading square part or anywhere."""
+    C = int(rng.integers(2, 8)) * M
+    R = C + int(rng.integers(1, 5)) * M
+    lim = C if rng.random() < 0.6 else R
+    mask = rng.random((lim, C)) < rng.uniform(0.05, 0.4)
+    r, c = np.nonzero(mask)
+    return oracle.from_tuples(R, C, r, c, rng.uniform(-1, 1, r.size), M)
+
+
+def test_venom_rectangular_matches_literal():
+    """new_id renames columns too, so with more padded rows than columns a
+    column can land past the n // M segments, where generate_segmat raises
+    IndexError (reordering.py:113-118).  The oracle refuses exactly where the
+    literal restatement raises and agrees with it everywhere else."""
+    rng = np.random.default_rng(12)
+    raised = kept = 0
+    for t in range(120):
+        M = (4, 8)[t % 2]
+        m = _rect(rng, M)
+        V = int(rng.choice([1, 4, 16]))
+        try:
+            want = L.venom_reorder(m.nrows, m.ncols, m.row.tolist(), m.col.tolist(), M, V)
+        except IndexError:
+            want = None
+        om = m.copy()
+        try:
+            got = oracle.venom_reorder(om, M, V)
+        except RuntimeError:
+            got = None
+        assert (want is None) == (got is None), t
+        if want is None:
+            raised += 1
+            continue
+        kept += 1
+        assert got.tolist() == want[0]
+        assert om.row.tolist() == want[1] and om.col.tolist() == want[2]
+    assert raised > 0 and kept > 0
+
+
+def test_reload_refuses_rows_past_the_written_shape():
+    """The reload writes coo_matrix(shape=(n, n)), n the padded column count
+    (reordering.py:180); scipy refuses a row index >= n, and so does the
+    oracle's reload and the hybrid that runs it."""
+    from scipy.sparse import coo_matrix
+    rng = np.random.default_rng(13)
+    outcomes = set()
+    for t in range(60):
+        M = (4, 8)[t % 2]
+        m = _rect(rng, M)
+        n = (m.ncols + M - 1) // M * M
+        try:
+            coo_matrix((m.val, (m.row, m.col)), shape=(n, n))
+            ok = True
+        except ValueError:
+            ok = False
+        om = m.copy()
+        try:
+            oracle.reload(om, M)
+            got = True
+        except RuntimeError:
+            got = False
+        assert ok == got, t
+        if ok:
+            assert (om.nrows, om.ncols) == (n, n)
+        outcomes.add(ok)
+    assert outcomes == {True, False}
+    # the hybrid refuses when a round's reload or VENOM step would, and on
+    # success ends at the written shape
+    hy = set()
+    for t in range(80):
+        M = (4, 8)[t % 2]
+        m = _rect(rng, M)
+        try:
+            st, out = oracle.hybrid(m.copy(), 2, M, int(rng.choice([1, 4, 16])), 2, t % 4)
+            n = (m.ncols + M - 1) // M * M
+            assert (out.nrows, out.ncols) == (n, n)
+            assert out.nnz == 0 or (out.row.max() < n and out.col.max() < n)
+            hy.add(True)
+        except RuntimeError:
+            hy.add(False)
+    assert hy == {True, False}

@@ -41,18 +41,19 @@ int graph_clone(vnm_graph *src, vnm_graph **out) {
     g->ncols = src->ncols;
     g->M = src->M;
     g->nnz = src->nnz;
+    g->col_overflow = src->col_overflow;
     hipStream_t st = src->ctx->stream;
     int rc;
     const size_t n = (size_t)src->nnz;
     if ((rc = g->row.alloc(n)) || (rc = g->col.alloc(n)) || (rc = g->val.alloc(n)) ||
-        (rc = g->perm_total.alloc(std::max(g->nrows, 1)))) {
+        (rc = g->perm_total.alloc(src->perm_total.n))) {
         delete g;
         return rc;
     }
     if (hipMemcpyAsync(g->row.p, src->row.p, n * 4, hipMemcpyDeviceToDevice, st) != hipSuccess ||
         hipMemcpyAsync(g->col.p, src->col.p, n * 4, hipMemcpyDeviceToDevice, st) != hipSuccess ||
         hipMemcpyAsync(g->val.p, src->val.p, n * 4, hipMemcpyDeviceToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(g->perm_total.p, src->perm_total.p, (size_t)std::max(g->nrows, 1) * 4,
+        hipMemcpyAsync(g->perm_total.p, src->perm_total.p, src->perm_total.n * 4,
                        hipMemcpyDeviceToDevice, st) != hipSuccess) {
         delete g;
         return set_error(VNM_E_HIP, "graph clone copy failed");
@@ -213,8 +214,16 @@ int vnm_graph_download(vnm_graph *g, vnm_coo *out) {
 int vnm_graph_perm(vnm_graph *g, int32_t *perm) {
     if (!g || !perm) return set_error(VNM_E_ARG, "null argument");
     hipStream_t st = g->ctx->stream;
-    VNM_HIP_TRY(hipMemcpyAsync(perm, g->perm_total.p, (size_t)g->nrows * 4, hipMemcpyDeviceToHost, st));
+    VNM_HIP_TRY(hipMemcpyAsync(perm, g->perm_total.p, g->perm_total.n * 4, hipMemcpyDeviceToHost, st));
     VNM_HIP_TRY(hipStreamSynchronize(st));
+    return VNM_OK;
+}
+
+int vnm_graph_shape(vnm_graph *g, int32_t *nrows, int32_t *ncols, int64_t *nnz) {
+    if (!g || !nrows || !ncols || !nnz) return set_error(VNM_E_ARG, "null argument");
+    *nrows = g->nrows;
+    *ncols = g->ncols;
+    *nnz = g->nnz;
     return VNM_OK;
 }
 

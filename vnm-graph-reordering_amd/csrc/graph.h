@@ -81,7 +81,11 @@ struct vnm_graph {
     // buffers above, which live as long as the graph)
     hipGraph_t nm_graph = nullptr;
     hipGraphExec_t nm_exec = nullptr;
+    std::vector<uint8_t> nm_key;          // the arguments nm_exec was captured with
     size_t tab_slots = 0;
+    // a relabel of a matrix with fewer columns than rows moved entries past
+    // the padded column count (the row permutation also renames columns)
+    bool col_overflow = false;
     // bumped whenever the COO is relabelled or re-sorted; caches built from
     // the COO (the proposer's adjacency) are valid for one epoch
     int64_t epoch = 0;
@@ -101,6 +105,7 @@ int graph_scratch(vnm_graph *g);
 int build_csc(vnm_graph *g);
 int locate_all(vnm_graph *g, int N, int32_t *d_cnt, int32_t *d_nzr);
 int relabel(vnm_graph *g, const int32_t *d_perm);
+int max_ids(vnm_graph *g, const int32_t *row, const int32_t *col, int32_t *max_row, int32_t *max_col);
 int nm_reorder(vnm_graph *g, int N, int sched, int maxiter, vnm_nm_stats *st);
 int nm_reorder_batch(vnm_graph *const *gs, int count, int N, const int *scheds, int maxiter,
                      vnm_nm_stats *stats);

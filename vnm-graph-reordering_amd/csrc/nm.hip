@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 namespace vnm {
@@ -295,9 +296,31 @@ __global__ void k_relabel(int32_t *__restrict__ row, int32_t *__restrict__ col, 
     }
 }
 
-__global__ void k_compose(int32_t *__restrict__ total, const int32_t *__restrict__ p, int32_t n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) total[i] = p[total[i]];
+// Largest row and column id of the entries: one atomic per wave.
+__global__ __launch_bounds__(256) void k_max_ids(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                                                 int64_t nnz, int32_t *__restrict__ out) {
+    int mr = -1, mc = -1;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        mr = max(mr, row[i]);
+        mc = max(mc, col[i]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mr = max(mr, __shfl_xor(mr, o));
+        mc = max(mc, __shfl_xor(mc, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&out[0], mr);
+        atomicMax(&out[1], mc);
+    }
+}
+
+// total[i] = p[total[i]] over the ids at creation; ids at or past the p's
+// length (empty rows a reload's smaller shape cut off) stay as they are.
+__global__ void k_compose(int32_t *__restrict__ total, int64_t n, const int32_t *__restrict__ p, int32_t R) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t t = total[i];
+        if (t < R) total[i] = p[t];
+    }
 }
 
 // ------------------------------------------------------------ greedy
@@ -1323,15 +1346,46 @@ int locate_all(vnm_graph *g, int N, int32_t *d_cnt, int32_t *d_nzr) {
     return VNM_OK;
 }
 
+int max_ids(vnm_graph *g, const int32_t *row, const int32_t *col, int32_t *max_row, int32_t *max_col) {
+    hipStream_t st = g->ctx->stream;
+    int32_t h[2] = {-1, -1};
+    if (g->nnz > 0) {
+        DBuf<int32_t> out;
+        int rc = out.alloc(2);
+        if (rc) return rc;
+        VNM_HIP_TRY(hipMemcpyAsync(out.p, h, 8, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_max_ids, dim3(std::min<int64_t>(grid_for(g->nnz, 256), 1024)), dim3(256), 0, st,
+                           row, col, g->nnz, out.p);
+        g->ctx->launches++;
+        VNM_LAUNCH_CHECK("k_max_ids");
+        VNM_HIP_TRY(hipMemcpyAsync(h, out.p, 8, hipMemcpyDeviceToHost, st));
+        VNM_HIP_TRY(hipStreamSynchronize(st));
+    }
+    *max_row = h[0];
+    *max_col = h[1];
+    return VNM_OK;
+}
+
 int relabel(vnm_graph *g, const int32_t *d_perm) {
     hipStream_t st = g->ctx->stream;
     g->epoch++;
     hipLaunchKernelGGL(k_relabel, dim3(grid_for(g->nnz, 256)), dim3(256), 0, st, g->row.p,
                        g->col.p, g->nnz, d_perm, g->nrows);
-    hipLaunchKernelGGL(k_compose, dim3(grid_for(g->nrows, 256)), dim3(256), 0, st,
-                       g->perm_total.p, d_perm, g->nrows);
+    hipLaunchKernelGGL(k_compose, dim3(grid_for((int64_t)g->perm_total.n, 256)), dim3(256), 0, st,
+                       g->perm_total.p, (int64_t)g->perm_total.n, d_perm, g->nrows);
     g->ctx->launches += 2;
     VNM_LAUNCH_CHECK("k_relabel");
+    // the Reorder kernel renames columns with the row permutation
+    // (reorder/reorder.h:23-35): with fewer columns than rows an entry can
+    // move past the padded column count, which the per-segment passes of
+    // VENOM and the packer must refuse
+    const int M = g->M;
+    if ((g->ncols + M - 1) / M * M < g->nrows && !g->col_overflow) {
+        int32_t mr, mc;
+        int rc = max_ids(g, g->row.p, g->col.p, &mr, &mc);
+        if (rc) return rc;
+        g->col_overflow = mc >= (g->ncols + M - 1) / M * M;
+    }
     return VNM_OK;
 }
 
@@ -1477,10 +1531,23 @@ static int greedy_run(vnm_graph *const *gs, const int *scheds, int G, int N, int
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (G == 1) {
-        if (!gs[0]->nm_exec &&
-            (rc = capture_rounds(st, hargs[0], d_args, 1, gpx, d_flag, &gs[0]->nm_graph, &gs[0]->nm_exec)))
-            return rc;
-        exec = gs[0]->nm_exec;
+        // the captured rounds hold the arguments by value: capture again when
+        // they changed (a reload that shrinks the rows moves records B)
+        vnm_graph *g0 = gs[0];
+        const uint8_t *key = (const uint8_t *)&hargs[0];
+        if (g0->nm_exec && (g0->nm_key.size() != sizeof(NMArgs) ||
+                            memcmp(g0->nm_key.data(), key, sizeof(NMArgs)) != 0)) {
+            (void)hipGraphExecDestroy(g0->nm_exec);
+            (void)hipGraphDestroy(g0->nm_graph);
+            g0->nm_exec = nullptr;
+            g0->nm_graph = nullptr;
+        }
+        if (!g0->nm_exec) {
+            if ((rc = capture_rounds(st, hargs[0], d_args, 1, gpx, d_flag, &g0->nm_graph, &g0->nm_exec)))
+                return rc;
+            g0->nm_key.assign(key, key + sizeof(NMArgs));
+        }
+        exec = g0->nm_exec;
     } else if ((rc = capture_rounds(st, hargs[0], d_args, G, gpx, d_flag, &graph, &exec))) {
         return rc;
     }

@@ -362,6 +362,9 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
     if (V <= 0 || V > 256) return set_error(VNM_E_ARG, "V must be in [1, 256] (got %d)", V);
     if (M != 4 && M != 8 && M != 16) return set_error(VNM_E_ARG, "M must be 4, 8 or 16");
     if (dtype < VNM_DT_F32 || dtype > VNM_DT_FP8) return set_error(VNM_E_ARG, "bad dtype %d", dtype);
+    if (g->col_overflow)
+        return set_error(VNM_E_ARG, "entries lie past the padded column count after a relabel of a matrix "
+                                    "with fewer columns than rows");
     hipStream_t st = g->ctx->stream;
     const int R = g->nrows;
     const int nseg = (g->ncols + M - 1) / M;

@@ -216,9 +216,13 @@ __global__ void k_unpack_sorted(const uint64_t *__restrict__ k, const int32_t *_
     }
 }
 
-__global__ void k_compose_v(int32_t *__restrict__ total, const int32_t *__restrict__ p, int32_t n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) total[i] = p[total[i]];
+// total[i] = p[total[i]] over the ids at creation (p holds R ids).
+__global__ void k_compose_v(int32_t *__restrict__ total, int64_t n, const int32_t *__restrict__ p, int32_t R) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int32_t t = total[i];
+        if (t < R) total[i] = p[t];
+    }
 }
 
 // ------------------------------------------------------------ host side
@@ -237,6 +241,9 @@ struct VenomWork {
 
     int init(vnm_graph *graph, int V_) {
         g = graph;
+        if (graph->col_overflow)
+            return set_error(VNM_E_ARG, "entries lie past the padded column count after a relabel of a matrix "
+                                        "with fewer columns than rows (the reference's segmat raises here)");
         M = g->M;
         V = V_;
         m = ((g->nrows + M - 1) / M) * M;
@@ -387,17 +394,12 @@ int reload_sort(vnm_graph *g, bool quantize) {
     // mmwrite(shape=(n, n)) with n the padded column count (reordering.py:180)
     const int M = g->M;
     const int n = ((g->ncols + M - 1) / M) * M;
-    g->nrows = n;
-    g->ncols = n;
-    if (nnz == 0) return VNM_OK;
-    int rc;
-    if (quantize) {
-        std::vector<float> v(nnz);
-        VNM_HIP_TRY(hipMemcpyAsync(v.data(), g->val.p, nnz * 4, hipMemcpyDeviceToHost, st));
-        VNM_HIP_TRY(hipStreamSynchronize(st));
-        quantize_values(v);
-        VNM_HIP_TRY(hipMemcpyAsync(g->val.p, v.data(), nnz * 4, hipMemcpyHostToDevice, st));
+    if (nnz == 0) {
+        g->nrows = n;
+        g->ncols = n;
+        return VNM_OK;
     }
+    int rc;
     DBuf<uint64_t> k1, k2;
     DBuf<int32_t> i1, i2, dup;
     DBuf<float> v2;
@@ -408,6 +410,28 @@ int reload_sort(vnm_graph *g, bool quantize) {
     hipLaunchKernelGGL(k_pos_keys, dim3(gb), dim3(256), 0, st, g->row.p, g->col.p, nnz, k1.p);
     hipLaunchKernelGGL(k_iota64, dim3(gb), dim3(256), 0, st, i1.p, nnz);
     if ((rc = sort_pairs_u64_i32(k1.p, k2.p, i1.p, i2.p, nnz, 64, st))) return rc;
+    if (g->col_overflow)
+        return set_error(VNM_E_ARG, "entries lie past the padded column count; the reference's "
+                                    "coo_matrix(shape=(n, n)) raises here");
+    if (g->nrows > n) {
+        // coo_matrix(shape=(n, n)) refuses a row index >= n; the last sorted
+        // key holds the largest row.  The matrix is left as it is.
+        uint64_t last = 0;
+        VNM_HIP_TRY(hipMemcpyAsync(&last, k2.p + nnz - 1, 8, hipMemcpyDeviceToHost, st));
+        VNM_HIP_TRY(hipStreamSynchronize(st));
+        if ((int64_t)(last >> 32) >= n)
+            return set_error(VNM_E_ARG, "row %lld exceeds the %d x %d shape the matrix is written with "
+                                        "(the reference's coo_matrix raises here)", (long long)(last >> 32), n, n);
+    }
+    g->nrows = n;
+    g->ncols = n;
+    if (quantize) {
+        std::vector<float> v(nnz);
+        VNM_HIP_TRY(hipMemcpyAsync(v.data(), g->val.p, nnz * 4, hipMemcpyDeviceToHost, st));
+        VNM_HIP_TRY(hipStreamSynchronize(st));
+        quantize_values(v);
+        VNM_HIP_TRY(hipMemcpyAsync(g->val.p, v.data(), nnz * 4, hipMemcpyHostToDevice, st));
+    }
     VNM_HIP_TRY(hipMemsetAsync(dup.p, 0, 4, st));
     hipLaunchKernelGGL(k_unpack_sorted, dim3(gb), dim3(256), 0, st, k2.p, i2.p, g->val.p, nnz,
                        g->row.p, g->col.p, v2.p, dup.p);
@@ -467,14 +491,31 @@ int venom_reorder(vnm_graph *g, int V, int maxiter, vnm_venom_stats *st) {
         hipLaunchKernelGGL(k_venom_relabel, dim3(grid_of(g->nnz, 256)), dim3(256), 0, s, cur_r,
                            cur_c, g->nnz, w.order.p, nxt_r, nxt_c);
         g->ctx->launches++;
+        if (w.n < w.m) {
+            // new_id renames columns too: with more padded rows than columns
+            // one can land past the n // M segments, where generate_segmat
+            // raises IndexError (reordering.py:113-118).  The matrix keeps
+            // the last kept round.
+            int32_t mr, mc;
+            if ((rc = max_ids(g, nxt_r, nxt_c, &mr, &mc))) return rc;
+            if (mc >= w.n) {
+                if (cur_r != g->row.p) {
+                    VNM_HIP_TRY(hipMemcpyAsync(g->row.p, cur_r, g->nnz * 4, hipMemcpyDeviceToDevice, s));
+                    VNM_HIP_TRY(hipMemcpyAsync(g->col.p, cur_c, g->nnz * 4, hipMemcpyDeviceToDevice, s));
+                    VNM_HIP_TRY(hipStreamSynchronize(s));
+                }
+                return set_error(VNM_E_ARG, "VENOM round %d moves column %d past the %d padded columns "
+                                            "(the reference's segmat raises here)", round, mc, w.n);
+            }
+        }
         int strict;
         if ((rc = w.count(nxt_r, nxt_c, &inv, &strict))) return rc;
         if (inv > prev) {  // reordering.py:242-246: keep the previous positions
             final_round = round;
             break;
         }
-        hipLaunchKernelGGL(k_compose_v, dim3((g->nrows + 255) / 256), dim3(256), 0, s,
-                           g->perm_total.p, w.order.p, g->nrows);
+        hipLaunchKernelGGL(k_compose_v, dim3((unsigned)((g->perm_total.n + 255) / 256)), dim3(256), 0, s,
+                           g->perm_total.p, (int64_t)g->perm_total.n, w.order.p, w.m);
         g->ctx->launches++;
         prev = inv;
         prev_strict = strict;

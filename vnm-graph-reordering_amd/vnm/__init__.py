@@ -29,7 +29,7 @@ EXPORTS = [
     "vnm_gen_er", "vnm_gen_rmat", "vnm_gen_chung_lu", "vnm_gen_normal",
     "vnm_ctx_create", "vnm_ctx_destroy", "vnm_ctx_stream", "vnm_ctx_launches",
     "vnm_ctx_timing", "vnm_ctx_timer_read",
-    "vnm_graph_create", "vnm_graph_destroy", "vnm_graph_download", "vnm_graph_perm",
+    "vnm_graph_create", "vnm_graph_destroy", "vnm_graph_download", "vnm_graph_perm", "vnm_graph_shape",
     "vnm_locate", "vnm_swap_score", "vnm_pq_pop_order", "vnm_nm_reorder",
     "vnm_venom_eval", "vnm_venom_reorder", "vnm_hybrid", "vnm_find_best_sched",
     "vnm_retained_nnz", "vnm_pack", "vnm_packed_info_get", "vnm_packed_destroy", "vnm_spmm",
@@ -155,6 +155,7 @@ def lib():
     L.vnm_graph_destroy.restype = None
     L.vnm_graph_download.argtypes = [vp, P(Coo)]
     L.vnm_graph_perm.argtypes = [vp, i32p]
+    L.vnm_graph_shape.argtypes = [vp, P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int64)]
     L.vnm_locate.argtypes = [vp, ctypes.c_int, i32p, i32p]
     L.vnm_swap_score.argtypes = [vp, ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, i32p]
     L.vnm_pq_pop_order.argtypes = [vp, i32p, ctypes.c_int, ctypes.c_int, i32p]
@@ -378,7 +379,26 @@ class Graph:
         _check(lib().vnm_graph_create(ctx.h, ctypes.byref(coo.c()), M, ctypes.byref(h)))
         self.h = h
         ctx._adopt(self)
-        self.nrows, self.ncols = coo.nrows, coo.ncols
+        self.nrows0 = coo.nrows  # ids at creation: the length of perm()
+
+    def _shape(self):
+        r, c, n = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        _check(lib().vnm_graph_shape(self.h, ctypes.byref(r), ctypes.byref(c), ctypes.byref(n)))
+        return r.value, c.value, n.value
+
+    @property
+    def nrows(self):
+        """Current rows: a VENOM reload resets the shape to (n, n), n the
+        padded column count (reordering.py:180)."""
+        return self._shape()[0]
+
+    @property
+    def ncols(self):
+        return self._shape()[1]
+
+    @property
+    def nnz(self):
+        return self._shape()[2]
 
     def close(self):
         if self.h:
@@ -391,7 +411,7 @@ class Graph:
         h = ctypes.c_void_p()
         _check(lib().vnm_graph_clone(self.h, ctypes.byref(h)))
         g = Graph.__new__(Graph)
-        g.ctx, g.M, g.h, g.nrows, g.ncols = self.ctx, self.M, h, self.nrows, self.ncols
+        g.ctx, g.M, g.h, g.nrows0 = self.ctx, self.M, h, self.nrows0
         self.ctx._adopt(g)
         return g
 
@@ -411,7 +431,7 @@ class Graph:
         return _take(c)
 
     def perm(self):
-        out = np.zeros(self.nrows, np.int32)
+        out = np.zeros(self.nrows0, np.int32)
         _check(lib().vnm_graph_perm(self.h, _p(out, ctypes.c_int32)))
         return out
 
