@@ -226,9 +226,21 @@ def test_binary_cache(tmp_path):
     assert not used
     got, used = vnm.mtx_read_cached(path, 8)
     assert used
-    # a damaged cache is ignored and replaced
+    # a damaged cache is ignored and replaced: cut short, or of the right
+    # size with two entries out of order
     data = bytearray(cache.read_bytes())
     cache.write_bytes(bytes(data[: len(data) // 2]))
     got, used = vnm.mtx_read_cached(path, 8)
     ref = vnm.mtx_read(path, 8)
+    assert not used and np.array_equal(got.row, ref.row) and np.array_equal(got.col, ref.col)
+    got, used = vnm.mtx_read_cached(path, 8)
+    assert used
+    data = bytearray(cache.read_bytes())
+    nnz = (len(data) - 64) // 12
+    rows = np.frombuffer(bytes(data[64:64 + 4 * nnz]), np.int32).copy()
+    k = int(np.nonzero(rows[1:] > rows[:-1])[0][0])  # rows[k] < rows[k + 1]
+    rows[k], rows[k + 1] = rows[k + 1], rows[k]
+    data[64:64 + 4 * nnz] = rows.tobytes()
+    cache.write_bytes(bytes(data))
+    got, used = vnm.mtx_read_cached(path, 8)
     assert not used and np.array_equal(got.row, ref.row) and np.array_equal(got.col, ref.col)

@@ -754,7 +754,7 @@ bool cache_load(const std::string &name, const struct stat &src, HostCoo &c) {
     CacheHeader h;
     bool ok = fread(&h, sizeof h, 1, f) == 1 && memcmp(h.magic, "VNMCOO1", 8) == 0 &&
               h.src_size == (uint64_t)src.st_size && h.src_mtime_ns == mtime_ns(src) && h.nnz >= 0 &&
-              h.nrows >= 0 && h.ncols >= 0;
+              h.nnz <= ((int64_t)1 << 40) && h.nrows >= 0 && h.ncols >= 0;
     if (ok) {
         struct stat cs;
         ok = fstat(fileno(f), &cs) == 0 && (uint64_t)cs.st_size == sizeof h + (uint64_t)h.nnz * 12u;
@@ -768,8 +768,11 @@ bool cache_load(const std::string &name, const struct stat &src, HostCoo &c) {
         const size_t n = (size_t)h.nnz;
         ok = fread(c.row.data(), 4, n, f) == n && fread(c.col.data(), 4, n, f) == n &&
              fread(c.val.data(), 4, n, f) == n;
+        // in range, and strictly increasing in (row, col) as the loader
+        // leaves them: a damaged file of the right size is refused too
         for (size_t i = 0; ok && i < n; ++i)
-            ok = c.row[i] >= 0 && c.row[i] < c.nrows && c.col[i] >= 0 && c.col[i] < c.ncols;
+            ok = c.row[i] >= 0 && c.row[i] < c.nrows && c.col[i] >= 0 && c.col[i] < c.ncols &&
+                 (i == 0 || c.row[i] > c.row[i - 1] || (c.row[i] == c.row[i - 1] && c.col[i] > c.col[i - 1]));
     }
     fclose(f);
     return ok;
