@@ -81,6 +81,17 @@ def test_venom_rmat_medium(ctx):
         assert np.array_equal(out.row, om.row) and np.array_equal(out.col, om.col)
 
 
+def _perm_maps_positions(g, m, out):
+    """Every relabel renames rows and columns alike and the reload only
+    re-sorts, so the final positions are the input's mapped through the
+    composed relabel."""
+    perm = g.perm()
+    assert perm.shape == (m.nrows,)
+    want = np.unique(perm[m.row].astype(np.int64) << 32 | perm[m.col].astype(np.int64))
+    got = np.unique(out.row.astype(np.int64) << 32 | out.col.astype(np.int64))
+    assert np.array_equal(want, got)
+
+
 def _hyb(ctx, m, M, V, rounds, sched):
     g = vnm.Graph(ctx, m, M)
     st = g.hybrid(2, V, rounds, sched).as_tuple()
@@ -89,6 +100,8 @@ def _hyb(ctx, m, M, V, rounds, sched):
     assert st[:15] == ost, (st, ost)
     assert np.array_equal(out.row, oout.row) and np.array_equal(out.col, oout.col)
     assert np.array_equal(out.val, oout.val)
+    if m.nnz:
+        _perm_maps_positions(g, m, out)
     return st
 
 
@@ -185,7 +198,11 @@ def test_rectangular_hybrid_venom_and_pack(ctx):
             assert (g.nrows, g.ncols) == (oout.nrows, oout.ncols)
             assert np.array_equal(out.row, oout.row) and np.array_equal(out.col, oout.col)
             assert np.array_equal(out.val, oout.val)
-            assert g.perm().shape == (m.nrows,)
+            _perm_maps_positions(g, m, out)
+            # a clone keeps the composed relabel over the ids at creation
+            c = g.clone()
+            assert np.array_equal(c.perm(), g.perm()) and (c.nrows, c.ncols) == (g.nrows, g.ncols)
+            c.close()
         hyb.add(err is None)
         g.close()
         # one N:M run, then the packer
