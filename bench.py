@@ -164,11 +164,12 @@ def bench_precision_131k(a):
             Bt, sc, sb = Bd, None, 4
         wall, kms = timed_spmm(ctx, p, Bt, dt, C, NC, a.steps, a.warmup, sc)
         info = p.info()
-        alg = info.bytes + coo.ncols * NC * sb + coo.nrows * NC * 4
+        alg = info.spmm_bytes + coo.ncols * NC * sb + coo.nrows * NC * 4
         res_dt[name] = {"cols_per_lane": p.get_shape(dt, NC), "ms_per_step": round(wall, 5),
                         "kernel_ms": round(kms, 5),
                         "tflops": round(2.0 * info.retained_nnz * NC / (wall * 1e-3) / 1e12, 4),
                         "alg_bytes": int(alg), "a_packed_bytes": int(info.bytes),
+                        "a_read_bytes": int(info.spmm_bytes),
                         "achieved_gbs": round(alg / (kms * 1e-3) / 1e9, 1),
                         "retained_nnz": int(info.retained_nnz)}
         outs[name] = C.cpu().numpy()
@@ -188,7 +189,8 @@ def bench_precision_131k(a):
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(head["achieved_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": "k_spmm", "kernel_ms": head["kernel_ms"],
-                     "alg_bytes": head["alg_bytes"], "a_packed_bytes": head["a_packed_bytes"]},
+                     "alg_bytes": head["alg_bytes"], "a_read_bytes": head["a_read_bytes"],
+                     "a_packed_bytes": head["a_packed_bytes"]},
         "reorder": {"method": "learned proposer + exact gains", "gpu_ms": round(st.ms, 2),
                     "init_cnt": st.init_cnt, "final_cnt": st.final_cnt,
                     "retained_nnz_before": retained0, "retained_nnz": head["retained_nnz"]},
@@ -296,7 +298,7 @@ def bench_batch64(a):
                                batch.info()["cols_per_lane"])
     retained = sum(i.retained_nnz for i in infos)
     flops = 2.0 * retained * NC
-    alg = sum(i.bytes + m.ncols * NC * 2 + m.nrows * NC * 4 for i, m in zip(infos, coos))
+    alg = sum(i.spmm_bytes + m.ncols * NC * 2 + m.nrows * NC * 4 for i, m in zip(infos, coos))
     achieved = alg / (k_ms * 1e-3) / 1e9
     res = {
         "metric": "V:N:M SpMM TFLOP/s (after GPU V:N:M reorder)",
@@ -418,7 +420,7 @@ def main():
     C32 = C.cpu().numpy()
     flops = 2.0 * info.retained_nnz * NCOLS
     tflops = flops / (ms_step * 1e-3) / 1e12
-    a_bytes = info.bytes
+    a_bytes = info.spmm_bytes
     alg_bytes = a_bytes + coo.ncols * NCOLS * 4 + coo.nrows * NCOLS * 4
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
 
@@ -427,7 +429,7 @@ def main():
     Bbf = Bd.to(torch.bfloat16)
     ms_bf, k_bf = run(pbf, Bbf, vnm.DT_BF16, a.steps)
     Cbf = C.cpu().numpy()
-    bf_bytes = pbf.info().bytes + coo.ncols * NCOLS * 2 + coo.nrows * NCOLS * 4
+    bf_bytes = pbf.info().spmm_bytes + coo.ncols * NCOLS * 2 + coo.nrows * NCOLS * 4
 
     # FP8 e4m3 A (per-block scales) and B (one scale per 128 rows)
     p8 = g.pack(V, vnm.DT_FP8)
@@ -461,7 +463,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"k_spmm<f32, f32, {vec32}>", "kernel_ms": round(k_ms, 5),
-                     "alg_bytes": int(alg_bytes), "a_packed_bytes": int(a_bytes)},
+                     "alg_bytes": int(alg_bytes), "a_read_bytes": int(a_bytes),
+                     "a_packed_bytes": int(info.bytes)},
         "reorder": {"gpu_ms": round(reorder_wall_ms, 2), "hybrid_ms": round(hs.milliseconds, 2),
                     "greedy_kernel_ms": round(greedy_ms, 2), "greedy_launches": greedy_n,
                     "nm_rounds": hs.nm_rounds, "vnm_rounds": hs.vnm_rounds,

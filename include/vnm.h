@@ -100,6 +100,8 @@ void vnm_gen_normal(uint64_t seed, int64_t count, float *out);
  * device memory of the graphs created on it.  No global or CWD state (the
  * reference shares ./temp.mtx and ./output.txt, main.cu:118-152). */
 int  vnm_ctx_create(int device, vnm_ctx **out);
+/* Destroy every graph, packed matrix, model and batch created on the
+ * context first: their destroy calls wait on the context's stream. */
 void vnm_ctx_destroy(vnm_ctx *ctx);
 /* The context's hipStream_t, for event timing by the caller. */
 void *vnm_ctx_stream(vnm_ctx *ctx);
@@ -250,6 +252,10 @@ typedef struct {
     int32_t dtype;
     int64_t retained_nnz;
     int64_t bytes;            /* device bytes of the packed format       */
+    int64_t spmm_bytes;       /* bytes of it one vnm_spmm reads: per entry
+                                 row, column, selector and two values
+                                 (plus, for FP8, the block index and the
+                                 block scales)                           */
 } vnm_packed_info;
 
 /* Replaces the cuSPARSELt prune/compress path (kernel/spmm.cu:402-447) and

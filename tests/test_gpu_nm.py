@@ -232,3 +232,20 @@ def test_greedy_rmat_fullsize_golden(ctx):
     for a in (out.row, out.col, out.val):
         h.update(np.ascontiguousarray(a).tobytes())
     assert h.hexdigest() == gold["sha256"]
+
+
+def test_context_outlives_a_garbage_cycle_of_its_graphs():
+    """Graphs kept only by a reference cycle are finalized by the cyclic
+    collector after their context was closed: the context's handle goes with
+    the last of them, never before one of them has destroyed its own."""
+    import gc
+    c = vnm.Context(0)
+    m = vnm.gen_er(256, 2000, 5)
+    a, b = vnm.Graph(c, m, 8), vnm.Graph(c, m, 8)
+    a.other, b.other = b, a  # a cycle: only the cyclic collector frees it
+    a.nm_reorder(2, 0)
+    c.close()
+    assert c.h is not None  # two graphs are still alive
+    del a, b
+    gc.collect()
+    assert c.h is None

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(256) void k_spmm_batch(const SpmmArgs *__restrict__
             else hi = mid - 1;
         }
         const int local = unit - unit_off[lo];
-        spmm_unit<AT, BT, VEC>(descs[lo], local / T, local % T);
+        spmm_unit<AT, BT, VEC, true>(descs[lo], local / T, local % T);
     }
 }
 

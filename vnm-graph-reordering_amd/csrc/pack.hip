@@ -715,6 +715,9 @@ int vnm_packed_info_get(const vnm_packed *p, vnm_packed_info *info) {
     const int blk = p->dtype == VNM_DT_FP8 ? 4 : 0;  // per-entry block index for the FP8 scale
     info->bytes = 4ll * (p->nrows + 1) + (int64_t)p->nent * (4 + 4 + 1 + 2 * vs + blk) +
                   (int64_t)p->nblocks * (4 + 2 + 4);
+    // what k_spmm reads: the entry arrays, and for FP8 A the scales of the
+    // blocks (rowptr is read at most twice per 64 entries)
+    info->spmm_bytes = (int64_t)p->nent * (4 + 4 + 1 + 2 * vs + blk) + (blk ? (int64_t)p->nblocks * 4 : 0);
     return VNM_OK;
 }
 

@@ -220,8 +220,10 @@ __device__ __forceinline__ float lane_bcast(float v, int k) {
 // B rows by buffer loads whose row offset is that broadcast scalar, and owns
 // VEC columns of the output row.  A row is flushed when the row changes;
 // atomically when an item boundary cuts it (such rows are pre-filled by the
-// zero list).
-template <int AT, int BT, int VEC>
+// zero list).  PIPE: two register sets of U entries, the gathers of the next
+// group in flight while the current one is folded in (the persistent batched
+// kernel runs faster this way; the one-matrix kernel does not).
+template <int AT, int BT, int VEC, bool PIPE = false>
 __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ erow,
                                           const int32_t *__restrict__ ecol, const uint8_t *__restrict__ eidx,
                                           const uint8_t *__restrict__ evals, const int32_t *__restrict__ eblk,
@@ -230,7 +232,10 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
                                           const float *__restrict__ bias, int beg, int cnt, int M, int ldb,
                                           int ldc, int b_rows, int tile) {
     constexpr int SB = bsize<BT>();
-    constexpr int U = BRaw<BT, VEC>::BYTES >= 16 ? 4 : 8;
+    // entries per group: up to 8; pipelined, 16 VGPRs of gathered values
+    // per register set
+    constexpr int NW = BRaw<BT, VEC>::NW;
+    constexpr int U = PIPE ? (NW >= 4 ? 2 : 8 / NW) : (BRaw<BT, VEC>::BYTES >= 16 ? 4 : 8);
     const int lane = threadIdx.x & 63;
     const int rowb = ldb * SB;
     const __amdgpu_buffer_rsrc_t rs =
@@ -298,18 +303,68 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
             if (o1[u] >= 0) fma_raw<BT, VEC>(acc, w[u][1], raw[u][1]);
         }
     };
-    const int full = cnt & ~(U - 1);
-    for (int k = 0; k < full; k += U) group(k, std::false_type{});
-    if (full < cnt) group(full, std::true_type{});
+    if constexpr (!PIPE) {
+        const int full = cnt & ~(U - 1);
+        for (int k = 0; k < full; k += U) group(k, std::false_type{});
+        if (full < cnt) group(full, std::true_type{});
+    } else {
+        // the same gathers and folds as group(), split so that a group's
+        // loads are issued before the previous group is folded in; entries
+        // past cnt gather nothing and fold zeros (their lanes hold zero
+        // weights and no second value)
+        using Raw = BRaw<BT, VEC>[U][2];
+        auto issue = [&](int k, Raw &raw, auto check) {
+            constexpr bool CHECK = decltype(check)::value;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int kk = k + u;  // wave-uniform
+                const bool ok = !CHECK || kk < cnt;
+                const int ks = kk & 63;
+                raw[u][0] = ok ? load_row<BT, VEC>(rs, voff, lane_bcast(off0, ks)) : zero_raw<BT, VEC>();
+                const int o1 = ok ? lane_bcast(off1, ks) : -1;
+                raw[u][1] = o1 >= 0 ? load_row<BT, VEC>(rs, voff, o1) : zero_raw<BT, VEC>();
+            }
+        };
+        auto issue_any = [&](int k, Raw &raw) {
+            if (k + U <= cnt) issue(k, raw, std::false_type{});
+            else issue(k, raw, std::true_type{});
+        };
+        auto fold = [&](int k, const Raw &raw) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int ks = (k + u) & 63;
+                if ((starts >> ks) & 1ull) {  // wave-uniform: a new row starts here
+                    const bool cut = (cur == first_row && rowptr[cur] < beg) ||
+                                     (cur == last_row && rowptr[cur + 1] > beg + cnt);
+                    flush_vec<VEC>(C, ldc, cur, col0, acc, cut, bias);
+#pragma unroll
+                    for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
+                    cur = lane_bcast(rr, ks);
+                }
+                fma_raw<BT, VEC>(acc, lane_bcast(a0, ks), raw[u][0]);
+                if (lane_bcast(off1, ks) >= 0) fma_raw<BT, VEC>(acc, lane_bcast(a1, ks), raw[u][1]);
+            }
+        };
+        Raw ra, rb;
+        issue_any(0, ra);
+        for (int k = 0;; k += 2 * U) {
+            if (k + U < cnt) issue_any(k + U, rb);
+            fold(k, ra);
+            if (k + U >= cnt) break;
+            if (k + 2 * U < cnt) issue_any(k + 2 * U, ra);
+            fold(k + U, rb);
+            if (k + 2 * U >= cnt) break;
+        }
+    }
     const bool cut = (cur == first_row && rowptr[cur] < beg) || (cur == last_row && rowptr[cur + 1] > beg + cnt);
     flush_vec<VEC>(C, ldc, cur, col0, acc, cut, bias);
 }
 
 // Work item `item` (wave-uniform), column tile `tile` of one packed matrix.
-template <int AT, int BT, int VEC>
+template <int AT, int BT, int VEC, bool PIPE = false>
 __device__ __forceinline__ void spmm_unit(const SpmmArgs &a, int item, int tile) {
     const int beg = item * ITEM;
-    spmm_item<AT, BT, VEC>(a.rowptr, a.ent_row, a.ent_col, a.ent_idx, a.ent_val, a.ent_blk,
+    spmm_item<AT, BT, VEC, PIPE>(a.rowptr, a.ent_row, a.ent_col, a.ent_idx, a.ent_val, a.ent_blk,
                            a.blk_scale, a.b_scales, a.B, a.C, a.bias, beg, min(ITEM, a.E - beg), a.M, a.ldb, a.ldc,
                            a.b_rows, tile);
 }

@@ -13,7 +13,6 @@ no CPU fallback.
 """
 import ctypes
 import os
-import weakref
 
 import numpy as np
 
@@ -107,7 +106,7 @@ class LPStats(ctypes.Structure):
 
 class PackedInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("V", "M", "nrows", "ncols", "nblocks", "dtype")] + \
-               [("retained_nnz", ctypes.c_int64), ("bytes", ctypes.c_int64)]
+               [("retained_nnz", ctypes.c_int64), ("bytes", ctypes.c_int64), ("spmm_bytes", ctypes.c_int64)]
 
 
 _lib = None
@@ -314,21 +313,25 @@ class Context:
 
     def __init__(self, device=0):
         self.h = None
-        self._children = weakref.WeakSet()
+        # live objects created on the context, counted rather than held in a
+        # weak set: the cyclic collector clears every weak reference of a
+        # garbage cycle before it runs the finalizers, so a weak set can look
+        # empty while objects of the cycle still have to destroy their handles
+        self._live = 0
         self._closing = False
         h = ctypes.c_void_p()
         _check(lib().vnm_ctx_create(device, ctypes.byref(h)))
         self.h = h
 
     def _adopt(self, obj):
-        self._children.add(obj)
+        self._live += 1
 
     def _release(self, obj):
-        self._children.discard(obj)
+        self._live -= 1
         self._maybe_destroy()
 
     def _maybe_destroy(self):
-        if self.h and self._closing and len(self._children) == 0:
+        if self.h and self._closing and self._live == 0:
             lib().vnm_ctx_destroy(self.h)
             self.h = None
 
@@ -345,9 +348,10 @@ class Context:
         self._maybe_destroy()
 
     def __del__(self):
+        # objects created on the context hold it, so none is alive here
+        # unless one was never closed (then the handle is left to process exit)
         try:
             self._closing = True
-            self._children = weakref.WeakSet()
             self._maybe_destroy()
         except Exception:
             pass
