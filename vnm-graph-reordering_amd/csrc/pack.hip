@@ -116,22 +116,88 @@ __global__ void k_blk_heads(const uint64_t *__restrict__ bk, int U, int32_t *__r
     if (i < U) head[i] = (i == 0) || ((bk[i] >> 32) != (bk[i - 1] >> 32));
 }
 
-// One wave per block: best 4-column subset (lexicographic order of ascending
-// column tuples; ties to the smallest).  nsub = C(M, 4) <= 70.
-__global__ void k_blk_select(const uint64_t *__restrict__ bk, const int32_t *__restrict__ rid_sorted,
-                             const int32_t *__restrict__ bhead_excl, const int32_t *__restrict__ bhead,
+// The selected subset's 4 column offsets (ascending), 4 bits each.
+__device__ __forceinline__ uint16_t subset_sel(uint32_t S, int M) {
+    uint16_t sel = 0;
+    int k = 0;
+    for (int c = 0; c < M && k < 4; ++c)
+        if (S & (1u << (M - 1 - c))) sel |= (uint16_t)(c << (4 * k++));
+    return sel;
+}
+
+constexpr int LANE_RUNS = 8;  // runs of a block that one lane scores
+
+// Block selection, one lane per block of at most LANE_RUNS runs (nearly all
+// blocks of sparse graphs): the run bitmaps stay in registers and the
+// subsets are scored in ascending index order with strict > (ties to the
+// smallest), the order k_blk_select's argmax yields.  Longer blocks are
+// listed for k_blk_select.
+__global__ __launch_bounds__(256) void k_blk_select_lane(
+    const uint64_t *__restrict__ bk, const int32_t *__restrict__ rid_sorted, int U, int nb, int M, int nblk,
+    const uint32_t *__restrict__ bits, const uint32_t *__restrict__ subs, int nsub,
+    const int32_t *__restrict__ blk_first, int32_t *__restrict__ blk_seg, uint16_t *__restrict__ blk_sel,
+    uint32_t *__restrict__ blk_mask, int32_t *__restrict__ run_blk, unsigned long long *__restrict__ retained,
+    int32_t *__restrict__ longb, int32_t *__restrict__ nlong) {
+    int kept = 0;
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) {
+        const int first = blk_first[b];
+        const int last = b + 1 < nb ? blk_first[b + 1] : U;
+        const int n = last - first;
+        if (n > LANE_RUNS) {
+            longb[atomicAdd(nlong, 1)] = b;
+        } else {
+            uint32_t r[LANE_RUNS];
+#pragma unroll
+            for (int j = 0; j < LANE_RUNS; ++j) r[j] = j < n ? bits[rid_sorted[first + j]] : 0u;
+            int best_score = -1, best_idx = 0;
+            for (int s = 0; s < nsub; ++s) {
+                const uint32_t S = subs[s];
+                int sc = 0;
+#pragma unroll
+                for (int j = 0; j < LANE_RUNS; ++j) sc += min(2, popc32(r[j] & S));
+                if (sc > best_score) {
+                    best_score = sc;
+                    best_idx = s;
+                }
+            }
+            const uint32_t S = subs[best_idx];
+            blk_seg[b] = (int32_t)((bk[first] >> 32) / (uint64_t)nblk);
+            blk_sel[b] = subset_sel(S, M);
+            blk_mask[b] = S;
+            kept += best_score;
+            for (int j = 0; j < n; ++j) run_blk[rid_sorted[first + j]] = b;
+        }
+    }
+    // one atomic per workgroup: atomics on a single address serialise at the
+    // L2, so the grid is capped and the workgroup sums first
+    __shared__ int wsum[4];
+    for (int d = 32; d >= 1; d >>= 1) kept += __shfl_xor(kept, d, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = kept;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (t) atomicAdd(retained, (unsigned long long)t);
+    }
+}
+
+// One wave per listed block: best 4-column subset (lexicographic order of
+// ascending column tuples; ties to the smallest).  nsub = C(M, 4) <= 70.
+__global__ __launch_bounds__(256) void k_blk_select(const uint64_t *__restrict__ bk, const int32_t *__restrict__ rid_sorted,
                              int U, int nb, int M, int nblk, const uint32_t *__restrict__ bits,
                              const uint32_t *__restrict__ subs, int nsub,
-                             int32_t *__restrict__ blk_first, int32_t *__restrict__ blk_seg,
+                             const int32_t *__restrict__ blk_first, int32_t *__restrict__ blk_seg,
                              uint16_t *__restrict__ blk_sel, uint32_t *__restrict__ blk_mask,
-                             int32_t *__restrict__ run_blk, unsigned long long *__restrict__ retained) {
+                             int32_t *__restrict__ run_blk, unsigned long long *__restrict__ retained,
+                             const int32_t *__restrict__ list, const int32_t *__restrict__ nlist) {
     const int lane = threadIdx.x & 63;
     const int wpb = blockDim.x / 64;
-    // blocks in a grid-stride loop per wave; the retained count is summed per
-    // wave and added once (one atomic per block on a single address
-    // serialises at the L2)
+    const int count = *nlist;
+    // listed blocks in a grid-stride loop per wave; the retained count is
+    // summed per wave and added once (one atomic per block on a single
+    // address serialises at the L2)
     unsigned long long kept = 0;
-    for (int b = blockIdx.x * wpb + (threadIdx.x >> 6); b < nb; b += gridDim.x * wpb) {
+    for (int i = blockIdx.x * wpb + (threadIdx.x >> 6); i < count; i += gridDim.x * wpb) {
+        const int b = list[i];
         const int first = blk_first[b];
         int last = U;
         if (b + 1 < nb) last = blk_first[b + 1];
@@ -167,18 +233,20 @@ __global__ void k_blk_select(const uint64_t *__restrict__ bk, const int32_t *__r
         if (lane == 0) {
             const uint64_t key = bk[first] >> 32;
             blk_seg[b] = (int32_t)(key / (uint64_t)nblk);
-            // 4 column offsets (ascending), 4 bits each
-            uint16_t sel = 0;
-            int k = 0;
-            for (int c = 0; c < M && k < 4; ++c)
-                if (S & (1u << (M - 1 - c))) sel |= (uint16_t)(c << (4 * k++));
-            blk_sel[b] = sel;
+            blk_sel[b] = subset_sel(S, M);
             blk_mask[b] = S;
             kept += (unsigned long long)best_score;
         }
         for (int q = first + lane; q < last; q += 64) run_blk[rid_sorted[q]] = b;
     }
-    if (lane == 0 && kept) atomicAdd(retained, kept);
+    // one atomic per workgroup (lane 0 of each wave holds the wave's sum)
+    __shared__ unsigned long long wsum[4];
+    if (lane == 0) wsum[threadIdx.x >> 6] = kept;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        if (t) atomicAdd(retained, t);
+    }
 }
 
 __global__ void k_blk_first(const int32_t *__restrict__ head, const int32_t *__restrict__ excl, int U,
@@ -226,14 +294,29 @@ __global__ void k_run_keep(const uint64_t *__restrict__ keys, const int32_t *__r
     nkeep[u] = n;
 }
 
+// Kept runs per row.  Runs are sorted by row, so a wave's lanes of one row
+// are contiguous: the last lane of each such segment adds the segment's
+// kept count (one atomic per row and wave; a hub row's thousands of runs
+// would otherwise serialise on one address).
 __global__ void k_row_counts(const uint64_t *__restrict__ keys, const int32_t *__restrict__ rstart,
                              const int32_t *__restrict__ nkeep, int U, int32_t *__restrict__ rcnt,
                              uint8_t *__restrict__ flag) {
-    int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= U) return;
-    const bool keep = nkeep[u] > 0;
-    flag[u] = keep;
-    if (keep) atomicAdd(&rcnt[(int)(keys[rstart[u]] >> 32)], 1);
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = u < U;
+    const bool keep = valid && nkeep[u] > 0;
+    const int row = valid ? (int)(keys[rstart[u]] >> 32) : -1;
+    if (valid) flag[u] = keep;
+    const uint64_t kmask = __ballot(keep);
+    const int prev = __shfl_up(row, 1, 64), next = __shfl_down(row, 1, 64);
+    const uint64_t heads = __ballot(lane == 0 || row != prev);
+    const bool tail = lane == 63 || row != next;
+    if (valid && tail) {
+        const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+        const int start = 63 - __builtin_clzll(heads & upto);
+        const int c = __popcll(kmask & upto & ~((1ull << start) - 1));
+        if (c) atomicAdd(&rcnt[row], c);
+    }
 }
 
 __global__ void k_iota_i(int32_t *p, int n) {
@@ -379,6 +462,7 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
     DBuf<float> kv;
     DBuf<uint8_t> kidx, flag;
     DBuf<unsigned long long> dret;
+    DBuf<int32_t> longb, nlong;
     if ((rc = keys.alloc(e1)) || (rc = keys2.alloc(e1)) || (rc = idx.alloc(e1)) ||
         (rc = idx2.alloc(e1)) || (rc = head.alloc(e1)) || (rc = excl.alloc(e1)) ||
         (rc = dret.alloc(1)) || (rc = cnt.alloc(4)))
@@ -451,9 +535,19 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
     if (nb > 0) {
         hipLaunchKernelGGL(k_blk_first, dim3(grid_of(U, 256)), dim3(256), 0, st, bhead.p, bexcl.p, U,
                            blk_first.p);
+        // short blocks one per lane; the rest one per wave, from a list the
+        // lane kernel fills (its length read on the device: no host wait)
+        if ((rc = longb.alloc(b1)) || (rc = nlong.alloc(1))) {
+            delete p;
+            return rc;
+        }
+        VNM_HIP_TRY(hipMemsetAsync(nlong.p, 0, 4, st));
+        hipLaunchKernelGGL(k_blk_select_lane, dim3(std::min(grid_of(nb, 256), 2048)), dim3(256), 0, st, bkey2.p, rid2.p, U, nb,
+                           M, nblk, bits.p, dsubs.p, (int)subs.size(), blk_first.p, p->blk_seg.p, p->blk_sel.p,
+                           blk_mask.p, run_blk.p, dret.p, longb.p, nlong.p);
         hipLaunchKernelGGL(k_blk_select, dim3(std::min((nb + 3) / 4, 4096)), dim3(256), 0, st, bkey2.p, rid2.p,
-                           bexcl.p, bhead.p, U, nb, M, nblk, bits.p, dsubs.p, (int)subs.size(),
-                           blk_first.p, p->blk_seg.p, p->blk_sel.p, blk_mask.p, run_blk.p, dret.p);
+                           U, nb, M, nblk, bits.p, dsubs.p, (int)subs.size(), blk_first.p, p->blk_seg.p,
+                           p->blk_sel.p, blk_mask.p, run_blk.p, dret.p, longb.p, nlong.p);
         VNM_HIP_TRY(hipMemsetAsync(absmax.p, 0, (size_t)nb * 4, st));
         hipLaunchKernelGGL(k_run_keep, dim3(grid_of(U, 256)), dim3(256), 0, st, keys2.p, idx2.p,
                            g->val.p, U, M, rstart.p, rlen.p, run_blk.p, blk_mask.p, kv.p, kidx.p,
@@ -497,7 +591,7 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
         hipLaunchKernelGGL(k_emit, dim3(grid_of(E, 256)), dim3(256), 0, st, sel_runs.p, E, run_blk.p,
                            keys2.p, rstart.p, kv.p, kidx.p, p->blk_scale.p, p->blk_seg.p, p->blk_sel.p,
                            M, dtype, p->ent_row.p, p->ent_col.p, p->ent_blk.p, p->ent_idx.p, p->ent_val.p);
-    g->ctx->launches += 16;
+    g->ctx->launches += 17;
     VNM_LAUNCH_CHECK("pack emit");
     VNM_HIP_TRY(hipStreamSynchronize(st));
     *out = p;
