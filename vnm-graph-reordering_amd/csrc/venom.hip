@@ -123,22 +123,35 @@ __global__ void k_cells_clear(const int32_t *__restrict__ rrow, const int32_t *_
 
 // Row token lists: rcnt[row] = runs of the row; tok[u] for run u.
 // hash (reordering.py:95-107): k with k ^ (k >> 1) == b (inverse Gray code,
-// the prefix XOR of b), negated when b has more than two ones.
-__global__ void k_tokens(const int32_t *__restrict__ rrow, const int32_t *__restrict__ rseg,
-                         const uint32_t *__restrict__ bits, int U, int S,
-                         int32_t *__restrict__ tok, int32_t *__restrict__ rcnt) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i < U; i += gridDim.x * blockDim.x) {
-        const uint32_t b = bits[i];
-        uint32_t k = b;
-        k ^= k >> 1;
-        k ^= k >> 2;
-        k ^= k >> 4;
-        k ^= k >> 8;
-        const int h = popc32(b) <= 2 ? (int)k : -(int)k;
-        const int span = S - rseg[i];
-        tok[i] = (h > 0 ? span : -span) * 512 + h + 256;
-        atomicAdd(&rcnt[rrow[i]], 1);
+// the prefix XOR of b), negated when b has more than two ones.  Runs are in
+// row order, so a wave's runs of one row are contiguous lanes: the last of
+// them adds the count (one atomic per row and wave instead of one per run,
+// which serialises on a hub row's counter).
+__global__ __launch_bounds__(256) void k_tokens(const int32_t *__restrict__ rrow, const int32_t *__restrict__ rseg,
+                                                const uint32_t *__restrict__ bits, int U, int S,
+                                                int32_t *__restrict__ tok, int32_t *__restrict__ rcnt) {
+    const int lane = threadIdx.x & 63;
+    for (int base = blockIdx.x * blockDim.x; base < U; base += gridDim.x * blockDim.x) {
+        const int i = base + threadIdx.x;
+        const bool valid = i < U;
+        const int row = valid ? rrow[i] : -1;
+        if (valid) {
+            const uint32_t b = bits[i];
+            uint32_t k = b;
+            k ^= k >> 1;
+            k ^= k >> 2;
+            k ^= k >> 4;
+            k ^= k >> 8;
+            const int h = popc32(b) <= 2 ? (int)k : -(int)k;
+            const int span = S - rseg[i];
+            tok[i] = (h > 0 ? span : -span) * 512 + h + 256;
+        }
+        const int prev = __shfl_up(row, 1, 64), next = __shfl_down(row, 1, 64);
+        const uint64_t heads = __ballot(lane == 0 || row != prev);
+        if (valid && (lane == 63 || row != next)) {
+            const int start = 63 - __builtin_clzll(heads & (lane == 63 ? ~0ull : ((2ull << lane) - 1)));
+            atomicAdd(&rcnt[row], lane - start + 1);
+        }
     }
 }
 
