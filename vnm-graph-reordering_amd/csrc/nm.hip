@@ -418,6 +418,7 @@ struct GreedyShared {
     uint64_t wtop[NW][KMAX + 1];
     uint64_t wbest[NW];
     uint64_t topk[KMAX + 1];
+    int selthr[KMAX + 1], nselthr;  // threads whose segments hold the round's top keys
     int64_t wkeys[NW][64];
 };
 
@@ -1008,6 +1009,12 @@ __global__ __launch_bounds__(GT) void k_gr_select(NMArgs a0, const NMArgs *__res
         __syncthreads();
     } else {
         const int k = min(M, Lr - 1) + 1;  // targets + the primary itself
+        // Top-k traversal keys in three steps.  Thread t scans the segments
+        // s = t (mod GT) and keeps only its largest key; keys are distinct
+        // (they hold s), so every key of the top k lies in a thread whose own
+        // maximum is among the k largest thread maxima.  Those threads'
+        // segments are then scored again, spread over the workgroup, and the
+        // top k of them is the top k of all.
         uint64_t tk[KMAX + 1];
 #pragma unroll
         for (int i = 0; i <= KMAX; ++i) tk[i] = 0;
@@ -1022,16 +1029,35 @@ __global__ __launch_bounds__(GT) void k_gr_select(NMArgs a0, const NMArgs *__res
                 if (c == 0) continue;
                 uint64_t lk = rev ? key_rev(s, c) : key_fwd(s, c);
                 best_l = lk > best_l ? lk : best_l;
-                uint64_t x = tfwd ? key_fwd(s, c) : key_rev(s, c);
-#pragma unroll
-                for (int i = 0; i <= KMAX; ++i) {
-                    uint64_t hi = tk[i] > x ? tk[i] : x;
-                    x = tk[i] > x ? x : tk[i];
-                    tk[i] = hi;
-                }
+                const uint64_t x = tfwd ? key_fwd(s, c) : key_rev(s, c);
+                tk[0] = x > tk[0] ? x : tk[0];
             }
         }
         uint64_t bl = block_max_u64(best_l, S);
+        block_topk(tk, k, S);  // the k largest thread maxima
+        if (tid == 0) {
+            int n = 0;
+            for (int i = 0; i < k; ++i)
+                if (S.topk[i] != 0) S.selthr[n++] = seg_of_key(S.topk[i], tfwd) % GT;
+            S.nselthr = n;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i <= KMAX; ++i) tk[i] = 0;
+        const int per = (nseg + GT - 1) / GT;  // segments per thread, at most
+        const int ncand = S.nselthr * per;
+        for (int j = tid; j < ncand; j += GT) {
+            const int s = S.selthr[j / per] + (j % per) * GT;
+            const int c = s < nseg ? ld_l2(&A.cnt[s]) : 0;
+            if (c == 0) continue;
+            uint64_t x = tfwd ? key_fwd(s, c) : key_rev(s, c);
+#pragma unroll
+            for (int i = 0; i <= KMAX; ++i) {
+                uint64_t hi = tk[i] > x ? tk[i] : x;
+                x = tk[i] > x ? x : tk[i];
+                tk[i] = hi;
+            }
+        }
         block_topk(tk, k, S);
         if (tid == 0) {
             int primary = seg_of_key(bl, !rev);
