@@ -46,7 +46,7 @@ def _rand(rng, R, dens, M, zero_frac=0.05):
 def test_retained_nnz_matches_oracle(ctx):
     rng = np.random.default_rng(41)
     cases = [(_rand(rng, int(rng.integers(1, 30)) * M, rng.uniform(0.02, 0.6), M), M)
-             for M in (4, 8) for _ in range(5)]
+             for M in (4, 8, 16) for _ in range(5)]
     cases.append((vnm.gen_er(2048, 41943, 1), 8))
     cases.append((vnm.gen_rmat(13, 60000, 3), 8))
     cases.append((vnm.gen_rmat(12, 40000, 4, M=4), 4))
@@ -107,7 +107,7 @@ def _run(ctx, m, M, V, n, adt, bdt, seed=5, vec=None):
 @pytest.mark.parametrize("n", [64, 128, 256])
 def test_spmm_fp32_exactish(ctx, n):
     rng = np.random.default_rng(n)
-    for M in (4, 8):
+    for M in (4, 8, 16):
         m = _rand(rng, 40 * M, 0.05, M)
         C, ref, info = _run(ctx, m, M, 16, n, vnm.DT_F32, vnm.DT_F32)
         assert info.retained_nnz == oracle.retained_nnz(_om(m), 16, 2, M)
