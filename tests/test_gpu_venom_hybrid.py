@@ -223,3 +223,30 @@ def test_rectangular_hybrid_venom_and_pack(ctx):
         packs.add(over)
         g2.close()
     assert hyb == {True, False} and packs == {True, False}
+
+
+def test_venom_m16_matches_oracle(ctx):
+    """M = 16: 16-bit run bitmaps, the strict flag above them and row tokens
+    scaled by 2^17; counts, stats and positions are the oracle's."""
+    rng = np.random.default_rng(61)
+    for t in range(8):
+        R = int(rng.integers(2, 12)) * 16
+        m = _rand(rng, R, rng.uniform(0.02, 0.3), 16, sym=t % 2 == 0)
+        for V in (1, 4, 16, 64):
+            g = vnm.Graph(ctx, m, 16)
+            assert g.venom_eval(V) == oracle.venom_eval(_om(m), 16, V), (t, V)
+            st = g.venom_reorder(V, 10)
+            out = g.download()
+            om = _om(m).copy()
+            ost = oracle.venom_reorder(om, 16, V, 10)
+            oracle.reload(om, 16)
+            assert st == ost.tolist(), (t, V)
+            assert np.array_equal(out.row, om.row) and np.array_equal(out.col, om.col)
+            assert np.array_equal(out.val, om.val)
+            g.close()
+    # 32-bit row tokens bound the segment count at 16,382
+    n = 16 * 16383
+    g = vnm.Graph(ctx, vnm.coo_from_tuples(n, n, [0], [1], np.ones(1, np.float32), 16), 16)
+    with pytest.raises(vnm.VnmError):
+        g.venom_reorder(4, 10)
+    g.close()

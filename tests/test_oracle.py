@@ -436,3 +436,20 @@ def test_reload_refuses_rows_past_the_written_shape():
         except RuntimeError:
             hy.add(False)
     assert hy == {True, False}
+
+
+def test_venom_m16_matches_literal():
+    """M = 16: the oracle's VENOM against the literal restatement of
+    reordering.py (16-bit Gray-code hashes, the same literal 4 and 2)."""
+    rng = np.random.default_rng(16)
+    for t in range(8):
+        R = int(rng.integers(2, 8)) * 16
+        mask = rng.random((R, R)) < rng.uniform(0.02, 0.3)
+        r, c = np.nonzero(mask)
+        m = oracle.from_tuples(R, R, r, c, rng.uniform(-1, 1, r.size), 16)
+        V = int(rng.choice([1, 4, 16]))
+        want = L.venom_reorder(m.nrows, m.ncols, m.row.tolist(), m.col.tolist(), 16, V)
+        om = m.copy()
+        got = oracle.venom_reorder(om, 16, V)
+        assert got.tolist() == want[0], t
+        assert om.row.tolist() == want[1] and om.col.tolist() == want[2]

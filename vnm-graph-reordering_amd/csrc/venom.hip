@@ -9,7 +9,8 @@
 //      ORs its bitmap (and a "row has > 2 ones" flag) into the cell
 //      (segment, row / V); the first run to visit a cell afterwards counts it;
 //   3. the stable row sort by the dense hash lists (:162-169, :219): tokens
-//      tok = sign(h) * (S - seg) * 512 + h + 256 make the dense lexicographic
+//      tok = sign(h) * (S - seg) * scale + h + scale / 2 (scale 512 for
+//      M <= 8, 2^17 for M = 16, so |h| < scale / 2) make the dense lexicographic
 //      order (absent segments hash to 0) a plain lexicographic order of the
 //      rows' token lists with an end marker 0; rows are then sorted by bucket
 //      refinement: at level k every row still tied with another is stably
@@ -86,7 +87,7 @@ __global__ void k_cells_or(const int32_t *__restrict__ rrow, const int32_t *__re
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     for (; i < U; i += gridDim.x * blockDim.x) {
         const uint32_t b = bits[i];
-        const uint32_t flag = popc32(b) > 2 ? 0x100u : 0u;  // strict: a row with > 2 ones
+        const uint32_t flag = popc32(b) > 2 ? 0x10000u : 0u;  // strict: a row with > 2 ones (bitmaps <= 16 bits)
         atomicOr(&cells[(size_t)rseg[i] * nblk + rrow[i] / V], b | flag);
     }
 }
@@ -99,9 +100,9 @@ __global__ void k_cells_count(const int32_t *__restrict__ rrow, const int32_t *_
     for (; i < U; i += gridDim.x * blockDim.x) {
         const uint32_t old = atomicOr(&cells[(size_t)rseg[i] * nblk + rrow[i] / V], 0x80000000u);
         if (!(old & 0x80000000u)) {  // first visitor counts the cell
-            const bool inv = popc32(old & 0xFFu) > 4;
+            const bool inv = popc32(old & 0xFFFFu) > 4;
             a += inv;
-            b += inv || (old & 0x100u);
+            b += inv || (old & 0x10000u);
         }
     }
     // wave then block reduction
@@ -128,7 +129,7 @@ __global__ void k_cells_clear(const int32_t *__restrict__ rrow, const int32_t *_
 // them adds the count (one atomic per row and wave instead of one per run,
 // which serialises on a hub row's counter).
 __global__ __launch_bounds__(256) void k_tokens(const int32_t *__restrict__ rrow, const int32_t *__restrict__ rseg,
-                                                const uint32_t *__restrict__ bits, int U, int S,
+                                                const uint32_t *__restrict__ bits, int U, int S, int scale,
                                                 int32_t *__restrict__ tok, int32_t *__restrict__ rcnt) {
     const int lane = threadIdx.x & 63;
     for (int base = blockIdx.x * blockDim.x; base < U; base += gridDim.x * blockDim.x) {
@@ -144,7 +145,9 @@ __global__ __launch_bounds__(256) void k_tokens(const int32_t *__restrict__ rrow
             k ^= k >> 8;
             const int h = popc32(b) <= 2 ? (int)k : -(int)k;
             const int span = S - rseg[i];
-            tok[i] = (h > 0 ? span : -span) * 512 + h + 256;
+            // |h| < scale / 2: the hash orders rows at one span, the
+            // signed span orders a present segment against an absent one
+            tok[i] = (h > 0 ? span : -span) * scale + h + scale / 2;
         }
         const int prev = __shfl_up(row, 1, 64), next = __shfl_down(row, 1, 64);
         const uint64_t heads = __ballot(lane == 0 || row != prev);
@@ -328,7 +331,7 @@ struct VenomWork {
         VNM_HIP_TRY(hipMemsetAsync(rcnt.p, 0, (size_t)(m + 1) * 4, st));
         if (U > 0)
             hipLaunchKernelGGL(k_tokens, dim3(grid_of(U, 256)), dim3(256), 0, st, rrow.p, rseg.p,
-                               bits.p, U, S, tok.p, rcnt.p);
+                               bits.p, U, S, M <= 8 ? 512 : (1 << 17), tok.p, rcnt.p);
         if ((rc = scan_excl_i32(rcnt.p, rptr.p, m + 1, st))) return rc;
         hipLaunchKernelGGL(k_ref_init, dim3((m + 255) / 256), dim3(256), 0, st, order.p, bstart.p,
                            act.p, m);
@@ -481,7 +484,11 @@ int reload_sort(vnm_graph *g, bool quantize) {
 
 static int check_venom_args(vnm_graph *g, int V) {
     if (V <= 0) return set_error(VNM_E_ARG, "V must be positive (got %d)", V);
-    if (g->M != 4 && g->M != 8) return set_error(VNM_E_ARG, "VENOM on the GPU supports M = 4 or 8");
+    if (g->M != 4 && g->M != 8 && g->M != 16)
+        return set_error(VNM_E_ARG, "VENOM on the GPU supports M = 4, 8 or 16");
+    // row tokens are (span * scale + hash) in 32 bits: scale 2^17 for M = 16
+    if (g->M == 16 && (g->ncols + 15) / 16 >= (1 << 14) - 1)
+        return set_error(VNM_E_ARG, "VENOM with M = 16 supports at most %d columns", ((1 << 14) - 2) * 16);
     return VNM_OK;
 }
 
