@@ -63,16 +63,35 @@ __device__ __forceinline__ uint32_t hget(const HeapRef &H, int slot) { return H.
 __device__ __forceinline__ void hset(HeapRef &H, int slot, uint32_t v) { H.h[slot * H.stride] = (uint16_t)v; }
 __device__ __forceinline__ bool heap_le(uint32_t a, uint32_t b) { return (a >> 8) <= (b >> 8); }
 
+constexpr int HEAP_DEPTH = 6;  // heaps of at most 64 entries (M <= 8)
+
+// __push_heap(first, hole, top, value): the hole moves up while the parent
+// p satisfies Comparator(p, value).  The path hole, (hole - 1) / 2, ... is
+// fixed, so every ancestor on it is read at once (independent LDS reads
+// instead of one dependent read per level); the passed ancestors then move
+// down one slot and the value takes the last hole.  The slots read are above
+// every slot written, so reading first changes nothing.
 __device__ __forceinline__ void heap_sift_up(HeapRef &H, int hole, int top, uint32_t value) {
-    int parent = (hole - 1) / 2;
-    while (hole > top) {
-        const uint32_t pv = hget(H, parent);
-        if (!heap_le(pv, value)) break;
-        hset(H, hole, pv);
-        hole = parent;
-        parent = (hole - 1) / 2;
+    int pos[HEAP_DEPTH + 1];
+    uint32_t anc[HEAP_DEPTH];
+    pos[0] = hole;
+#pragma unroll
+    for (int j = 0; j < HEAP_DEPTH; ++j) {
+        const bool up = pos[j] > top;
+        pos[j + 1] = up ? (pos[j] - 1) / 2 : pos[j];
+        anc[j] = up ? hget(H, pos[j + 1]) : 0u;
     }
-    hset(H, hole, value);
+    int t = 0;
+    bool go = true;
+#pragma unroll
+    for (int j = 0; j < HEAP_DEPTH; ++j) {
+        go = go && pos[j] > top && heap_le(anc[j], value);
+        if (go) {
+            hset(H, pos[j], anc[j]);
+            t = j + 1;
+        }
+    }
+    hset(H, pos[t], value);
 }
 
 __device__ __forceinline__ void heap_push(HeapRef &H, uint32_t value) {
