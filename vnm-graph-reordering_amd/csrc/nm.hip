@@ -1163,8 +1163,12 @@ __global__ __launch_bounds__(GT) void k_gr_greedy(NMArgs a0, const NMArgs *__res
             }
             __builtin_amdgcn_wave_barrier();
             int rnk = 0;
-            if (lane < MM)
+            if (lane < MM) {
+                // independent broadcast reads: unrolled so they issue back to
+                // back instead of one LDS round trip per key
+#pragma unroll 16
                 for (int j = 0; j < MM; ++j) rnk += S.wkeys[w][j] < key;
+            }
             // the push order is element order: stage entry e in slot e
             if (lane < MM) S.heap[lane][q] = (uint16_t)((rnk << 8) | lane);
             __builtin_amdgcn_wave_barrier();
