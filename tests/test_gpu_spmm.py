@@ -9,6 +9,9 @@ max|dC| / max|C|):
   A FP8,  B FP8  : 1e-1   (e4m3, per-block / per-128-row scales)
   A FP16, B FP32 : 2e-3   (fp16 rounding of the values: 2^-11 relative)
   A FP16, B BF16 : 1e-2   (bf16 rounding of B dominates)
+The product is also bit-identical across runs and launch shapes: every
+output value is one fixed sequence of FP32 operations (rows longer than a
+work item are summed from per-chunk slots in item order; no atomics).
 """
 import os
 
@@ -205,3 +208,43 @@ def test_fp8_decode_every_code(ctx):
         p.set_shape(vnm.DT_FP8, n, v)
         p.spmm(Bd.data_ptr(), n, n, vnm.DT_FP8, C.data_ptr(), n, sc.data_ptr())
         assert torch.equal(C, ref), v
+
+
+def _bits(C):
+    return C.view(torch.int32).cpu().numpy()
+
+
+@pytest.mark.parametrize("adt,bdt,n", [(vnm.DT_F32, vnm.DT_F32, 256), (vnm.DT_BF16, vnm.DT_BF16, 256),
+                                       (vnm.DT_FP8, vnm.DT_FP8, 128), (vnm.DT_F16, vnm.DT_F32, 128)])
+def test_spmm_bit_identical_across_runs_and_shapes(ctx, adt, bdt, n):
+    # hub rows of up to ~600 row entries: cut into many work items
+    m = vnm.gen_rmat(14, 200000, 12)
+    g = vnm.Graph(ctx, m, 8)
+    p = g.pack(128, adt)
+    B = vnm.gen_normal(13, (m.ncols, n))
+    dev = torch.device("cuda:0")
+    sc = None
+    if bdt == vnm.DT_F32:
+        Bd = torch.from_numpy(B).to(dev)
+    elif bdt == vnm.DT_BF16:
+        Bd = torch.from_numpy(B).to(torch.bfloat16).to(dev)
+    else:
+        q, s_ = _e4m3_rows(B)
+        Bd, sc = q.to(dev), torch.from_numpy(s_).to(dev)
+    first = None
+    for v in _vecs(bdt, n):
+        p.set_shape(bdt, n, v)
+        for rep in range(2):
+            C = torch.full((m.nrows, n), float("nan"), dtype=torch.float32, device=dev)
+            p.spmm(Bd.data_ptr(), n, n, bdt, C.data_ptr(), n, sc.data_ptr() if sc is not None else None)
+            torch.cuda.synchronize()
+            b = _bits(C)
+            if first is None:
+                first = b
+                ref = oracle.spmm_vnm_f64(_om(m), 128, 8, B)
+                fro, mx = _errs(C.cpu().numpy(), ref)
+                tol = {vnm.DT_F32: 1e-5, vnm.DT_BF16: 1e-2, vnm.DT_FP8: 1e-1}[bdt]
+                tol = 2e-3 if adt == vnm.DT_F16 else tol
+                assert fro < tol and mx < tol, (fro, mx)
+            else:
+                assert np.array_equal(b, first), (v, rep)

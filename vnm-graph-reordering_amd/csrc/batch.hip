@@ -6,14 +6,14 @@
 // CU walks it, each XCD taking a contiguous range so neighbouring items of
 // one matrix share that XCD's L2.  A device table holds every matrix's
 // pointers (SpmmArgs).  The batch is tuned once (values per lane, timed on the
-// whole batch) and then captured as a HIP graph of two kernels -- the pre-fill
-// of rows that keep nothing or are cut by an item boundary, and the SpMM --
-// replayed by vnm_batch_run.  Pre-fill is a kernel, not a memset node.
+// whole batch) and then captured as a HIP graph of two kernels -- the SpMM
+// and the fix-up pass (rows that keep nothing, rows cut into chunks whose
+// partial sums are added in item order) -- replayed by vnm_batch_run.
 //
 // GCN layer (the consumer of gnn/single/pyg_gcn.py:41-74, aggr='add'):
 // Y = A_vnm (X W) + bias.  X W runs on MFMA (BF16 inputs, FP32 accumulation,
 // BF16 result into a workspace kept with the packed matrix), then the SpMM
-// with the bias folded into its row flush and pre-fill.
+// with the bias folded into its row stores and fix-up pass.
 #include "graph.h"
 #include "hip_common.h"
 #include "packed.h"
@@ -25,10 +25,11 @@
 struct vnm_batch {
     vnm_ctx *ctx = nullptr;
     int count = 0, n = 0, bt = 0, at = 0, vec = 0, T = 1;
-    int total_units = 0, nzero = 0, grid = 0, per_xcd = 0, waves_per_xcd = 0;
+    int total_units = 0, nfix = 0, grid = 0, per_xcd = 0, waves_per_xcd = 0;
     vnm::DBuf<vnm::SpmmArgs> descs;
     vnm::DBuf<int32_t> unit_off;
-    vnm::DBuf<int2> zero;
+    vnm::DBuf<int4> fix;          // {matrix, row, first slot, slots}
+    vnm::DBuf<float> part;        // partial-sum slots of every matrix
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~vnm_batch() {
@@ -61,21 +62,19 @@ __global__ __launch_bounds__(256) void k_spmm_batch(const SpmmArgs *__restrict__
     }
 }
 
-// Pre-fill the listed (matrix, row) pairs with the matrix's bias or zero.
-__global__ void k_zero_batch(const int2 *__restrict__ list, int nz, int n, const SpmmArgs *__restrict__ descs) {
+// Fix-up of the listed (matrix, row) pairs: rows that keep nothing get the
+// matrix's bias or zero, cut rows the sum of their slots in item order.
+__global__ void k_fix_batch(const int4 *__restrict__ list, int nf, int n, const SpmmArgs *__restrict__ descs) {
     const int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (w >= nz) return;
-    const int lane = threadIdx.x & 63;
-    const int2 e = list[w];
+    if (w >= nf) return;
+    const int4 e = list[w];
     const SpmmArgs &a = descs[e.x];
-    float *cp = a.C + (size_t)e.y * a.ldc;
-    for (int c = lane * 4; c < n; c += 256)
-        *(float4 *)(cp + c) = a.bias ? *(const float4 *)(a.bias + c) : make_float4(0.f, 0.f, 0.f, 0.f);
+    fix_row(a.C + (size_t)e.y * a.ldc, a.part, a.pn, e.z, e.w, n, a.bias);
 }
 
-__global__ void k_tag_rows(const int32_t *__restrict__ rows, int nz, int g, int2 *__restrict__ out) {
+__global__ void k_tag_fix(const int4 *__restrict__ fix, int nf, int g, int4 *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nz) out[i] = make_int2(g, rows[i]);
+    if (i < nf) out[i] = make_int4(g, fix[i].x, fix[i].y, fix[i].z);
 }
 
 template <int AT, int BT, int VEC>
@@ -120,9 +119,6 @@ static void batch_shape(vnm_batch *b, const std::vector<int> &items, int vec, st
 }
 
 static int batch_enqueue(vnm_batch *b, hipStream_t st) {
-    if (b->nzero > 0)
-        hipLaunchKernelGGL(k_zero_batch, dim3((b->nzero + 3) / 4), dim3(256), 0, st, b->zero.p, b->nzero, b->n,
-                           b->descs.p);
     if (b->total_units > 0) {
         switch (b->at) {
             case VNM_DT_F32: launch_batch_a<VNM_DT_F32>(b, st); break;
@@ -131,6 +127,9 @@ static int batch_enqueue(vnm_batch *b, hipStream_t st) {
             default: launch_batch_a<VNM_DT_FP8>(b, st); break;
         }
     }
+    if (b->nfix > 0)
+        hipLaunchKernelGGL(k_fix_batch, dim3((b->nfix + 3) / 4), dim3(256), 0, st, b->fix.p, b->nfix, b->n,
+                           b->descs.p);
     VNM_LAUNCH_CHECK("k_spmm_batch");
     return VNM_OK;
 }
@@ -165,26 +164,27 @@ static int batch_create(vnm_ctx *ctx, vnm_packed *const *ps, int count, const vo
     b->at = at;
     std::vector<SpmmArgs> d(count);
     std::vector<int> items(count);
+    std::vector<size_t> part_off(count + 1, 0);
+    std::vector<int> fix_off(count + 1, 0);
     int rc;
-    std::vector<std::pair<ZeroList *, int>> zl;
-    int nz = 0;
     for (int g = 0; g < count; ++g) {
-        d[g] = packed_args(ps[g], B[g], ldb[g], bs ? bs[g] : nullptr, C[g], ldc[g], bias ? bias[g] : nullptr);
-        items[g] = (ps[g]->nent + ITEM - 1) / ITEM;
-        ZeroList *z;
-        if ((rc = zero_list(ps[g], ITEM, &z))) return rc;
-        zl.push_back({z, nz});
-        nz += z->n;
+        items[g] = ps[g]->nitems;
+        part_off[g + 1] = part_off[g] + (size_t)ps[g]->nslots * n;
+        fix_off[g + 1] = fix_off[g] + ps[g]->nfix;
     }
-    b->nzero = nz;
-    if ((rc = b->descs.alloc(count)) || (rc = b->unit_off.alloc(count + 1)) || (rc = b->zero.alloc(std::max(nz, 1))))
+    b->nfix = fix_off[count];
+    if ((rc = b->descs.alloc(count)) || (rc = b->unit_off.alloc(count + 1)) ||
+        (rc = b->fix.alloc(std::max(b->nfix, 1))) || (rc = b->part.alloc(std::max<size_t>(part_off[count], 1))))
         return rc;
+    for (int g = 0; g < count; ++g)
+        d[g] = packed_args(ps[g], B[g], ldb[g], bs ? bs[g] : nullptr, C[g], ldc[g], bias ? bias[g] : nullptr,
+                           b->part.p + part_off[g], n);
     VNM_HIP_TRY(hipMemcpyAsync(b->descs.p, d.data(), count * sizeof(SpmmArgs), hipMemcpyHostToDevice, st));
     for (int g = 0; g < count; ++g)
-        if (zl[g].first->n)
-            hipLaunchKernelGGL(k_tag_rows, dim3((zl[g].first->n + 255) / 256), dim3(256), 0, st, zl[g].first->rows.p,
-                               zl[g].first->n, g, b->zero.p + zl[g].second);
-    VNM_LAUNCH_CHECK("k_tag_rows");
+        if (ps[g]->nfix)
+            hipLaunchKernelGGL(k_tag_fix, dim3((ps[g]->nfix + 255) / 256), dim3(256), 0, st, ps[g]->fix.p,
+                               ps[g]->nfix, g, b->fix.p + fix_off[g]);
+    VNM_LAUNCH_CHECK("k_tag_fix");
     std::vector<int32_t> off;
     if (vec == 0) {
         // tuner: each usable shape, one warm-up and three timed batches
@@ -334,7 +334,7 @@ int vnm_batch_info(const vnm_batch *b, int32_t *cols_per_lane, int32_t *units, i
     if (!b) return set_error(VNM_E_ARG, "null batch");
     if (cols_per_lane) *cols_per_lane = b->vec;
     if (units) *units = b->total_units;
-    if (zero_rows) *zero_rows = b->nzero;
+    if (zero_rows) *zero_rows = b->nfix;
     return VNM_OK;
 }
 

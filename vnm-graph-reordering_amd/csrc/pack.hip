@@ -22,20 +22,20 @@
 // SpMM: C[R x n] = A_vnm B, FP32 accumulation.  Row entries also carry the
 // absolute column of their first value (ent_col) and the in-segment offset of
 // the second (low nibble of ent_idx, 0x10 = present), so the SpMM reads no
-// block data (except the block scale of FP8 A).  A work item is 64
-// consecutive row entries (rows may start or end inside it); a wave takes one
-// item for one column tile of 64 * VEC columns: lane k first resolves entry k
-// (its two B rows and scaled values), then the wave walks the entries with the
-// entry data broadcast by scalar reads, every lane gathering VEC consecutive
-// values of each B row (2 to 16 bytes) four entries deep and owning VEC
-// columns of the output row, which is flushed when the row changes.  Rows
-// wholly inside an item are stored once; a row cut by an item boundary adds
-// into C with FP32 atomics (those rows, and rows that keep nothing, are zeroed
-// first from a list built on first use).  When the tile count T divides 8,
-// tile t runs on its own 8/T XCDs, so each XCD's L2 serves one slice of B.
-// VEC is chosen per (B dtype, n) by timing the usable values on first use
-// (vnm_spmm_set_shape overrides).  FP8 B is decoded by the gfx950 e4m3
-// conversion instruction.
+// block data (except the block scale of FP8 A).  A work item holds whole rows
+// of at most 64 row entries, or a 64-entry chunk of a longer row (item table
+// built once per packed matrix); a wave takes one item for one column tile of
+// 64 * VEC columns: lane k first resolves entry k (its two B rows and scaled
+// values), then the wave walks the entries with the entry data broadcast by
+// scalar reads, every lane gathering VEC consecutive values of each B row (2
+// to 16 bytes) and owning VEC columns of the output row, which is stored when
+// the row changes.  The chunks of a long row store partial sums into slots,
+// and a fix-up pass adds each cut row's slots in item order (and writes the
+// rows that keep nothing), so C is bit-identical for every launch shape and
+// run: no atomics.  When the tile count T divides 8, tile t runs on its own
+// 8/T XCDs, so each XCD's L2 serves one slice of B.  VEC is chosen per (B
+// dtype, n) by timing the usable values on first use (vnm_spmm_set_shape
+// overrides).  FP8 B is decoded by the gfx950 e4m3 conversion instruction.
 #include "graph.h"
 #include "hip_common.h"
 #include "packed.h"
@@ -369,23 +369,6 @@ __global__ void k_emit(const int32_t *__restrict__ sel_runs, int E, const int32_
     }
 }
 
-// Rows the SpMM must zero before accumulating: rows that keep nothing and
-// rows cut by a boundary of gsz-entry groups.
-__global__ void k_zero_flags(const int32_t *__restrict__ rowptr, int R, int gsz, int32_t *__restrict__ flag) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= R) return;
-    const int b = rowptr[r], e = rowptr[r + 1];
-    const bool empty = e == b;
-    const bool cut = !empty && (b / gsz != (e - 1) / gsz);
-    flag[r] = empty || cut;
-}
-
-__global__ void k_zero_list(const int32_t *__restrict__ flag, const int32_t *__restrict__ excl, int R,
-                            int32_t *__restrict__ list) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < R && flag[r]) list[excl[r]] = r;
-}
-
 // ------------------------------------------------------------ SpMM
 
 // ------------------------------------------------------------ SpMM kernel
@@ -411,16 +394,14 @@ __global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int T, int items, int 
     spmm_unit<AT, BT, VEC>(a, item, __builtin_amdgcn_readfirstlane(tile));
 }
 
-// Pre-fill the rows of a zero list (rows that keep nothing, rows cut by an
-// item boundary) with the bias or zero: one wave per row, float4 stores.
-__global__ void k_zero_rows(const int32_t *__restrict__ list, int nz, int n, float *__restrict__ C, int ldc,
-                            const float *__restrict__ bias) {
+// Fix-up pass: rows that keep nothing get the bias or zero, rows cut into
+// chunks the sum of their partial-sum slots in item order (one wave each).
+__global__ void k_spmm_fix(const int4 *__restrict__ fix, int nfix, const float *__restrict__ part, int pn, int n,
+                           float *__restrict__ C, int ldc, const float *__restrict__ bias) {
     const int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (w >= nz) return;
-    const int lane = threadIdx.x & 63;
-    float *cp = C + (size_t)list[w] * ldc;
-    for (int c = lane * 4; c < n; c += 256)
-        *(float4 *)(cp + c) = bias ? *(const float4 *)(bias + c) : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (w >= nfix) return;
+    const int4 f = fix[w];
+    fix_row(C + (size_t)f.x * ldc, part, pn, f.y, f.z, n, bias);
 }
 
 // ------------------------------------------------------------ host
@@ -438,6 +419,66 @@ static std::vector<uint32_t> subsets4(int M) {
                     out.push_back((1u << (M - 1 - a)) | (1u << (M - 1 - b)) | (1u << (M - 1 - c)) |
                                   (1u << (M - 1 - d)));
     return out;
+}
+
+// SpMM work items from the row pointers (host pass, once per packed matrix):
+// consecutive whole rows while they fit in ITEM entries; a row longer than
+// ITEM starts a new item and is cut into ITEM-entry chunks, each with a
+// partial-sum slot of its own (consecutive slots), the last chunk staying
+// open for the rows that follow.  So an item's only cut row is its first.
+// The fix-up list holds the rows that keep nothing (no slot) and the cut
+// rows with their slot range.
+static int build_items(vnm_packed *p) {
+    hipStream_t st = p->ctx->stream;
+    const int R = p->nrows;
+    std::vector<int32_t> rp(R + 1);
+    VNM_HIP_TRY(hipMemcpyAsync(rp.data(), p->rowptr.p, (size_t)(R + 1) * 4, hipMemcpyDeviceToHost, st));
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    std::vector<int32_t> beg, slot;
+    std::vector<int4> fix;
+    int cur_len = 0, nslots = 0;
+    bool open = false;
+    for (int r = 0; r < R; ++r) {
+        const int b = rp[r], L = rp[r + 1] - b;
+        if (L == 0) {
+            fix.push_back(make_int4(r, 0, 0, 0));
+            continue;
+        }
+        if (L <= ITEM) {
+            if (open && cur_len + L > ITEM) open = false;
+            if (!open) {
+                beg.push_back(b);
+                slot.push_back(-1);
+                open = true;
+                cur_len = 0;
+            }
+            cur_len += L;
+        } else {
+            const int nch = (L + ITEM - 1) / ITEM;
+            fix.push_back(make_int4(r, nslots, nch, 0));
+            for (int c = 0; c < nch; ++c) {
+                beg.push_back(b + c * ITEM);
+                slot.push_back(nslots++);
+            }
+            open = true;
+            cur_len = L - (nch - 1) * ITEM;
+        }
+    }
+    beg.push_back(rp[R]);
+    p->nitems = (int)slot.size();
+    p->nslots = nslots;
+    p->nfix = (int)fix.size();
+    int rc;
+    if ((rc = p->item_beg.alloc(beg.size())) || (rc = p->item_slot.alloc(std::max<size_t>(slot.size(), 1))) ||
+        (rc = p->fix.alloc(std::max<size_t>(fix.size(), 1))))
+        return rc;
+    VNM_HIP_TRY(hipMemcpyAsync(p->item_beg.p, beg.data(), beg.size() * 4, hipMemcpyHostToDevice, st));
+    if (!slot.empty())
+        VNM_HIP_TRY(hipMemcpyAsync(p->item_slot.p, slot.data(), slot.size() * 4, hipMemcpyHostToDevice, st));
+    if (!fix.empty())
+        VNM_HIP_TRY(hipMemcpyAsync(p->fix.p, fix.data(), fix.size() * sizeof(int4), hipMemcpyHostToDevice, st));
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    return VNM_OK;
 }
 
 int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_only) {
@@ -593,37 +634,17 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
                            M, dtype, p->ent_row.p, p->ent_col.p, p->ent_blk.p, p->ent_idx.p, p->ent_val.p);
     g->ctx->launches += 17;
     VNM_LAUNCH_CHECK("pack emit");
-    VNM_HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = build_items(p))) { delete p; return rc; }
     *out = p;
     return VNM_OK;
 }
 
-// Rows to zero for groups of gsz entries, built on first use.
-int zero_list(vnm_packed *p, int gsz, ZeroList **out) {
-    auto it = p->zero.find(gsz);
-    if (it != p->zero.end()) {
-        *out = it->second.get();
-        return VNM_OK;
-    }
-    hipStream_t st = p->ctx->stream;
-    const int R = p->nrows;
-    auto z = std::make_unique<ZeroList>();
-    DBuf<int32_t> zflag, zexcl;
-    int rc;
-    if ((rc = zflag.alloc(R + 1)) || (rc = zexcl.alloc(R + 1))) return rc;
-    VNM_HIP_TRY(hipMemsetAsync(zflag.p, 0, (size_t)(R + 1) * 4, st));
-    hipLaunchKernelGGL(k_zero_flags, dim3(grid_of(R, 256)), dim3(256), 0, st, p->rowptr.p, R, gsz, zflag.p);
-    if ((rc = scan_excl_i32(zflag.p, zexcl.p, R + 1, st))) return rc;
-    VNM_HIP_TRY(hipMemcpyAsync(&z->n, zexcl.p + R, 4, hipMemcpyDeviceToHost, st));
-    VNM_HIP_TRY(hipStreamSynchronize(st));
-    if ((rc = z->rows.alloc(std::max(z->n, 1)))) return rc;
-    hipLaunchKernelGGL(k_zero_list, dim3(grid_of(R, 256)), dim3(256), 0, st, zflag.p, zexcl.p, R, z->rows.p);
-    p->ctx->launches += 2;
-    VNM_LAUNCH_CHECK("k_zero_list");
-    VNM_HIP_TRY(hipStreamSynchronize(st));
-    *out = z.get();
-    p->zero[gsz] = std::move(z);
-    return VNM_OK;
+// Partial-sum slots for n output columns, grown on demand.
+int ensure_part(vnm_packed *p, int n) {
+    const size_t need = std::max<size_t>((size_t)p->nslots * n, 1);
+    if (p->part.n >= need) return VNM_OK;
+    VNM_HIP_TRY(hipStreamSynchronize(p->ctx->stream));  // the old buffer may be in use
+    return p->part.alloc(need);
 }
 
 int bsize_of(int bt) { return bt == VNM_DT_F32 ? 4 : bt == VNM_DT_BF16 ? 2 : 1; }
@@ -637,9 +658,10 @@ bool vec_ok(int bt, int n, int vec) {
 }
 
 SpmmArgs packed_args(const vnm_packed *p, const void *B, int ldb, const float *bs, float *C, int ldc,
-                     const float *bias) {
+                     const float *bias, float *part, int pn) {
     SpmmArgs a;
-    a.rowptr = p->rowptr.p;
+    a.item_beg = p->item_beg.p;
+    a.item_slot = p->item_slot.p;
     a.ent_row = p->ent_row.p;
     a.ent_col = p->ent_col.p;
     a.ent_idx = p->ent_idx.p;
@@ -649,20 +671,21 @@ SpmmArgs packed_args(const vnm_packed *p, const void *B, int ldb, const float *b
     a.B = (const uint8_t *)B;
     a.b_scales = bs;
     a.C = C;
+    a.part = part;
     a.bias = bias;
-    a.E = p->nent;
+    a.items = p->nitems;
     a.M = p->M;
     a.ldb = ldb;
     a.ldc = ldc;
+    a.pn = pn;
     a.b_rows = p->ncols;
     return a;
 }
 
 template <int AT, int BT, int VEC>
-static void launch_v(vnm_packed *p, const void *B, int ldb, int n, const float *bs, float *C, int ldc,
-                     const float *bias, hipStream_t st) {
+static void launch_v(const SpmmArgs &args, int n, hipStream_t st) {
     const int T = n / (64 * VEC);
-    const int items = (p->nent + ITEM - 1) / ITEM;
+    const int items = args.items;
     const int iwg = (items + 3) / 4;
     int per_g;
     if (8 % T == 0) {
@@ -671,55 +694,63 @@ static void launch_v(vnm_packed *p, const void *B, int ldb, int n, const float *
     } else {
         per_g = (iwg * T + 7) / 8;
     }
-    hipLaunchKernelGGL((k_spmm<AT, BT, VEC>), dim3(8 * per_g), dim3(256), 0, st, packed_args(p, B, ldb, bs, C, ldc, bias),
-                       T, items, per_g);
+    hipLaunchKernelGGL((k_spmm<AT, BT, VEC>), dim3(8 * per_g), dim3(256), 0, st, args, T, items, per_g);
 }
 
 template <int AT>
-static void launch_a(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, const float *bs, float *C,
-                     int ldc, const float *bias, hipStream_t st) {
+static void launch_a(const SpmmArgs &args, int vec, int n, int bt, hipStream_t st) {
     if (bt == VNM_DT_F32) {
-        if (vec == 4) launch_v<AT, VNM_DT_F32, 4>(p, B, ldb, n, bs, C, ldc, bias, st);
-        else if (vec == 2) launch_v<AT, VNM_DT_F32, 2>(p, B, ldb, n, bs, C, ldc, bias, st);
-        else launch_v<AT, VNM_DT_F32, 1>(p, B, ldb, n, bs, C, ldc, bias, st);
+        if (vec == 4) launch_v<AT, VNM_DT_F32, 4>(args, n, st);
+        else if (vec == 2) launch_v<AT, VNM_DT_F32, 2>(args, n, st);
+        else launch_v<AT, VNM_DT_F32, 1>(args, n, st);
     } else if (bt == VNM_DT_BF16) {
-        if (vec == 8) launch_v<AT, VNM_DT_BF16, 8>(p, B, ldb, n, bs, C, ldc, bias, st);
-        else if (vec == 4) launch_v<AT, VNM_DT_BF16, 4>(p, B, ldb, n, bs, C, ldc, bias, st);
-        else if (vec == 2) launch_v<AT, VNM_DT_BF16, 2>(p, B, ldb, n, bs, C, ldc, bias, st);
-        else launch_v<AT, VNM_DT_BF16, 1>(p, B, ldb, n, bs, C, ldc, bias, st);
+        if (vec == 8) launch_v<AT, VNM_DT_BF16, 8>(args, n, st);
+        else if (vec == 4) launch_v<AT, VNM_DT_BF16, 4>(args, n, st);
+        else if (vec == 2) launch_v<AT, VNM_DT_BF16, 2>(args, n, st);
+        else launch_v<AT, VNM_DT_BF16, 1>(args, n, st);
     } else {
-        if (vec == 16) launch_v<AT, VNM_DT_FP8, 16>(p, B, ldb, n, bs, C, ldc, bias, st);
-        else if (vec == 8) launch_v<AT, VNM_DT_FP8, 8>(p, B, ldb, n, bs, C, ldc, bias, st);
-        else if (vec == 4) launch_v<AT, VNM_DT_FP8, 4>(p, B, ldb, n, bs, C, ldc, bias, st);
-        else launch_v<AT, VNM_DT_FP8, 2>(p, B, ldb, n, bs, C, ldc, bias, st);
+        if (vec == 16) launch_v<AT, VNM_DT_FP8, 16>(args, n, st);
+        else if (vec == 8) launch_v<AT, VNM_DT_FP8, 8>(args, n, st);
+        else if (vec == 4) launch_v<AT, VNM_DT_FP8, 4>(args, n, st);
+        else launch_v<AT, VNM_DT_FP8, 2>(args, n, st);
     }
 }
 
-// One SpMM with `vec` values per lane: zero the listed rows, then the gather kernel.
-static int spmm_v(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, const float *bs, float *C,
-                  int ldc, const float *bias, bool timed) {
+// One SpMM with `vec` values per lane and the partial-sum slots at `part`
+// (nslots x n floats): the gather kernel, then the fix-up pass.
+int spmm_run(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, const float *bs, float *C,
+                    int ldc, const float *bias, float *part, bool timed) {
     hipStream_t st = p->ctx->stream;
-    ZeroList *z;
-    int rc = zero_list(p, ITEM, &z);
-    if (rc) return rc;
-    if (z->n > 0)
-        hipLaunchKernelGGL(k_zero_rows, dim3((z->n + 3) / 4), dim3(256), 0, st, z->rows.p, z->n, n, C, ldc, bias);
-    if (p->nent > 0) {
+    const SpmmArgs args = packed_args(p, B, ldb, bs, C, ldc, bias, part, n);
+    if (p->nitems > 0) {
         KernelTimer kt(timed ? p->ctx : nullptr, "k_spmm");
         switch (p->dtype) {
-            case VNM_DT_F32: launch_a<VNM_DT_F32>(p, vec, B, ldb, n, bt, bs, C, ldc, bias, st); break;
-            case VNM_DT_BF16: launch_a<VNM_DT_BF16>(p, vec, B, ldb, n, bt, bs, C, ldc, bias, st); break;
-            case VNM_DT_F16: launch_a<VNM_DT_F16>(p, vec, B, ldb, n, bt, bs, C, ldc, bias, st); break;
-            default: launch_a<VNM_DT_FP8>(p, vec, B, ldb, n, bt, bs, C, ldc, bias, st); break;
+            case VNM_DT_F32: launch_a<VNM_DT_F32>(args, vec, n, bt, st); break;
+            case VNM_DT_BF16: launch_a<VNM_DT_BF16>(args, vec, n, bt, st); break;
+            case VNM_DT_F16: launch_a<VNM_DT_F16>(args, vec, n, bt, st); break;
+            default: launch_a<VNM_DT_FP8>(args, vec, n, bt, st); break;
         }
         kt.stop();
+        p->ctx->launches++;
     }
-    p->ctx->launches += 2;
+    if (p->nfix > 0) {
+        hipLaunchKernelGGL(k_spmm_fix, dim3((p->nfix + 3) / 4), dim3(256), 0, st, p->fix.p, p->nfix, part, n, n, C,
+                           ldc, bias);
+        p->ctx->launches++;
+    }
     VNM_LAUNCH_CHECK("k_spmm");
     return VNM_OK;
 }
 
-static int shape_key(int bt, int n) { return (bt << 24) | n; }
+// spmm_run with the packed matrix's own slots.
+static int spmm_v(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, const float *bs, float *C,
+                  int ldc, const float *bias, bool timed) {
+    int rc = ensure_part(p, n);
+    if (rc) return rc;
+    return spmm_run(p, vec, B, ldb, n, bt, bs, C, ldc, bias, p->part.p, timed);
+}
+
+int shape_key(int bt, int n) { return (bt << 24) | n; }
 
 // Autotuner: time every usable values-per-lane setting on this matrix and
 // shape (one warm-up and three timed calls each, HIP events on the context
@@ -754,8 +785,8 @@ static int tune_vec(vnm_packed *p, const void *B, int ldb, int n, int bt, const 
     return VNM_OK;
 }
 
-int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc,
-         const float *bias) {
+int spmm_check(const vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs,
+                      const float *C, int ldc) {
     if (!p || !B || !C || n <= 0 || n % 64 != 0 || ldb < n || ldc < n)
         return set_error(VNM_E_ARG, "spmm: n must be a positive multiple of 64 with ldb, ldc >= n");
     if (bt != VNM_DT_F32 && bt != VNM_DT_BF16 && bt != VNM_DT_FP8)
@@ -767,13 +798,19 @@ int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, 
                                     "units (ldb * sizeof(B value) and 4 * ldc multiples of 16)");
     if ((uint64_t)p->ncols * (uint64_t)ldb * (uint64_t)bsize_of(bt) > 0xFFFFFFFFull)
         return set_error(VNM_E_ARG, "spmm: B (ncols x ldb) must be smaller than 4 GiB");
+    return VNM_OK;
+}
+
+int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc,
+         const float *bias) {
+    int rc = spmm_check(p, B, ldb, n, bt, bs, C, ldc);
+    if (rc) return rc;
     int v = 0;
     auto it = p->shapes.find(shape_key(bt, n));
     if (it != p->shapes.end()) {
         v = it->second;
     } else {
-        int rc = tune_vec(p, B, ldb, n, bt, bs, C, ldc, bias, &v);
-        if (rc) return rc;
+        if ((rc = tune_vec(p, B, ldb, n, bt, bs, C, ldc, bias, &v))) return rc;
         p->shapes[shape_key(bt, n)] = v;
     }
     return spmm_v(p, v, B, ldb, n, bt, bs, C, ldc, bias, true);

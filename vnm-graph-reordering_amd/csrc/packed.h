@@ -14,13 +14,6 @@
 #include "graph.h"
 #include "hip_common.h"
 
-namespace vnm {
-struct ZeroList {
-    int n = 0;
-    DBuf<int32_t> rows;  // rows that keep nothing or are cut by a group boundary
-};
-}  // namespace vnm
-
 struct vnm_packed {
     vnm_ctx *ctx = nullptr;
     int32_t V = 0, M = 0, nrows = 0, ncols = 0, dtype = 0;
@@ -30,7 +23,14 @@ struct vnm_packed {
     vnm::DBuf<uint8_t> ent_idx, ent_val;                   // values: 2 per entry, dtype-sized
     vnm::DBuf<uint16_t> blk_sel;
     vnm::DBuf<float> blk_scale;
-    std::map<int, std::unique_ptr<vnm::ZeroList>> zero;    // by entries per item
+    // SpMM work items: whole rows of at most ITEM row entries, and rows longer
+    // than ITEM cut into ITEM-entry chunks whose partial sums go to slots
+    int32_t nitems = 0, nslots = 0, nfix = 0;
+    vnm::DBuf<int32_t> item_beg;                           // nitems + 1 entry offsets
+    vnm::DBuf<int32_t> item_slot;                          // slot of the item's first row, or -1
+    vnm::DBuf<int4> fix;                                   // {row, first slot, slots, 0}: rows the
+                                                           // fix-up pass writes (empty or cut)
+    vnm::DBuf<float> part;                                 // nslots x n partial sums
     std::map<int, int> shapes;                             // (b dtype << 24 | n) -> values per lane
     vnm::DBuf<uint16_t> gcn_h;                             // X W of the last GCN layer (BF16)
 };
@@ -139,20 +139,13 @@ __device__ __forceinline__ void fma_raw(float (&acc)[VEC], float w, const BRaw<B
     }
 }
 
-// Write a finished output row slice (plus the bias, when given), or add a
-// partial one atomically (the row was pre-filled with the bias or zero).
+// Store a finished output row slice (plus the bias slice, when given).
 template <int NV>
-__device__ __forceinline__ void flush_vec(float *__restrict__ C, int ldc, int row, int col0,
-                                          const float (&acc)[NV], bool atomic, const float *__restrict__ bias) {
-    float *cp = C + (size_t)row * ldc + col0;
-    if (atomic) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) atomicAdd(&cp[i], acc[i]);
-        return;
-    }
+__device__ __forceinline__ void store_vec(float *__restrict__ cp, const float (&acc)[NV],
+                                          const float *__restrict__ bias) {
     float v[NV];
 #pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = bias ? acc[i] + bias[col0 + i] : acc[i];
+    for (int i = 0; i < NV; ++i) v[i] = bias ? acc[i] + bias[i] : acc[i];
     if constexpr (NV % 4 == 0) {
 #pragma unroll
         for (int i = 0; i < NV; i += 4) *(float4 *)(cp + i) = make_float4(v[i], v[i + 1], v[i + 2], v[i + 3]);
@@ -165,15 +158,16 @@ __device__ __forceinline__ void flush_vec(float *__restrict__ C, int ldc, int ro
 
 // One packed matrix as the SpMM kernels see it.
 struct SpmmArgs {
-    const int32_t *rowptr, *ent_row, *ent_col;
+    const int32_t *item_beg, *item_slot, *ent_row, *ent_col;
     const uint8_t *ent_idx, *ent_val;
     const int32_t *ent_blk;
     const float *blk_scale;
     const uint8_t *B;
     const float *b_scales;
     float *C;
+    float *part;        // nslots x pn partial sums of cut rows
     const float *bias;  // nullptr or n floats added to every output row
-    int E, M, ldb, ldc;
+    int items, M, ldb, ldc, pn;
     int b_rows;         // rows of B (columns of A); b_rows * ldb * sizeof(value) < 2^32
 };
 
@@ -211,26 +205,30 @@ __device__ __forceinline__ float lane_bcast(float v, int k) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-// C = A_vnm B for one work item (ITEM row entries starting at `beg`, `cnt`
-// of them) and column tile `tile` (64 * VEC columns), one wave.  Lane k
-// first resolves entry k: its row, its scaled values and the byte offsets
-// of its B rows (column * ldb * sizeof(B value); -1 for an absent second
-// value).  The wave then walks the entries, U at a time, with each entry's
-// data broadcast by readlane: every lane gathers VEC values of the entry's
-// B rows by buffer loads whose row offset is that broadcast scalar, and owns
-// VEC columns of the output row.  A row is flushed when the row changes;
-// atomically when an item boundary cuts it (such rows are pre-filled by the
-// zero list).  PIPE: two register sets of U entries, the gathers of the next
-// group in flight while the current one is folded in (the persistent batched
-// kernel runs faster this way; the one-matrix kernel does not).
+// C = A_vnm B for one work item (`cnt` <= ITEM row entries starting at
+// `beg`) and column tile `tile` (64 * VEC columns), one wave.  Lane k first
+// resolves entry k: its row, its scaled values and the byte offsets of its
+// B rows (column * ldb * sizeof(B value); -1 for an absent second value).
+// The wave then walks the entries, U at a time, with each entry's data
+// broadcast by readlane: every lane gathers VEC values of the entry's B rows
+// by buffer loads whose row offset is that broadcast scalar, and owns VEC
+// columns of the output row.  A row is stored when the row changes.  Items
+// hold whole rows, except chunks of rows longer than ITEM entries: such a
+// chunk is the first row of its item, and its sum goes to the item's
+// partial-sum slot (slot >= 0) instead of C; the fix-up pass adds a cut
+// row's slots in item order.  So every output value is one fixed sequence of
+// FP32 operations, whatever the launch shape.  PIPE: two register sets of U
+// entries, the gathers of the next group in flight while the current one is
+// folded in (the persistent batched kernel runs faster this way; the
+// one-matrix kernel does not).
 template <int AT, int BT, int VEC, bool PIPE = false>
-__device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ erow,
-                                          const int32_t *__restrict__ ecol, const uint8_t *__restrict__ eidx,
-                                          const uint8_t *__restrict__ evals, const int32_t *__restrict__ eblk,
-                                          const float *__restrict__ blk_scale, const float *__restrict__ bsc,
-                                          const uint8_t *__restrict__ B, float *__restrict__ C,
-                                          const float *__restrict__ bias, int beg, int cnt, int M, int ldb,
-                                          int ldc, int b_rows, int tile) {
+__device__ __forceinline__ void spmm_item(const int32_t *__restrict__ erow, const int32_t *__restrict__ ecol,
+                                          const uint8_t *__restrict__ eidx, const uint8_t *__restrict__ evals,
+                                          const int32_t *__restrict__ eblk, const float *__restrict__ blk_scale,
+                                          const float *__restrict__ bsc, const uint8_t *__restrict__ B,
+                                          float *__restrict__ C, float *__restrict__ part,
+                                          const float *__restrict__ bias, int beg, int cnt, int slot, int M,
+                                          int ldb, int ldc, int pn, int b_rows, int tile) {
     constexpr int SB = bsize<BT>();
     // entries per group: up to 8; pipelined, 16 VGPRs of gathered values
     // per register set
@@ -260,16 +258,21 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
         }
     }
     const int first_row = lane_bcast(rr, 0);
-    const int last_row = lane_bcast(rr, cnt - 1);
     // entries that start a new row (entry 0 does not: cur starts there)
     const int prev_rr = __shfl_up(rr, 1, 64);
     const uint64_t starts = __ballot(lane > 0 && lane < cnt && rr != prev_rr);
     const int col0 = tile * (64 * VEC) + lane * VEC;
     const int voff = col0 * SB;
+    const float *bias_c = bias ? bias + col0 : nullptr;
     float acc[VEC];
 #pragma unroll
     for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
     int cur = first_row;
+    // the finished row `cur`: to its slot when it is the item's cut first row
+    auto flush = [&]() {
+        if (slot >= 0 && cur == first_row) store_vec<VEC>(part + (size_t)slot * pn + col0, acc, nullptr);
+        else store_vec<VEC>(C + (size_t)cur * ldc + col0, acc, bias_c);
+    };
     // U entries: broadcast, gather, then fold in (flushing on a row change);
     // CHECK only for the item's last, partial group
     auto group = [&](int k, auto check) {
@@ -292,15 +295,15 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
         for (int u = 0; u < U; ++u) {
             const int kk = k + u;
             if ((starts >> (kk & 63)) & 1ull) {  // wave-uniform: a new row starts here
-                const bool cut = (cur == first_row && rowptr[cur] < beg) ||
-                                 (cur == last_row && rowptr[cur + 1] > beg + cnt);
-                flush_vec<VEC>(C, ldc, cur, col0, acc, cut, bias);
+                flush();
 #pragma unroll
                 for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
                 cur = lane_bcast(rr, kk & 63);
             }
-            fma_raw<BT, VEC>(acc, w[u][0], raw[u][0]);
-            if (o1[u] >= 0) fma_raw<BT, VEC>(acc, w[u][1], raw[u][1]);
+            if (!CHECK || kk < cnt) {
+                fma_raw<BT, VEC>(acc, w[u][0], raw[u][0]);
+                if (o1[u] >= 0) fma_raw<BT, VEC>(acc, w[u][1], raw[u][1]);
+            }
         }
     };
     if constexpr (!PIPE) {
@@ -310,8 +313,7 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
     } else {
         // the same gathers and folds as group(), split so that a group's
         // loads are issued before the previous group is folded in; entries
-        // past cnt gather nothing and fold zeros (their lanes hold zero
-        // weights and no second value)
+        // past cnt gather and fold nothing
         using Raw = BRaw<BT, VEC>[U][2];
         auto issue = [&](int k, Raw &raw, auto check) {
             constexpr bool CHECK = decltype(check)::value;
@@ -332,17 +334,18 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
         auto fold = [&](int k, const Raw &raw) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int ks = (k + u) & 63;
+                const int kk = k + u;
+                const int ks = kk & 63;
                 if ((starts >> ks) & 1ull) {  // wave-uniform: a new row starts here
-                    const bool cut = (cur == first_row && rowptr[cur] < beg) ||
-                                     (cur == last_row && rowptr[cur + 1] > beg + cnt);
-                    flush_vec<VEC>(C, ldc, cur, col0, acc, cut, bias);
+                    flush();
 #pragma unroll
                     for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
                     cur = lane_bcast(rr, ks);
                 }
-                fma_raw<BT, VEC>(acc, lane_bcast(a0, ks), raw[u][0]);
-                if (lane_bcast(off1, ks) >= 0) fma_raw<BT, VEC>(acc, lane_bcast(a1, ks), raw[u][1]);
+                if (kk < cnt) {
+                    fma_raw<BT, VEC>(acc, lane_bcast(a0, ks), raw[u][0]);
+                    if (lane_bcast(off1, ks) >= 0) fma_raw<BT, VEC>(acc, lane_bcast(a1, ks), raw[u][1]);
+                }
             }
         };
         Raw ra, rb;
@@ -356,27 +359,57 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ rowptr, co
             if (k + 2 * U >= cnt) break;
         }
     }
-    const bool cut = (cur == first_row && rowptr[cur] < beg) || (cur == last_row && rowptr[cur + 1] > beg + cnt);
-    flush_vec<VEC>(C, ldc, cur, col0, acc, cut, bias);
+    flush();
 }
 
 // Work item `item` (wave-uniform), column tile `tile` of one packed matrix.
 template <int AT, int BT, int VEC, bool PIPE = false>
 __device__ __forceinline__ void spmm_unit(const SpmmArgs &a, int item, int tile) {
-    const int beg = item * ITEM;
-    spmm_item<AT, BT, VEC, PIPE>(a.rowptr, a.ent_row, a.ent_col, a.ent_idx, a.ent_val, a.ent_blk,
-                           a.blk_scale, a.b_scales, a.B, a.C, a.bias, beg, min(ITEM, a.E - beg), a.M, a.ldb, a.ldc,
-                           a.b_rows, tile);
+    const int beg = a.item_beg[item];
+    const int cnt = a.item_beg[item + 1] - beg;
+    spmm_item<AT, BT, VEC, PIPE>(a.ent_row, a.ent_col, a.ent_idx, a.ent_val, a.ent_blk, a.blk_scale,
+                                 a.b_scales, a.B, a.C, a.part, a.bias, beg, cnt, a.item_slot[item], a.M,
+                                 a.ldb, a.ldc, a.pn, a.b_rows, tile);
+}
+
+// Fix-up of one listed row (a wave): the sum of its `ns` partial-sum slots
+// from `s0` on, in slot order, plus the bias; rows that keep nothing
+// (ns = 0) get the bias or zero.
+__device__ __forceinline__ void fix_row(float *__restrict__ crow, const float *__restrict__ part, int pn, int s0,
+                                        int ns, int n, const float *__restrict__ bias) {
+    const int lane = threadIdx.x & 63;
+    for (int c = lane * 4; c < n; c += 256) {
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int j = 0; j < ns; ++j) {
+            const float4 v = *(const float4 *)(part + (size_t)(s0 + j) * pn + c);
+            acc.x += v.x;
+            acc.y += v.y;
+            acc.z += v.z;
+            acc.w += v.w;
+        }
+        if (bias) {
+            const float4 b = *(const float4 *)(bias + c);
+            acc.x += b.x;
+            acc.y += b.y;
+            acc.z += b.z;
+            acc.w += b.w;
+        }
+        *(float4 *)(crow + c) = acc;
+    }
 }
 
 // ------------------------------------------------------------ host helpers (pack.hip)
 
-int zero_list(vnm_packed *p, int gsz, ZeroList **out);
+int ensure_part(vnm_packed *p, int n);
 int bsize_of(int bt);
 bool vec_ok(int bt, int n, int vec);
 SpmmArgs packed_args(const vnm_packed *p, const void *B, int ldb, const float *bs, float *C, int ldc,
-                     const float *bias);
+                     const float *bias, float *part, int pn);
 int spmm(vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc,
          const float *bias);
+int spmm_check(const vnm_packed *p, const void *B, int ldb, int n, int bt, const float *bs, const float *C, int ldc);
+int spmm_run(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc,
+             const float *bias, float *part, bool timed);
+int shape_key(int bt, int n);
 
 }  // namespace vnm
