@@ -7,8 +7,8 @@
 // one matrix share that XCD's L2.  A device table holds every matrix's
 // pointers (SpmmArgs).  The batch is tuned once (values per lane, timed on the
 // whole batch) and then captured as a HIP graph of two kernels -- the SpMM
-// and the fix-up pass (rows that keep nothing, rows cut into chunks whose
-// partial sums are added in item order) -- replayed by vnm_batch_run.
+// and the fix-up pass (rows cut into chunks, whose partial sums are added in
+// item order) -- replayed by vnm_batch_run.
 //
 // GCN layer (the consumer of gnn/single/pyg_gcn.py:41-74, aggr='add'):
 // Y = A_vnm (X W) + bias.  X W runs on MFMA (BF16 inputs, FP32 accumulation,
@@ -62,8 +62,8 @@ __global__ __launch_bounds__(256) void k_spmm_batch(const SpmmArgs *__restrict__
     }
 }
 
-// Fix-up of the listed (matrix, row) pairs: rows that keep nothing get the
-// matrix's bias or zero, cut rows the sum of their slots in item order.
+// Fix-up of the listed (matrix, row) pairs: cut rows get the sum of their
+// slots in item order (rows of a matrix without work items, the bias or zero).
 __global__ void k_fix_batch(const int4 *__restrict__ list, int nf, int n, const SpmmArgs *__restrict__ descs) {
     const int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (w >= nf) return;

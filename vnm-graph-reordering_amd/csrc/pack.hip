@@ -29,10 +29,10 @@
 // values), then the wave walks the entries with the entry data broadcast by
 // scalar reads, every lane gathering VEC consecutive values of each B row (2
 // to 16 bytes) and owning VEC columns of the output row, which is stored when
-// the row changes.  The chunks of a long row store partial sums into slots,
-// and a fix-up pass adds each cut row's slots in item order (and writes the
-// rows that keep nothing), so C is bit-identical for every launch shape and
-// run: no atomics.  When the tile count T divides 8, tile t runs on its own
+// the row changes (rows that keep nothing are written in the gaps).  The
+// chunks of a long row store partial sums into slots, and a fix-up pass adds
+// each cut row's slots in item order, so C is bit-identical for every launch
+// shape and run: no atomics.  When the tile count T divides 8, tile t runs on its own
 // 8/T XCDs, so each XCD's L2 serves one slice of B.  VEC is chosen per (B
 // dtype, n) by timing the usable values on first use (vnm_spmm_set_shape
 // overrides).  FP8 B is decoded by the gfx950 e4m3 conversion instruction.
@@ -394,8 +394,9 @@ __global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int T, int items, int 
     spmm_unit<AT, BT, VEC>(a, item, __builtin_amdgcn_readfirstlane(tile));
 }
 
-// Fix-up pass: rows that keep nothing get the bias or zero, rows cut into
-// chunks the sum of their partial-sum slots in item order (one wave each).
+// Fix-up pass: rows cut into chunks get the sum of their partial-sum slots
+// in item order (one wave each); a matrix without work items, the bias or
+// zero in every row.
 __global__ void k_spmm_fix(const int4 *__restrict__ fix, int nfix, const float *__restrict__ part, int pn, int n,
                            float *__restrict__ C, int ldc, const float *__restrict__ bias) {
     const int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
@@ -426,8 +427,8 @@ static std::vector<uint32_t> subsets4(int M) {
 // ITEM starts a new item and is cut into ITEM-entry chunks, each with a
 // partial-sum slot of its own (consecutive slots), the last chunk staying
 // open for the rows that follow.  So an item's only cut row is its first.
-// The fix-up list holds the rows that keep nothing (no slot) and the cut
-// rows with their slot range.
+// The fix-up list holds the cut rows with their slot range (the SpMM writes
+// the rows that keep nothing; without any item, the list holds every row).
 static int build_items(vnm_packed *p) {
     hipStream_t st = p->ctx->stream;
     const int R = p->nrows;
@@ -440,10 +441,7 @@ static int build_items(vnm_packed *p) {
     bool open = false;
     for (int r = 0; r < R; ++r) {
         const int b = rp[r], L = rp[r + 1] - b;
-        if (L == 0) {
-            fix.push_back(make_int4(r, 0, 0, 0));
-            continue;
-        }
+        if (L == 0) continue;
         if (L <= ITEM) {
             if (open && cur_len + L > ITEM) open = false;
             if (!open) {
@@ -465,6 +463,8 @@ static int build_items(vnm_packed *p) {
         }
     }
     beg.push_back(rp[R]);
+    if (slot.empty())
+        for (int r = 0; r < R; ++r) fix.push_back(make_int4(r, 0, 0, 0));
     p->nitems = (int)slot.size();
     p->nslots = nslots;
     p->nfix = (int)fix.size();
@@ -679,6 +679,7 @@ SpmmArgs packed_args(const vnm_packed *p, const void *B, int ldb, const float *b
     a.ldc = ldc;
     a.pn = pn;
     a.b_rows = p->ncols;
+    a.c_rows = p->nrows;
     return a;
 }
 

@@ -169,6 +169,7 @@ struct SpmmArgs {
     const float *bias;  // nullptr or n floats added to every output row
     int items, M, ldb, ldc, pn;
     int b_rows;         // rows of B (columns of A); b_rows * ldb * sizeof(value) < 2^32
+    int c_rows;         // rows of C (of A)
 };
 
 // VEC values of one B row for this lane, by a buffer load whose row offset
@@ -212,7 +213,9 @@ __device__ __forceinline__ float lane_bcast(float v, int k) {
 // The wave then walks the entries, U at a time, with each entry's data
 // broadcast by readlane: every lane gathers VEC values of the entry's B rows
 // by buffer loads whose row offset is that broadcast scalar, and owns VEC
-// columns of the output row.  A row is stored when the row changes.  Items
+// columns of the output row.  A row is stored when the row changes, and the
+// rows that keep nothing between it and the next (up to the next item's first
+// row `next_row`; item 0 also from row 0) get the bias or zero.  Items
 // hold whole rows, except chunks of rows longer than ITEM entries: such a
 // chunk is the first row of its item, and its sum goes to the item's
 // partial-sum slot (slot >= 0) instead of C; the fix-up pass adds a cut
@@ -227,8 +230,9 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ erow, cons
                                           const int32_t *__restrict__ eblk, const float *__restrict__ blk_scale,
                                           const float *__restrict__ bsc, const uint8_t *__restrict__ B,
                                           float *__restrict__ C, float *__restrict__ part,
-                                          const float *__restrict__ bias, int beg, int cnt, int slot, int M,
-                                          int ldb, int ldc, int pn, int b_rows, int tile) {
+                                          const float *__restrict__ bias, int beg, int cnt, int slot,
+                                          int lead_row, int next_row, int M, int ldb, int ldc, int pn,
+                                          int b_rows, int tile) {
     constexpr int SB = bsize<BT>();
     // entries per group: up to 8; pipelined, 16 VGPRs of gathered values
     // per register set
@@ -268,6 +272,14 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ erow, cons
 #pragma unroll
     for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
     int cur = first_row;
+    // rows [r0, r1) keep nothing: the bias or zero
+    auto fill = [&](int r0, int r1) {
+        float z[VEC];
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) z[i] = 0.f;
+        for (int r = r0; r < r1; ++r) store_vec<VEC>(C + (size_t)r * ldc + col0, z, bias_c);
+    };
+    fill(lead_row, first_row);
     // the finished row `cur`: to its slot when it is the item's cut first row
     auto flush = [&]() {
         if (slot >= 0 && cur == first_row) store_vec<VEC>(part + (size_t)slot * pn + col0, acc, nullptr);
@@ -298,7 +310,9 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ erow, cons
                 flush();
 #pragma unroll
                 for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
-                cur = lane_bcast(rr, kk & 63);
+                const int nxt = lane_bcast(rr, kk & 63);
+                fill(cur + 1, nxt);
+                cur = nxt;
             }
             if (!CHECK || kk < cnt) {
                 fma_raw<BT, VEC>(acc, w[u][0], raw[u][0]);
@@ -340,7 +354,9 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ erow, cons
                     flush();
 #pragma unroll
                     for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
-                    cur = lane_bcast(rr, ks);
+                    const int nxt = lane_bcast(rr, ks);
+                    fill(cur + 1, nxt);
+                    cur = nxt;
                 }
                 if (kk < cnt) {
                     fma_raw<BT, VEC>(acc, lane_bcast(a0, ks), raw[u][0]);
@@ -360,32 +376,50 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ erow, cons
         }
     }
     flush();
+    fill(cur + 1, next_row);
 }
 
 // Work item `item` (wave-uniform), column tile `tile` of one packed matrix.
 template <int AT, int BT, int VEC, bool PIPE = false>
 __device__ __forceinline__ void spmm_unit(const SpmmArgs &a, int item, int tile) {
     const int beg = a.item_beg[item];
-    const int cnt = a.item_beg[item + 1] - beg;
+    const int end = a.item_beg[item + 1];
+    // rows that keep nothing: item 0 fills those before its first row, every
+    // item those between its rows and after its last, up to the next item's
+    // first row (the last item: up to the row count)
+    const int lead = item == 0 ? 0 : a.c_rows;
+    const int next_row = item + 1 < a.items ? a.ent_row[end] : a.c_rows;
     spmm_item<AT, BT, VEC, PIPE>(a.ent_row, a.ent_col, a.ent_idx, a.ent_val, a.ent_blk, a.blk_scale,
-                                 a.b_scales, a.B, a.C, a.part, a.bias, beg, cnt, a.item_slot[item], a.M,
-                                 a.ldb, a.ldc, a.pn, a.b_rows, tile);
+                                 a.b_scales, a.B, a.C, a.part, a.bias, beg, end - beg, a.item_slot[item],
+                                 lead, next_row, a.M, a.ldb, a.ldc, a.pn, a.b_rows, tile);
 }
 
 // Fix-up of one listed row (a wave): the sum of its `ns` partial-sum slots
-// from `s0` on, in slot order, plus the bias; rows that keep nothing
-// (ns = 0) get the bias or zero.
+// from `s0` on, in slot order, plus the bias; a matrix without work items
+// lists every row with ns = 0 (the bias or zero).  The slots' loads go out
+// 16 at a time before they are added (in order), so a row of many chunks
+// waits for a few round trips rather than one per slot.
 __device__ __forceinline__ void fix_row(float *__restrict__ crow, const float *__restrict__ part, int pn, int s0,
                                         int ns, int n, const float *__restrict__ bias) {
+    constexpr int FB = 16;
     const int lane = threadIdx.x & 63;
     for (int c = lane * 4; c < n; c += 256) {
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (int j = 0; j < ns; ++j) {
-            const float4 v = *(const float4 *)(part + (size_t)(s0 + j) * pn + c);
-            acc.x += v.x;
-            acc.y += v.y;
-            acc.z += v.z;
-            acc.w += v.w;
+        const float *pc = part + (size_t)s0 * pn + c;
+        for (int j0 = 0; j0 < ns; j0 += FB) {
+            float4 v[FB];
+#pragma unroll
+            for (int j = 0; j < FB; ++j)
+                v[j] = j0 + j < ns ? *(const float4 *)(pc + (size_t)(j0 + j) * pn) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int j = 0; j < FB; ++j) {
+                if (j0 + j < ns) {
+                    acc.x += v[j].x;
+                    acc.y += v[j].y;
+                    acc.z += v[j].z;
+                    acc.w += v[j].w;
+                }
+            }
         }
         if (bias) {
             const float4 b = *(const float4 *)(bias + c);
