@@ -235,6 +235,16 @@ int vnm_hybrid(vnm_graph *g, int N, int V, int rounds, int sched, vnm_hybrid_sta
  * reference compares wall times). */
 int vnm_find_best_sched(vnm_graph *g, int N, int nsched, int32_t *best);
 
+/* Stable LSD radix sort of device arrays on the context stream: keys_out =
+ * keys_in ordered by bits [0, end_bit) (equal keys keep their input order),
+ * vals_out the values moved with them (both value pointers NULL: keys only).
+ * No counterpart in the reference, whose VENOM row order is Python's stable
+ * sorted() over per-row keys (reordering.py:219); this is the library's own
+ * sort behind the VENOM, packing and proposer passes.  Outputs must not
+ * alias inputs; n < 2^31. */
+int vnm_sort_pairs(vnm_ctx *ctx, const uint64_t *keys_in, uint64_t *keys_out, const int32_t *vals_in,
+                   int32_t *vals_out, int64_t n, int end_bit);
+
 /* ---------------------------------------------------------------- V:N:M */
 
 /* Entries kept by the V:N:M (N = 2) format: per V x M block, the 4-column

@@ -1,5 +1,6 @@
-// Device-wide sort/scan/select primitives used by the VENOM and packing
-// passes (implemented over rocPRIM in sortlib.hip).
+// Device-wide sort/scan/select primitives used by the VENOM, packing and
+// proposer passes (hand-written in sortlib.hip).  Scratch is per host thread
+// and kernels run on `st`.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,11 +10,14 @@
 
 namespace vnm {
 
+// Stable LSD radix sort on bits [0, end_bit) of the keys (the input is not
+// written; out must not alias it).
 int sort_keys_u64(const uint64_t *in, uint64_t *out, int64_t n, int end_bit, hipStream_t st);
 int sort_pairs_u64_i32(const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t *vout,
                        int64_t n, int end_bit, hipStream_t st);
 int scan_excl_i32(const int32_t *in, int32_t *out, int64_t n, hipStream_t st);
 int scan_incl_max_i32(const int32_t *in, int32_t *out, int64_t n, hipStream_t st);
+// out[j] = in[i] for the j-th i with flags[i] != 0, in order; *d_count = j's.
 int select_flagged_i32(const int32_t *in, const uint8_t *flags, int32_t *out, int32_t *d_count,
                        int64_t n, hipStream_t st);
 

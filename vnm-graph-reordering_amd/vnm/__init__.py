@@ -38,7 +38,7 @@ EXPORTS = [
     "vnm_spmm_set_shape", "vnm_spmm_get_shape", "vnm_gcn_layer", "vnm_batch_create",
     "vnm_batch_run", "vnm_batch_info", "vnm_batch_destroy", "vnm_nm_reorder_allpairs",
     "vnm_nm_reorder_batch", "vnm_mtx_read_ex", "vnm_mtx_write_ex", "vnm_lp_reorder_batch",
-    "vnm_mtx_read_cached",
+    "vnm_mtx_read_cached", "vnm_sort_pairs",
 ]
 
 LP_NPARAMS = 22784
@@ -171,6 +171,7 @@ def lib():
     L.vnm_packed_destroy.argtypes = [vp]
     L.vnm_packed_destroy.restype = None
     L.vnm_spmm.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int]
+    L.vnm_sort_pairs.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int]
     f32p = P(ctypes.c_float)
     L.vnm_nm_trace.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p,
                                ctypes.c_int64, P(ctypes.c_int64), P(NMStats)]
@@ -364,6 +365,10 @@ class Context:
         ms, n = ctypes.c_double(), ctypes.c_int64()
         _check(lib().vnm_ctx_timer_read(self.h, kernel.encode(), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def sort_pairs(self, keys_in, keys_out, n, end_bit=64, vals_in=None, vals_out=None):
+        """Stable radix sort of device arrays (pointers) on the context stream."""
+        _check(lib().vnm_sort_pairs(self.h, keys_in, keys_out, vals_in, vals_out, n, end_bit))
 
     def pq_pop_order(self, scores, M):
         scores = np.ascontiguousarray(scores, np.int32).reshape(-1, 2 * M * M)
