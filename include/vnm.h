@@ -267,6 +267,12 @@ typedef struct {
                                  row, column, selector and two values
                                  (plus, for FP8, the block index and the
                                  block scales)                           */
+    int32_t items;            /* SpMM work items (whole rows of at most 64
+                                 row entries, or 64-entry chunks of longer
+                                 rows)                                    */
+    int32_t slots;            /* partial-sum slots (one per chunk of a row
+                                 longer than 64 row entries): a product
+                                 with n columns needs slots * n floats   */
 } vnm_packed_info;
 
 /* Replaces the cuSPARSELt prune/compress path (kernel/spmm.cu:402-447) and
@@ -283,6 +289,19 @@ void vnm_packed_destroy(vnm_packed *p);
  * scale per 128 rows in b_scales), C FP32, FP32 accumulation. */
 int vnm_spmm(vnm_packed *p, const void *B, int ldb, int n, int b_dtype,
              const float *b_scales, float *C, int ldc);
+
+/* vnm_spmm without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Bit for bit the
+ * result of vnm_spmm.  The partial sums of rows cut into work-item chunks go
+ * to a static device array of 8 Mi floats (32 MiB) that each device holds
+ * until the process ends; every call on a device shares it, so calls that
+ * overlap on one GPU (other contexts or streams) race.  A product needing
+ * more (vnm_packed_info.slots * n floats) returns VNM_E_ARG.  The launch
+ * shape is the one vnm_spmm chose or vnm_spmm_set_shape fixed for
+ * (b_dtype, n), else 8 bytes per lane. */
+int vnm_spmm_noalloc(vnm_packed *p, const void *B, int ldb, int n, int b_dtype,
+                     const float *b_scales, float *C, int ldc);
 
 /* Launch shape of vnm_spmm for (b_dtype, n): each lane gathers
  * cols_per_lane consecutive values of a B row per entry (2 to 16 bytes) and

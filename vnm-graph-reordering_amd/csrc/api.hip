@@ -128,6 +128,11 @@ int vnm_ctx_create(int device, vnm_ctx **out) {
     VNM_HIP_TRY(hipSetDevice(device));
     vnm_ctx *c = new vnm_ctx();
     c->device = device;
+    int rc = noalloc_setup(device, &c->noalloc_slots);
+    if (rc) {
+        delete c;
+        return rc;
+    }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;

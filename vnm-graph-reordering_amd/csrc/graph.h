@@ -19,6 +19,7 @@ struct vnm_timer_pending {
 struct vnm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    float *noalloc_slots = nullptr;  // device address of the static SpMM slot array on `device`
     int64_t launches = 0;
     bool timing = false;
     std::vector<vnm_timer_pending> pending;
@@ -121,6 +122,7 @@ int venom_reorder(vnm_graph *g, int V, int maxiter, vnm_venom_stats *st);
 int reload_sort(vnm_graph *g, bool quantize);
 int graph_clone(vnm_graph *src, vnm_graph **out);
 int ensure_tab(vnm_graph *g, size_t slots);
+int noalloc_setup(int device, float **out);
 
 // Event pair around one launch when the context's timers are on.
 struct KernelTimer {

@@ -850,6 +850,8 @@ int vnm_packed_info_get(const vnm_packed *p, vnm_packed_info *info) {
     // what k_spmm reads: the entry arrays, and for FP8 A the scales of the
     // blocks (rowptr is read at most twice per 64 entries)
     info->spmm_bytes = (int64_t)p->nent * (4 + 4 + 1 + 2 * vs + blk) + (blk ? (int64_t)p->nblocks * 4 : 0);
+    info->items = p->nitems;
+    info->slots = p->nslots;
     return VNM_OK;
 }
 

@@ -38,7 +38,7 @@ EXPORTS = [
     "vnm_spmm_set_shape", "vnm_spmm_get_shape", "vnm_gcn_layer", "vnm_batch_create",
     "vnm_batch_run", "vnm_batch_info", "vnm_batch_destroy", "vnm_nm_reorder_allpairs",
     "vnm_nm_reorder_batch", "vnm_mtx_read_ex", "vnm_mtx_write_ex", "vnm_lp_reorder_batch",
-    "vnm_mtx_read_cached", "vnm_sort_pairs",
+    "vnm_mtx_read_cached", "vnm_spmm_noalloc", "vnm_sort_pairs",
 ]
 
 LP_NPARAMS = 22784
@@ -106,7 +106,8 @@ class LPStats(ctypes.Structure):
 
 class PackedInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("V", "M", "nrows", "ncols", "nblocks", "dtype")] + \
-               [("retained_nnz", ctypes.c_int64), ("bytes", ctypes.c_int64), ("spmm_bytes", ctypes.c_int64)]
+               [("retained_nnz", ctypes.c_int64), ("bytes", ctypes.c_int64), ("spmm_bytes", ctypes.c_int64),
+                ("items", ctypes.c_int32), ("slots", ctypes.c_int32)]
 
 
 _lib = None
@@ -171,6 +172,7 @@ def lib():
     L.vnm_packed_destroy.argtypes = [vp]
     L.vnm_packed_destroy.restype = None
     L.vnm_spmm.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int]
+    L.vnm_spmm_noalloc.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int]
     L.vnm_sort_pairs.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int]
     f32p = P(ctypes.c_float)
     L.vnm_nm_trace.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p,
@@ -639,6 +641,10 @@ class Packed:
 
     def spmm(self, B_ptr, ldb, n, b_dtype, C_ptr, ldc, b_scales_ptr=None):
         _check(lib().vnm_spmm(self.h, B_ptr, ldb, n, b_dtype, b_scales_ptr, C_ptr, ldc))
+
+    def spmm_noalloc(self, B_ptr, ldb, n, b_dtype, C_ptr, ldc, b_scales_ptr=None):
+        """spmm without any device allocation (static slot array of the library)."""
+        _check(lib().vnm_spmm_noalloc(self.h, B_ptr, ldb, n, b_dtype, b_scales_ptr, C_ptr, ldc))
 
     def set_shape(self, b_dtype, n, cols_per_lane):
         """Fix the SpMM launch shape for (b_dtype, n); 0 re-enables tuning."""
