@@ -153,8 +153,10 @@ def bench_precision_131k(a):
     C = torch.empty((coo.nrows, NC), dtype=torch.float32, device=dev)
     res_dt = {}
     outs = {}
-    for name, dt in (("bf16", vnm.DT_BF16), ("fp8", vnm.DT_FP8), ("f32", vnm.DT_F32)):
-        p = g.pack(V, dt)
+    for name, dt, dmin in (("bf16", vnm.DT_BF16, 0), ("bf16_dense4", vnm.DT_BF16, 4),
+                           ("bf16_dense8", vnm.DT_BF16, 8), ("bf16_dense16", vnm.DT_BF16, 16),
+                           ("fp8", vnm.DT_FP8, 0), ("f32", vnm.DT_F32, 0)):
+        p = g.pack(V, dt, dense_min=dmin)
         if dt == vnm.DT_FP8:
             Bt, sc = fp8_rows(Bd, NC)
             sb = 1
@@ -165,6 +167,12 @@ def bench_precision_131k(a):
         wall, kms = timed_spmm(ctx, p, Bt, dt, C, NC, a.steps, a.warmup, sc)
         info = p.info()
         alg = info.spmm_bytes + coo.ncols * NC * sb + coo.nrows * NC * 4
+        if dmin:
+            ctx.timing(True)
+            for _ in range(a.steps):
+                p.spmm(Bt.data_ptr(), NC, NC, dt, C.data_ptr(), NC)
+            torch.cuda.synchronize()
+            dms, dn = ctx.timer("k_spmm_dense")
         res_dt[name] = {"cols_per_lane": p.get_shape(dt, NC), "ms_per_step": round(wall, 5),
                         "kernel_ms": round(kms, 5),
                         "tflops": round(2.0 * info.retained_nnz * NC / (wall * 1e-3) / 1e12, 4),
@@ -172,8 +180,13 @@ def bench_precision_131k(a):
                         "a_read_bytes": int(info.spmm_bytes),
                         "achieved_gbs": round(alg / (kms * 1e-3) / 1e9, 1),
                         "retained_nnz": int(info.retained_nnz)}
+        if dmin:
+            res_dt[name].update({"dense_min": dmin, "dense_blocks": int(info.dense_blocks),
+                                 "dense_chunks": int(info.dense_chunks),
+                                 "dense_kernel_ms": round(dms / max(dn, 1), 5)})
         outs[name] = C.cpu().numpy()
     head = res_dt["bf16"]
+    dense_subset = sparse_mfma_subset(vnm, ctx, out_coo, V, M, Bd.to(torch.bfloat16), NC, a.steps)
     if a.copy_calib:
         copy_calibration(dev)
     traffic = measured_traffic("k_spmm_rmat131072_deg24_V64_M4_traffic.json", "k_spmm_bf16",
@@ -195,6 +208,7 @@ def bench_precision_131k(a):
                     "init_cnt": st.init_cnt, "final_cnt": st.final_cnt,
                     "retained_nnz_before": retained0, "retained_nnz": head["retained_nnz"]},
         "variants": res_dt,
+        "sparse_mfma_on_dense_subset": dense_subset,
     }
     if not a.no_cpu:
         import oracle
@@ -205,6 +219,52 @@ def bench_precision_131k(a):
             res["variants"][name]["rel_err_fro"] = float(np.linalg.norm(d) / np.linalg.norm(ref))
             res["variants"][name]["rel_err_max"] = float(np.abs(d).max() / np.abs(ref).max())
     print(json.dumps(res), flush=True)
+
+
+def sparse_mfma_subset(vnm, ctx, coo, V, M, Bt, n, steps):
+    """The row-entry SpMM and the 2:4 sparse-MFMA SpMM on the same matrix:
+    the entries of the V x M blocks keeping at least d values (M = 4, where
+    a block keeps all 4 columns and min(2, nonzeros) per row), for a few d.
+    Kernel times from HIP events on the library stream."""
+    import numpy as np
+    import torch
+    assert M == 4
+    nz = coo.val != 0
+    r, c, v = coo.row[nz], coo.col[nz], coo.val[nz]
+    nseg = coo.ncols // M
+    rs = r.astype(np.int64) * nseg + c // M
+    u, inv, cnt = np.unique(rs, return_inverse=True, return_counts=True)
+    kept_rs = np.minimum(cnt, 2)
+    blk = (u // nseg) // V * nseg + (u % nseg)
+    bu, binv = np.unique(blk, return_inverse=True)
+    bkept = np.bincount(binv, weights=kept_rs).astype(np.int64)
+    out = {}
+    for d in (4, 8, 16):
+        keep = bkept[binv][inv] >= d
+        sub = vnm.coo_from_tuples(coo.nrows, coo.ncols, r[keep], c[keep], v[keep], M)
+        g = vnm.Graph(ctx, sub, M)
+        C = torch.empty((sub.nrows, n), dtype=torch.float32, device="cuda")
+        res = {"values": int(np.sum(kept_rs[bkept[binv] >= d])), "blocks": int(np.sum(bkept >= d))}
+        for name, dmin in (("row_entries", 0), ("sparse_mfma", 1)):
+            p = g.pack(V, vnm.DT_BF16, dense_min=dmin)
+            for _ in range(3):
+                p.spmm(Bt.data_ptr(), n, n, vnm.DT_BF16, C.data_ptr(), n)
+            torch.cuda.synchronize()
+            ctx.timing(True)
+            t = time.perf_counter()
+            for _ in range(steps):
+                p.spmm(Bt.data_ptr(), n, n, vnm.DT_BF16, C.data_ptr(), n)
+            torch.cuda.synchronize()
+            wall = (time.perf_counter() - t) / steps * 1e3
+            ks = ctx.timer("k_spmm_dense" if dmin else "k_spmm")
+            res[name] = {"ms_per_call": round(wall, 5), "kernel_ms": round(ks[0] / max(ks[1], 1), 5)}
+            if dmin:
+                info = p.info()
+                res[name].update({"chunks": int(info.dense_chunks), "retained": int(info.retained_nnz)})
+            p.close()
+        g.close()
+        out[f"dense_min_{d}"] = res
+    return out
 
 
 def copy_calibration(dev):

@@ -270,9 +270,13 @@ typedef struct {
     int32_t items;            /* SpMM work items (whole rows of at most 64
                                  row entries, or 64-entry chunks of longer
                                  rows)                                    */
-    int32_t slots;            /* partial-sum slots (one per chunk of a row
-                                 longer than 64 row entries): a product
-                                 with n columns needs slots * n floats   */
+    int32_t slots;            /* rows of partial sums: one per chunk of a
+                                 row longer than 64 row entries, plus V
+                                 per extra work unit of the sparse-MFMA
+                                 part; a product with n columns needs
+                                 slots * n floats of scratch             */
+    int32_t dense_blocks;     /* blocks in the sparse-MFMA part          */
+    int32_t dense_chunks;     /* their chunks of 16 blocks               */
 } vnm_packed_info;
 
 /* Replaces the cuSPARSELt prune/compress path (kernel/spmm.cu:402-447) and
@@ -280,6 +284,16 @@ typedef struct {
  * matrix into the VENOM format (per stored V x M block: 4 column selectors,
  * V x 2 values + 2-bit indices). */
 int  vnm_pack(vnm_graph *g, int V, int dtype, vnm_packed **out);
+/* vnm_pack with a 2:4 sparse-MFMA part, replacing cuSPARSELt's 2:4 matmul
+ * (kernel/spmm.cu:402-469) where it pays: blocks keeping at least dense_min
+ * values (0: none) are packed as v_smfmac_f32_16x16x64_bf16 operands, 16
+ * blocks of one V-row block side by side as K = 64, and vnm_spmm multiplies
+ * them on the sparse matrix cores before the row entries of the other
+ * blocks.  BF16 values only, V a multiple of 16; the product then takes BF16
+ * B, and non-finite B values reach every row of a dense block's 16-row slice
+ * (0 x Inf), not only the rows that keep a value.  Matrices with such a part
+ * are not accepted by vnm_batch_create. */
+int  vnm_pack_ex(vnm_graph *g, int V, int dtype, int dense_min, vnm_packed **out);
 int  vnm_packed_info_get(const vnm_packed *p, vnm_packed_info *info);
 void vnm_packed_destroy(vnm_packed *p);
 

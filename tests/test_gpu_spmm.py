@@ -248,3 +248,109 @@ def test_spmm_bit_identical_across_runs_and_shapes(ctx, adt, bdt, n):
                 assert fro < tol and mx < tol, (fro, mx)
             else:
                 assert np.array_equal(b, first), (v, rep)
+
+
+def _bf16_run(p, m, n, B, vec=None, bias=None):
+    if vec is not None:
+        p.set_shape(vnm.DT_BF16, n, vec)
+    Bd = torch.from_numpy(B).to(torch.bfloat16).cuda()
+    C = torch.full((m.nrows, n), float("nan"), dtype=torch.float32, device="cuda")
+    p.spmm(Bd.data_ptr(), n, n, vnm.DT_BF16, C.data_ptr(), n)
+    torch.cuda.synchronize()
+    return C.cpu().numpy()
+
+
+@pytest.mark.parametrize("case", ["rmat_m8_v64", "rmat_m4_v64", "dense_m8_v16", "chunglu_v128"])
+def test_spmm_sparse_mfma_part(ctx, case):
+    """Blocks keeping >= dense_min values on v_smfmac_f32_16x16x64_bf16: the
+    same pruned matrix (retained count unchanged), the FP64 oracle's product
+    within the BF16 tolerance, bit-identical across runs and row-entry launch
+    shapes, for thresholds that move none, some and all blocks."""
+    rng = np.random.default_rng(7)
+    if case == "rmat_m8_v64":
+        m, M, V, n = vnm.gen_rmat(13, 60000, 31), 8, 64, 128
+    elif case == "rmat_m4_v64":
+        m, M, V, n = vnm.gen_rmat(12, 40000, 32, M=4), 4, 64, 256
+    elif case == "dense_m8_v16":
+        m, M, V, n = _rand(rng, 20 * 8, 0.3, 8), 8, 16, 64
+    else:
+        m, M, V, n = vnm.gen_chung_lu(4000, 2.1, 12.0, 33), 8, 128, 64
+    g = vnm.Graph(ctx, m, M)
+    B = vnm.gen_normal(34, (m.ncols, n))
+    ref = oracle.spmm_vnm_f64(_om(m), V, M, B.astype(np.float32))
+    retained = oracle.retained_nnz(_om(m), V, 2, M)
+    # B rounded to BF16 as the kernels read it
+    Bq = torch.from_numpy(B).to(torch.bfloat16).float().numpy()
+    refq = oracle.spmm_vnm_f64(_om(m), V, M, Bq)
+    seen_dense = False
+    for dmin in (1, 4, 8, 1 << 20):
+        p = g.pack(V, vnm.DT_BF16, dense_min=dmin)
+        info = p.info()
+        assert info.retained_nnz == retained
+        if dmin == 1:
+            assert info.dense_blocks == info.nblocks
+        if dmin == 1 << 20:
+            assert info.dense_blocks == 0
+        seen_dense |= info.dense_blocks > 0
+        first = None
+        for v in _vecs(vnm.DT_BF16, n):
+            for rep in range(2):
+                C = _bf16_run(p, m, n, B, vec=v)
+                if first is None:
+                    first = C
+                    fro, mx = _errs(C, ref)
+                    assert fro < 1e-2 and mx < 1e-2, (dmin, fro, mx)
+                    fro, mx = _errs(C, refq)  # only the matrix values' rounding left
+                    assert fro < 5e-3 and mx < 5e-3, (dmin, fro, mx)
+                else:
+                    assert np.array_equal(C.view(np.int32), first.view(np.int32)), (dmin, v, rep)
+        p.close()
+    assert seen_dense
+
+
+def test_spmm_sparse_mfma_gcn_bias_and_noalloc(ctx):
+    """The GCN layer (bias in every written row, including rows only the 2:4
+    part holds and rows cut into chunks) and the allocation-free entry on a
+    matrix with a sparse-MFMA part."""
+    m = vnm.gen_rmat(13, 80000, 35)  # hub rows cut into chunks
+    g = vnm.Graph(ctx, m, 8)
+    p = g.pack(64, vnm.DT_BF16, dense_min=4)
+    info = p.info()
+    assert info.dense_blocks > 0 and info.slots > 0
+    K, Nout = 64, 128
+    X = torch.from_numpy(vnm.gen_normal(36, (m.ncols, K))).to(torch.bfloat16).cuda()
+    W = (torch.from_numpy(vnm.gen_normal(37, (K, Nout))) / 8.0).to(torch.bfloat16).cuda()
+    b = torch.linspace(-0.5, 0.5, Nout, device="cuda")
+    Y = torch.full((m.nrows, Nout), float("nan"), device="cuda")
+    p.gcn_layer(X.data_ptr(), K, K, W.data_ptr(), Nout, Y.data_ptr(), Nout, b.data_ptr())
+    torch.cuda.synchronize()
+    H = (X.double() @ W.double()).to(torch.bfloat16).float().cpu().numpy()
+    ref = oracle.spmm_vnm_f64(_om(m), 64, 8, H) + b.double().cpu().numpy()
+    fro, mx = _errs(Y.cpu().numpy(), ref)
+    assert fro < 1e-2 and mx < 1e-2, (fro, mx)
+    n = 128
+    B = vnm.gen_normal(38, (m.ncols, n))
+    C1 = _bf16_run(p, m, n, B)
+    Bd = torch.from_numpy(B).to(torch.bfloat16).cuda()
+    C2 = torch.full((m.nrows, n), float("nan"), dtype=torch.float32, device="cuda")
+    p.spmm_noalloc(Bd.data_ptr(), n, n, vnm.DT_BF16, C2.data_ptr(), n)
+    torch.cuda.synchronize()
+    assert np.array_equal(C1.view(np.int32), C2.cpu().numpy().view(np.int32))
+
+
+def test_spmm_sparse_mfma_rejects(ctx):
+    m = vnm.gen_rmat(10, 5000, 39)
+    g = vnm.Graph(ctx, m, 8)
+    with pytest.raises(vnm.VnmError):
+        g.pack(64, vnm.DT_F32, dense_min=4)      # BF16 only
+    with pytest.raises(vnm.VnmError):
+        g.pack(24, vnm.DT_BF16, dense_min=4)     # V a multiple of 16
+    p = g.pack(64, vnm.DT_BF16, dense_min=2)
+    assert p.info().dense_blocks > 0
+    B = torch.zeros((m.ncols, 64), dtype=torch.float32, device="cuda")
+    C = torch.zeros((m.nrows, 64), dtype=torch.float32, device="cuda")
+    with pytest.raises(vnm.VnmError):
+        p.spmm(B.data_ptr(), 64, 64, vnm.DT_F32, C.data_ptr(), 64)  # BF16 B only
+    Bb = torch.zeros((m.ncols, 64), dtype=torch.bfloat16, device="cuda")
+    with pytest.raises(vnm.VnmError):
+        vnm.Batch(ctx, [p], [Bb.data_ptr()], [64], 64, vnm.DT_BF16, [C.data_ptr()], [64])

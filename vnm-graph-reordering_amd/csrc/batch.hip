@@ -69,7 +69,7 @@ __global__ void k_fix_batch(const int4 *__restrict__ list, int nf, int n, const 
     if (w >= nf) return;
     const int4 e = list[w];
     const SpmmArgs &a = descs[e.x];
-    fix_row(a.C + (size_t)e.y * a.ldc, a.part, a.pn, e.z, e.w, n, a.bias);
+    fix_row(a.C + (size_t)e.y * a.ldc, a.part, a.pn, e.z, e.w, n, a.bias, false);
 }
 
 __global__ void k_tag_fix(const int4 *__restrict__ fix, int nf, int g, int4 *__restrict__ out) {
@@ -146,6 +146,8 @@ static int batch_create(vnm_ctx *ctx, vnm_packed *const *ps, int count, const vo
         const vnm_packed *p = ps[g];
         if (!p || p->ctx != ctx) return set_error(VNM_E_ARG, "batch: matrix %d missing or on another context", g);
         if (p->dtype != at) return set_error(VNM_E_ARG, "batch: every matrix needs the same value dtype");
+        if (p->ngroups > 0)
+            return set_error(VNM_E_ARG, "batch: matrix %d has a sparse-MFMA part (pack it with vnm_pack)", g);
         if (!B[g] || !C[g] || ldb[g] < n || ldc[g] < n || (bt == VNM_DT_FP8 && (!bs || !bs[g])))
             return set_error(VNM_E_ARG, "batch: operands of matrix %d", g);
         if ((ldb[g] * bsize_of(bt)) % 16 || ldc[g] % 4 || ((uintptr_t)B[g] & 15) || ((uintptr_t)C[g] & 15) ||

@@ -59,11 +59,11 @@ static int spmm_noalloc(vnm_packed *p, const void *B, int ldb, int n, int bt, co
     int rc = spmm_check(p, B, ldb, n, bt, bs, C, ldc);
     if (rc) return rc;
     if (!p->ctx->noalloc_slots) return set_error(VNM_E_HIP, "spmm_noalloc: no slot array on this device");
-    if ((size_t)p->nslots * (size_t)n > NOALLOC_FLOATS)
+    if (scratch_rows(p) * (size_t)n > NOALLOC_FLOATS)
         return set_error(VNM_E_ARG,
-                         "spmm_noalloc: %d cut-row slots x %d columns = %zu floats exceed the static slot array "
+                         "spmm_noalloc: %zu slot rows x %d columns = %zu floats exceed the static slot array "
                          "(%zu floats); use vnm_spmm",
-                         p->nslots, n, (size_t)p->nslots * (size_t)n, NOALLOC_FLOATS);
+                         scratch_rows(p), n, scratch_rows(p) * (size_t)n, NOALLOC_FLOATS);
     auto it = p->shapes.find(shape_key(bt, n));
     const int v = it != p->shapes.end() ? it->second : default_vec(bt, n);
     if (!v) return set_error(VNM_E_ARG, "spmm_noalloc: no launch shape fits n = %d", n);

@@ -299,12 +299,14 @@ __global__ void k_run_keep(const uint64_t *__restrict__ keys, const int32_t *__r
 // kept count (one atomic per row and wave; a hub row's thousands of runs
 // would otherwise serialise on one address).
 __global__ void k_row_counts(const uint64_t *__restrict__ keys, const int32_t *__restrict__ rstart,
-                             const int32_t *__restrict__ nkeep, int U, int32_t *__restrict__ rcnt,
+                             const int32_t *__restrict__ nkeep, const int32_t *__restrict__ run_blk,
+                             const int32_t *__restrict__ dmap, int U, int32_t *__restrict__ rcnt,
                              uint8_t *__restrict__ flag) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool valid = u < U;
-    const bool keep = valid && nkeep[u] > 0;
+    // runs of dense blocks (dmap >= 0) go to the sparse-MFMA part instead
+    const bool keep = valid && nkeep[u] > 0 && (dmap == nullptr || dmap[run_blk[u]] < 0);
     const int row = valid ? (int)(keys[rstart[u]] >> 32) : -1;
     if (valid) flag[u] = keep;
     const uint64_t kmask = __ballot(keep);
@@ -398,11 +400,12 @@ __global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int T, int items, int 
 // in item order (one wave each); a matrix without work items, the bias or
 // zero in every row.
 __global__ void k_spmm_fix(const int4 *__restrict__ fix, int nfix, const float *__restrict__ part, int pn, int n,
-                           float *__restrict__ C, int ldc, const float *__restrict__ bias) {
+                           float *__restrict__ C, int ldc, const float *__restrict__ bias,
+                           const uint8_t *__restrict__ dvb, int V) {
     const int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     if (w >= nfix) return;
     const int4 f = fix[w];
-    fix_row(C + (size_t)f.x * ldc, part, pn, f.y, f.z, n, bias);
+    fix_row(C + (size_t)f.x * ldc, part, pn, f.y, f.z, n, bias, dvb != nullptr && dvb[f.x / V] != 0);
 }
 
 // ------------------------------------------------------------ host
@@ -427,8 +430,9 @@ static std::vector<uint32_t> subsets4(int M) {
 // ITEM starts a new item and is cut into ITEM-entry chunks, each with a
 // partial-sum slot of its own (consecutive slots), the last chunk staying
 // open for the rows that follow.  So an item's only cut row is its first.
-// The fix-up list holds the cut rows with their slot range (the SpMM writes
-// the rows that keep nothing; without any item, the list holds every row).
+// The fix-up list holds the cut rows with their slot range and the rows of
+// runs of more than GAP_MAX rows that keep nothing (the SpMM's item walk
+// writes the shorter runs; without any item, the list holds every row).
 static int build_items(vnm_packed *p) {
     hipStream_t st = p->ctx->stream;
     const int R = p->nrows;
@@ -463,8 +467,19 @@ static int build_items(vnm_packed *p) {
         }
     }
     beg.push_back(rp[R]);
-    if (slot.empty())
+    if (slot.empty()) {
         for (int r = 0; r < R; ++r) fix.push_back(make_int4(r, 0, 0, 0));
+    } else {
+        // runs of more than GAP_MAX rows without entries: one fix-up wave per row
+        for (int r = 0; r < R;) {
+            if (rp[r + 1] != rp[r]) { ++r; continue; }
+            int e = r;
+            while (e < R && rp[e + 1] == rp[e]) ++e;
+            if (e - r > GAP_MAX)
+                for (int q = r; q < e; ++q) fix.push_back(make_int4(q, 0, 0, 0));
+            r = e;
+        }
+    }
     p->nitems = (int)slot.size();
     p->nslots = nslots;
     p->nfix = (int)fix.size();
@@ -481,8 +496,11 @@ static int build_items(vnm_packed *p) {
     return VNM_OK;
 }
 
-int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_only) {
+int pack(vnm_graph *g, int V, int dtype, int dense_min, vnm_packed **out, int64_t *retained_only) {
     const int M = g->M;
+    if (dense_min < 0 || (dense_min > 0 && (dtype != VNM_DT_BF16 || V % 16 != 0)))
+        return set_error(VNM_E_ARG, "the sparse-MFMA part needs BF16 values and V a multiple of 16 "
+                                    "(dense_min %d, dtype %d, V %d)", dense_min, dtype, V);
     if (V <= 0 || V > 256) return set_error(VNM_E_ARG, "V must be in [1, 256] (got %d)", V);
     if (M != 4 && M != 8 && M != 16) return set_error(VNM_E_ARG, "M must be 4, 8 or 16");
     if (dtype < VNM_DT_F32 || dtype > VNM_DT_FP8) return set_error(VNM_E_ARG, "bad dtype %d", dtype);
@@ -606,12 +624,20 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
         delete p;
         return VNM_OK;
     }
-    // row entries (runs that keep something), row-major
+    // dense blocks to the sparse-MFMA part
+    DBuf<int32_t> dmap;
+    if (dense_min > 0 && nb > 0 &&
+        (rc = build_dense(p, dense_min, U, nb, nblk, run_blk.p, nkeep.p, blk_first.p, bkey2.p, keys2.p, rstart.p,
+                          kv.p, kidx.p, dmap))) {
+        delete p;
+        return rc;
+    }
+    // row entries (runs of the other blocks that keep something), row-major
     VNM_HIP_TRY(hipMemsetAsync(rcnt.p, 0, (size_t)(R + 1) * 4, st));
     int E = 0;
     if (U > 0) {
         hipLaunchKernelGGL(k_row_counts, dim3(grid_of(U, 256)), dim3(256), 0, st, keys2.p, rstart.p,
-                           nkeep.p, U, rcnt.p, flag.p);
+                           nkeep.p, run_blk.p, p->ngroups > 0 ? dmap.p : nullptr, U, rcnt.p, flag.p);
         hipLaunchKernelGGL(k_iota_i, dim3(grid_of(U, 256)), dim3(256), 0, st, rid.p, U);
         if ((rc = select_flagged_i32(rid.p, flag.p, sel_runs.p, cnt.p, U, st))) { delete p; return rc; }
         VNM_HIP_TRY(hipMemcpyAsync(&E, cnt.p, 4, hipMemcpyDeviceToHost, st));
@@ -639,9 +665,13 @@ int pack(vnm_graph *g, int V, int dtype, vnm_packed **out, int64_t *retained_onl
     return VNM_OK;
 }
 
+// Rows of n floats of SpMM scratch: the partial-sum slots of rows cut into
+// chunks, then the dense units' slots (V rows each).
+size_t scratch_rows(const vnm_packed *p) { return (size_t)p->nslots + (size_t)p->ndslots * p->V; }
+
 // Partial-sum slots for n output columns, grown on demand.
 int ensure_part(vnm_packed *p, int n) {
-    const size_t need = std::max<size_t>((size_t)p->nslots * n, 1);
+    const size_t need = std::max<size_t>(scratch_rows(p) * n, 1);
     if (p->part.n >= need) return VNM_OK;
     VNM_HIP_TRY(hipStreamSynchronize(p->ctx->stream));  // the old buffer may be in use
     return p->part.alloc(need);
@@ -680,6 +710,8 @@ SpmmArgs packed_args(const vnm_packed *p, const void *B, int ldb, const float *b
     a.pn = pn;
     a.b_rows = p->ncols;
     a.c_rows = p->nrows;
+    a.dvb = p->ngroups > 0 ? p->dvb.p : nullptr;
+    a.V = p->V;
     return a;
 }
 
@@ -723,6 +755,9 @@ int spmm_run(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, cons
                     int ldc, const float *bias, float *part, bool timed) {
     hipStream_t st = p->ctx->stream;
     const SpmmArgs args = packed_args(p, B, ldb, bs, C, ldc, bias, part, n);
+    // the dense blocks' 2:4 products first: the row-entry pass adds to them
+    int rc = spmm_dense(p, (const uint16_t *)B, ldb, n, C, ldc, part + (size_t)p->nslots * n, st, timed);
+    if (rc) return rc;
     if (p->nitems > 0) {
         KernelTimer kt(timed ? p->ctx : nullptr, "k_spmm");
         switch (p->dtype) {
@@ -736,7 +771,7 @@ int spmm_run(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, cons
     }
     if (p->nfix > 0) {
         hipLaunchKernelGGL(k_spmm_fix, dim3((p->nfix + 3) / 4), dim3(256), 0, st, p->fix.p, p->nfix, part, n, n, C,
-                           ldc, bias);
+                           ldc, bias, args.dvb, p->V);
         p->ctx->launches++;
     }
     VNM_LAUNCH_CHECK("k_spmm");
@@ -799,6 +834,8 @@ int spmm_check(const vnm_packed *p, const void *B, int ldb, int n, int bt, const
                                     "units (ldb * sizeof(B value) and 4 * ldc multiples of 16)");
     if ((uint64_t)p->ncols * (uint64_t)ldb * (uint64_t)bsize_of(bt) > 0xFFFFFFFFull)
         return set_error(VNM_E_ARG, "spmm: B (ncols x ldb) must be smaller than 4 GiB");
+    if (p->ngroups > 0 && bt != VNM_DT_BF16)
+        return set_error(VNM_E_ARG, "spmm: a matrix packed with a sparse-MFMA part takes BF16 B");
     return VNM_OK;
 }
 
@@ -826,12 +863,17 @@ extern "C" {
 int vnm_retained_nnz(vnm_graph *g, int V, int N, int64_t *out) {
     if (!g || !out) return set_error(VNM_E_ARG, "null argument");
     if (N != 2) return set_error(VNM_E_ARG, "the V:N:M format keeps N = 2 per row (got %d)", N);
-    return pack(g, V, VNM_DT_F32, nullptr, out);
+    return pack(g, V, VNM_DT_F32, 0, nullptr, out);
 }
 
 int vnm_pack(vnm_graph *g, int V, int dtype, vnm_packed **out) {
     if (!g || !out) return set_error(VNM_E_ARG, "null argument");
-    return pack(g, V, dtype, out, nullptr);
+    return pack(g, V, dtype, 0, out, nullptr);
+}
+
+int vnm_pack_ex(vnm_graph *g, int V, int dtype, int dense_min, vnm_packed **out) {
+    if (!g || !out) return set_error(VNM_E_ARG, "null argument");
+    return pack(g, V, dtype, dense_min, out, nullptr);
 }
 
 int vnm_packed_info_get(const vnm_packed *p, vnm_packed_info *info) {
@@ -851,7 +893,14 @@ int vnm_packed_info_get(const vnm_packed *p, vnm_packed_info *info) {
     // blocks (rowptr is read at most twice per 64 entries)
     info->spmm_bytes = (int64_t)p->nent * (4 + 4 + 1 + 2 * vs + blk) + (blk ? (int64_t)p->nblocks * 4 : 0);
     info->items = p->nitems;
-    info->slots = p->nslots;
+    info->slots = (int32_t)scratch_rows(p);
+    info->dense_blocks = p->ndense;
+    info->dense_chunks = p->nchunks;
+    // the sparse-MFMA part reads per chunk its 64 column indices and per
+    // 16-row slice 64 lanes x (16 value bytes + 4 position bytes)
+    info->spmm_bytes += (int64_t)p->nchunks * (64 * 4 + (int64_t)(p->V / 16) * 64 * 20) + (int64_t)p->ngroups * 16;
+    info->bytes += (int64_t)p->nchunks * (64 * 4 + (int64_t)(p->V / 16) * 64 * 20) + (int64_t)p->ngroups * 16 +
+                   (p->ngroups ? (int64_t)((p->nrows + p->V - 1) / p->V) : 0);
     return VNM_OK;
 }
 

@@ -31,13 +31,23 @@ struct vnm_packed {
     vnm::DBuf<int4> fix;                                   // {row, first slot, slots, 0}: rows the
                                                            // fix-up pass writes (empty or cut)
     vnm::DBuf<float> part;                                 // nslots x n partial sums
+    // 2:4 sparse-MFMA part (dense.hip): blocks keeping >= dense_min values
+    int32_t dense_min = 0, ndense = 0, nchunks = 0, ngroups = 0, nmulti = 0, ndslots = 0;
+    vnm::DBuf<int4> dgrp;                                  // units {V-row block, first chunk, chunks, slot or -1}
+    vnm::DBuf<int4> dmulti;                                // {V-row block, first slot, slots, 0}
+    vnm::DBuf<int32_t> dcol;                               // chunks x 64 B rows (-1: none)
+    vnm::DBuf<uint32_t> dfrag;                             // chunks x V/16 x 64 lanes x 8 BF16
+    vnm::DBuf<uint32_t> didx;                              // chunks x V/16 x 64 lanes: positions
+    vnm::DBuf<uint8_t> dvb;                                // per V-row block: 1 = C holds a dense sum
     std::map<int, int> shapes;                             // (b dtype << 24 | n) -> values per lane
     vnm::DBuf<uint16_t> gcn_h;                             // X W of the last GCN layer (BF16)
 };
 
 namespace vnm {
 
-constexpr int ITEM = 64;  // row entries per work item
+constexpr int ITEM = 64;      // row entries per work item
+constexpr int GAP_MAX = 8;    // runs of rows without entries up to this long are written by the
+                              // item walk; longer runs by the fix-up pass (one wave per row)
 
 // ------------------------------------------------------------ value codecs
 
@@ -156,6 +166,22 @@ __device__ __forceinline__ void store_vec(float *__restrict__ cp, const float (&
     }
 }
 
+template <int NV>
+__device__ __forceinline__ void load_vec(const float *__restrict__ cp, float (&v)[NV]) {
+    if constexpr (NV % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; i += 4) {
+            const float4 x = *(const float4 *)(cp + i);
+            v[i] = x.x; v[i + 1] = x.y; v[i + 2] = x.z; v[i + 3] = x.w;
+        }
+    } else if constexpr (NV == 2) {
+        const float2 x = *(const float2 *)cp;
+        v[0] = x.x; v[1] = x.y;
+    } else {
+        v[0] = cp[0];
+    }
+}
+
 // One packed matrix as the SpMM kernels see it.
 struct SpmmArgs {
     const int32_t *item_beg, *item_slot, *ent_row, *ent_col;
@@ -167,9 +193,11 @@ struct SpmmArgs {
     float *C;
     float *part;        // nslots x pn partial sums of cut rows
     const float *bias;  // nullptr or n floats added to every output row
+    const uint8_t *dvb; // nullptr, or per V-row block 1 when C already holds the block's dense sums
     int items, M, ldb, ldc, pn;
     int b_rows;         // rows of B (columns of A); b_rows * ldb * sizeof(value) < 2^32
     int c_rows;         // rows of C (of A)
+    int V;
 };
 
 // VEC values of one B row for this lane, by a buffer load whose row offset
@@ -230,9 +258,9 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ erow, cons
                                           const int32_t *__restrict__ eblk, const float *__restrict__ blk_scale,
                                           const float *__restrict__ bsc, const uint8_t *__restrict__ B,
                                           float *__restrict__ C, float *__restrict__ part,
-                                          const float *__restrict__ bias, int beg, int cnt, int slot,
-                                          int lead_row, int next_row, int M, int ldb, int ldc, int pn,
-                                          int b_rows, int tile) {
+                                          const float *__restrict__ bias, const uint8_t *__restrict__ dvb,
+                                          int V, int beg, int cnt, int slot, int lead_row, int next_row, int M,
+                                          int ldb, int ldc, int pn, int b_rows, int tile) {
     constexpr int SB = bsize<BT>();
     // entries per group: up to 8; pipelined, 16 VGPRs of gathered values
     // per register set
@@ -272,18 +300,45 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ erow, cons
 #pragma unroll
     for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
     int cur = first_row;
-    // rows [r0, r1) keep nothing: the bias or zero
+    // rows whose V-row block has a dense part hold its sum already (written by
+    // the dense kernel): row-entry sums add to it
+    auto dense_row = [&](int r) { return dvb != nullptr && dvb[r / V] != 0; };
+    // rows [r0, r1) -- a maximal run without row entries -- get the bias or
+    // zero (plus a dense sum); runs longer than GAP_MAX are the fix-up
+    // pass's, so no wave walks thousands of empty rows
     auto fill = [&](int r0, int r1) {
+        if (r1 - r0 > GAP_MAX) return;
         float z[VEC];
 #pragma unroll
         for (int i = 0; i < VEC; ++i) z[i] = 0.f;
-        for (int r = r0; r < r1; ++r) store_vec<VEC>(C + (size_t)r * ldc + col0, z, bias_c);
+        for (int r = r0; r < r1; ++r) {
+            float *cp = C + (size_t)r * ldc + col0;
+            if (dense_row(r)) {
+                if (bias_c) {
+                    float d[VEC];
+                    load_vec<VEC>(cp, d);
+                    store_vec<VEC>(cp, d, bias_c);
+                }
+            } else {
+                store_vec<VEC>(cp, z, bias_c);
+            }
+        }
     };
     fill(lead_row, first_row);
     // the finished row `cur`: to its slot when it is the item's cut first row
     auto flush = [&]() {
-        if (slot >= 0 && cur == first_row) store_vec<VEC>(part + (size_t)slot * pn + col0, acc, nullptr);
-        else store_vec<VEC>(C + (size_t)cur * ldc + col0, acc, bias_c);
+        if (slot >= 0 && cur == first_row) {
+            store_vec<VEC>(part + (size_t)slot * pn + col0, acc, nullptr);
+        } else {
+            float *cp = C + (size_t)cur * ldc + col0;
+            if (dense_row(cur)) {
+                float d[VEC];
+                load_vec<VEC>(cp, d);
+#pragma unroll
+                for (int i = 0; i < VEC; ++i) acc[i] += d[i];
+            }
+            store_vec<VEC>(cp, acc, bias_c);
+        }
     };
     // U entries: broadcast, gather, then fold in (flushing on a row change);
     // CHECK only for the item's last, partial group
@@ -390,17 +445,17 @@ __device__ __forceinline__ void spmm_unit(const SpmmArgs &a, int item, int tile)
     const int lead = item == 0 ? 0 : a.c_rows;
     const int next_row = item + 1 < a.items ? a.ent_row[end] : a.c_rows;
     spmm_item<AT, BT, VEC, PIPE>(a.ent_row, a.ent_col, a.ent_idx, a.ent_val, a.ent_blk, a.blk_scale,
-                                 a.b_scales, a.B, a.C, a.part, a.bias, beg, end - beg, a.item_slot[item],
-                                 lead, next_row, a.M, a.ldb, a.ldc, a.pn, a.b_rows, tile);
+                                 a.b_scales, a.B, a.C, a.part, a.bias, a.dvb, a.V, beg, end - beg,
+                                 a.item_slot[item], lead, next_row, a.M, a.ldb, a.ldc, a.pn, a.b_rows, tile);
 }
 
 // Fix-up of one listed row (a wave): the sum of its `ns` partial-sum slots
-// from `s0` on, in slot order, plus the bias; a matrix without work items
-// lists every row with ns = 0 (the bias or zero).  The slots' loads go out
+// from `s0` on, in slot order, plus the row's dense sum when it has one,
+// plus the bias; a matrix without work items lists every row with ns = 0.  The slots' loads go out
 // 16 at a time before they are added (in order), so a row of many chunks
 // waits for a few round trips rather than one per slot.
 __device__ __forceinline__ void fix_row(float *__restrict__ crow, const float *__restrict__ part, int pn, int s0,
-                                        int ns, int n, const float *__restrict__ bias) {
+                                        int ns, int n, const float *__restrict__ bias, bool dense) {
     constexpr int FB = 16;
     const int lane = threadIdx.x & 63;
     for (int c = lane * 4; c < n; c += 256) {
@@ -421,6 +476,13 @@ __device__ __forceinline__ void fix_row(float *__restrict__ crow, const float *_
                 }
             }
         }
+        if (dense) {  // the row's dense sum, written by the dense kernel
+            const float4 d = *(const float4 *)(crow + c);
+            acc.x += d.x;
+            acc.y += d.y;
+            acc.z += d.z;
+            acc.w += d.w;
+        }
         if (bias) {
             const float4 b = *(const float4 *)(bias + c);
             acc.x += b.x;
@@ -435,6 +497,7 @@ __device__ __forceinline__ void fix_row(float *__restrict__ crow, const float *_
 // ------------------------------------------------------------ host helpers (pack.hip)
 
 int ensure_part(vnm_packed *p, int n);
+size_t scratch_rows(const vnm_packed *p);
 int bsize_of(int bt);
 bool vec_ok(int bt, int n, int vec);
 SpmmArgs packed_args(const vnm_packed *p, const void *B, int ldb, const float *bs, float *C, int ldc,
@@ -445,5 +508,10 @@ int spmm_check(const vnm_packed *p, const void *B, int ldb, int n, int bt, const
 int spmm_run(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, const float *bs, float *C, int ldc,
              const float *bias, float *part, bool timed);
 int shape_key(int bt, int n);
+int build_dense(vnm_packed *p, int dense_min, int U, int nb, int nblk, const int32_t *run_blk,
+                const int32_t *nkeep, const int32_t *blk_first, const uint64_t *bkey, const uint64_t *keys,
+                const int32_t *rstart, const float *kv, const uint8_t *kidx, DBuf<int32_t> &dmap);
+int spmm_dense(vnm_packed *p, const uint16_t *B, int ldb, int n, float *C, int ldc, float *dslot, hipStream_t st,
+               bool timed);
 
 }  // namespace vnm

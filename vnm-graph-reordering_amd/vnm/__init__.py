@@ -38,7 +38,7 @@ EXPORTS = [
     "vnm_spmm_set_shape", "vnm_spmm_get_shape", "vnm_gcn_layer", "vnm_batch_create",
     "vnm_batch_run", "vnm_batch_info", "vnm_batch_destroy", "vnm_nm_reorder_allpairs",
     "vnm_nm_reorder_batch", "vnm_mtx_read_ex", "vnm_mtx_write_ex", "vnm_lp_reorder_batch",
-    "vnm_mtx_read_cached", "vnm_spmm_noalloc", "vnm_sort_pairs",
+    "vnm_mtx_read_cached", "vnm_spmm_noalloc", "vnm_sort_pairs", "vnm_pack_ex",
 ]
 
 LP_NPARAMS = 22784
@@ -107,7 +107,8 @@ class LPStats(ctypes.Structure):
 class PackedInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("V", "M", "nrows", "ncols", "nblocks", "dtype")] + \
                [("retained_nnz", ctypes.c_int64), ("bytes", ctypes.c_int64), ("spmm_bytes", ctypes.c_int64),
-                ("items", ctypes.c_int32), ("slots", ctypes.c_int32)]
+                ("items", ctypes.c_int32), ("slots", ctypes.c_int32), ("dense_blocks", ctypes.c_int32),
+                ("dense_chunks", ctypes.c_int32)]
 
 
 _lib = None
@@ -173,6 +174,7 @@ def lib():
     L.vnm_packed_destroy.restype = None
     L.vnm_spmm.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int]
     L.vnm_spmm_noalloc.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int]
+    L.vnm_pack_ex.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(vp)]
     L.vnm_sort_pairs.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int]
     f32p = P(ctypes.c_float)
     L.vnm_nm_trace.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p,
@@ -558,8 +560,10 @@ class Graph:
         _check(lib().vnm_retained_nnz(self.h, V, N, ctypes.byref(out)))
         return out.value
 
-    def pack(self, V, dtype):
-        return Packed(self, V, dtype)
+    def pack(self, V, dtype, dense_min=0):
+        """V:N:M packing; dense_min > 0 puts blocks keeping at least that many
+        values on the 2:4 sparse matrix cores (BF16 only)."""
+        return Packed(self, V, dtype, dense_min)
 
 
 class LPModel:
@@ -627,9 +631,12 @@ class LPModel:
 
 
 class Packed:
-    def __init__(self, graph, V, dtype):
+    def __init__(self, graph, V, dtype, dense_min=0):
         h = ctypes.c_void_p()
-        _check(lib().vnm_pack(graph.h, V, dtype, ctypes.byref(h)))
+        if dense_min:
+            _check(lib().vnm_pack_ex(graph.h, V, dtype, dense_min, ctypes.byref(h)))
+        else:
+            _check(lib().vnm_pack(graph.h, V, dtype, ctypes.byref(h)))
         self.h = h
         self.graph = graph
         graph.ctx._adopt(self)
