@@ -453,3 +453,29 @@ def test_venom_m16_matches_literal():
         got = oracle.venom_reorder(om, 16, V)
         assert got.tolist() == want[0], t
         assert om.row.tolist() == want[1] and om.col.tolist() == want[2]
+
+
+def test_config_goldens_pin_the_generators():
+    """The oracle runs in tests/golden/configs.json were made on these seeded
+    inputs: the generators still produce them bit for bit (the GPU tests
+    compare against those runs)."""
+    import vnm
+    from golden.make_config_golden import CHUNGLU_SAMPLE, chunglu_sizes
+    gold = json.load(open(os.path.join(GOLD, "configs.json")))
+
+    def digest(m):
+        h = hashlib.sha256()
+        for a in (m.row, m.col, m.val):
+            h.update(np.ascontiguousarray(a).tobytes())
+        return h.hexdigest()
+    assert digest(vnm.gen_rmat(16, 524288, 2, M=8)) == gold["rmat65536_hybrid"]["input_sha256"]
+    assert digest(vnm.gen_rmat(17, 1572864, 4, M=4)) == gold["rmat131072_sched0"]["input_sha256"]
+    for i in CHUNGLU_SAMPLE:
+        m = vnm.gen_chung_lu(chunglu_sizes()[i], 2.1, 12.0, 2000 + i, M=8)
+        assert digest(m) == gold[f"chunglu{i}_sched0"]["input_sha256"]
+    # a small sampled graph's oracle run is cheap: rerun it against the record
+    m = vnm.gen_chung_lu(chunglu_sizes()[21], 2.1, 12.0, 2021, M=8)
+    om = oracle.Matrix(m.nrows, m.ncols, m.row.copy(), m.col.copy(), m.val.copy())
+    st, _ = oracle.nm_reorder(om, 2, 8, 0, 10)
+    assert st.tolist() == gold["chunglu21_sched0"]["stats"]
+    assert digest(om) == gold["chunglu21_sched0"]["sha256"]
