@@ -57,6 +57,21 @@ def test_loader_matches_oracle_on_fixtures(M):
             assert _same(vnm.mtx_read(p, M), oracle.mtx_read(p, M)), f
 
 
+def test_loader_matches_reference_loader_corners():
+    # the library's loader (sequential and threaded) against the reference
+    # loader's own output on the corner-case files (ref_load.json)
+    import json
+    for rec in json.load(open(os.path.join(GOLD, "ref_load.json"))):
+        p = os.path.join(GOLD, rec["dir"], rec["file"])
+        for threads in (1, 0):
+            m = vnm.mtx_read(p, rec["M"], threads=threads)
+            assert (m.nrows, m.ncols, m.nnz) == (rec["nrows"], rec["ncols"], rec["nnz"]), rec["file"]
+            if "row" in rec:
+                assert m.row.tolist() == rec["row"], rec["file"]
+                assert m.col.tolist() == rec["col"], rec["file"]
+                assert np.array_equal(m.val, np.array(rec["val"], np.float32)), rec["file"]
+
+
 def test_tuples_with_duplicates_and_symmetry():
     rng = np.random.default_rng(3)
     for t in range(30):
