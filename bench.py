@@ -99,6 +99,99 @@ def learned_vs_exact(vnm, ctx, coo, N, M, V, exact_scheds, topk=4096, window=16,
     out["best_exact"] = [k for k, v in out["exact"].items() if v is best][0]
     out["speedup_vs_best_exact"] = round(best["ms"] / t_eq, 2) if t_eq else None
     out["learned_retained_over_best_exact"] = round(full["retained_nnz"] / best["retained_nnz"], 5)
+    # the proposer's ablation baselines under the same exact judge and
+    # acceptance rule: seeded random proposals, every window pair scored
+    out["ablation"] = {}
+    for name, mode in (("random", vnm.LP_RANDOM), ("exhaustive_window", vnm.LP_EXHAUSTIVE)):
+        g = vnm.Graph(ctx, coo, M)
+        sa = g.lp_reorder(None, N, mode=mode, topk=topk, window=window, maxiter=maxiter)
+        ra = g.retained_nnz(V)
+        g.close()
+        t_a = None
+        for k in range(1, sa.iters + 1):
+            g = vnm.Graph(ctx, coo, M)
+            sk = g.lp_reorder(None, N, mode=mode, topk=topk, window=window, maxiter=k)
+            r = g.retained_nnz(V)
+            g.close()
+            if r >= best["retained_nnz"]:
+                t_a = sk.ms
+                break
+        out["ablation"][name] = {"ms": round(sa.ms, 2), "final_cnt": sa.final_cnt, "iters": sa.iters,
+                                 "exact_pairs": sa.exact_pairs, "retained_nnz": ra,
+                                 "ms_to_exact_retained": round(t_a, 2) if t_a is not None else None}
+    return out
+
+
+def _digest(m):
+    import hashlib
+    import numpy as np
+    h = hashlib.sha256()
+    for arr in (m.row, m.col, m.val):
+        h.update(np.ascontiguousarray(arr).tobytes())
+    return h.hexdigest()
+
+
+def cpu_reference_path_c1(vnm, ctx, reps=3):
+    """The reference's timed region (main.cu:210-319: the first N:M run, then
+    10 rounds of VENOM + N:M) on its CPU-runnable case: ER 2,048 x 2,048,
+    41,943 entries (seed 1), V:N:M 64:2:8, schedule 0 fixed.  CPU: the oracle
+    (the reference's algorithm restated in C++) on 1 thread and on all host
+    cores; GPU: vnm_hybrid.  All three give the same 15 CSV fields and the
+    same relabelled matrix (asserted).  Best of `reps` runs each."""
+    import oracle
+    import torch
+    m = vnm.gen_er(2048, 41943, 1)
+    cores = max(1, min(16, os.cpu_count() or 1))
+    cpu = {}
+    for th in (1, cores):
+        oracle.set_threads(th)
+        best = None
+        for _ in range(reps):
+            om = oracle.Matrix(m.nrows, m.ncols, m.row.copy(), m.col.copy(), m.val.copy())
+            t = time.perf_counter()
+            st, out = oracle.hybrid(om, 2, 8, 64, 10, 0)
+            dt = (time.perf_counter() - t) * 1e3
+            best = dt if best is None or dt < best else best
+        cpu[th] = (best, [float(x) for x in st[:15]], _digest(out))
+    oracle.set_threads(1)
+    assert cpu[1][1:] == cpu[cores][1:], "oracle threads changed the result"
+    gbest, ginner = None, None
+    for _ in range(reps):
+        g = vnm.Graph(ctx, m, 8)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        hs = g.hybrid(2, 64, 10, 0)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) * 1e3
+        gstats, gdig = [float(x) for x in hs.as_tuple()[:15]], _digest(g.download())
+        g.close()
+        assert (gstats, gdig) == tuple(cpu[1][1:]), "GPU hybrid differs from the oracle"
+        gbest = dt if gbest is None or dt < gbest else gbest
+        ginner = hs.milliseconds if ginner is None or hs.milliseconds < ginner else ginner
+    return {"workload": "er2048_1pct_V64_N2_M8_maxiter10_sched0",
+            "cpu_1thread_ms": round(cpu[1][0], 2), "cpu_all_cores_ms": round(cpu[cores][0], 2),
+            "cores": cores, "gpu_ms": round(gbest, 2), "gpu_timed_region_ms": round(ginner, 2),
+            "gpu_speedup_vs_1thread": round(cpu[1][0] / gbest, 2),
+            "outputs_identical": True, "csv_fields": cpu[1][1]}
+
+
+def reorder_rooflines(ctx, nnz, R, nseg):
+    """HBM rooflines of the reorder passes from the context's kernel timers,
+    with the algorithmic bytes per unit of work (§8(d) of the build plan):
+    locate 4 nnz + 4 (R + 1) + 4 nseg per call; one VENOM iteration
+    2 (4 nnz + 4 (R + 1)) + 8 R; the relabel 16 nnz + 4 R (row and column
+    read and written, one id table read)."""
+    out = {}
+    for name, unit in (("k_locate_all", 4 * nnz + 4 * (R + 1) + 4 * nseg),
+                       ("venom_iteration", 2 * (4 * nnz + 4 * (R + 1)) + 8 * R),
+                       ("k_relabel", 16 * nnz + 4 * R)):
+        ms, n = ctx.timer(name)
+        if n == 0:
+            continue
+        avg = ms / n
+        gbs = unit / (avg * 1e-3) / 1e9
+        out[name] = {"calls": n, "avg_ms": round(avg, 5), "alg_bytes": int(unit),
+                     "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     return out
 
 
@@ -392,6 +485,60 @@ def bench_batch64(a):
     print(json.dumps(res), flush=True)
 
 
+def hybrid_exact_vs_learned(vnm, ctx, coo, N, M, V, rounds):
+    """hybrid_reordering as configured (main.cu:195-337, --maxiter rounds of
+    VENOM + N:M, the N:M schedule chosen by find_best_2toX, main.cu:348)
+    against the same hybrid with the learned proposer as its N:M step: GPU
+    time, final counts and retained nnz, and the time the learned hybrid needs
+    to reach the exact hybrid's retained nnz (the fewest rounds k whose
+    output retains at least as much)."""
+    import torch
+    model = vnm.LPModel.load(ctx, WEIGHTS)
+    g = vnm.Graph(ctx, coo, M)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    best = g.find_best_sched(N)
+    torch.cuda.synchronize()
+    sel_ms = (time.perf_counter() - t) * 1e3
+    g.close()
+    g = vnm.Graph(ctx, coo, M)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    hs = g.hybrid(N, V, rounds, best)
+    torch.cuda.synchronize()
+    ex_ms = (time.perf_counter() - t) * 1e3
+    ex_ret = g.retained_nnz(V)
+    g.close()
+    out = {"rounds": rounds, "exact": {"sched": best, "find_best_ms": round(sel_ms, 2), "ms": round(ex_ms, 2),
+                                       "final_cnt": int(hs.final_cnt), "final_vnm_blocks": hs.final_mb,
+                                       "nm_rounds": hs.nm_rounds, "vnm_rounds": hs.vnm_rounds,
+                                       "retained_nnz": ex_ret}}
+    def learned(k):
+        g = vnm.Graph(ctx, coo, M)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        hl = g.hybrid_learned(model, N, V, k, topk=4096, window=16, maxiter=40)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) * 1e3
+        r = g.retained_nnz(V)
+        g.close()
+        return hl, ms, r
+    hl, l_ms, l_ret = learned(rounds)
+    out["learned"] = {"ms": round(l_ms, 2), "final_cnt": int(hl.final_cnt), "final_vnm_blocks": hl.final_mb,
+                      "nm_iterations": hl.nm_rounds, "vnm_rounds": hl.vnm_rounds, "retained_nnz": l_ret}
+    t_eq = k_eq = None
+    for k in range(0, rounds + 1):
+        _, ms, r = learned(k)
+        if r >= ex_ret:
+            t_eq, k_eq = ms, k
+            break
+    out["learned_ms_to_exact_retained"] = round(t_eq, 2) if t_eq is not None else None
+    out["learned_rounds_to_exact_retained"] = k_eq
+    out["speedup_vs_exact"] = round(ex_ms / t_eq, 2) if t_eq else None
+    out["learned_retained_over_exact"] = round(l_ret / ex_ret, 5)
+    return out
+
+
 def bench_learned_262k(a):
     import torch
     import vnm
@@ -400,6 +547,8 @@ def bench_learned_262k(a):
     ctx = vnm.Context(0)
     torch.cuda.synchronize()
     lv = learned_vs_exact(vnm, ctx, coo, N, M, V, exact_scheds=(0, 1, 2, 3))
+    print(f"[bench] single-run comparison done", file=sys.stderr, flush=True)
+    hyb = hybrid_exact_vs_learned(vnm, ctx, coo, N, M, V, 20)
     res = {
         "metric": "reorder time to equal retained nnz (learned vs exact)",
         "value": lv["speedup_vs_best_exact"], "unit": "x (exact GPU time / learned GPU time)",
@@ -410,6 +559,7 @@ def bench_learned_262k(a):
         "config": {"workload": "rmat262144_deg32_V128_N2_M8", "nodes": coo.nrows, "nnz": coo.nnz,
                    "V": V, "N": N, "M": M, "topk": 4096, "window": 16, "parallelism": "none (1 GPU)"},
         "learned_vs_exact": lv,
+        "hybrid_maxiter20": hyb,
     }
     print(json.dumps(res), flush=True)
 
@@ -448,6 +598,7 @@ def main():
     torch.cuda.synchronize()
     reorder_wall_ms = (time.time() - t0) * 1e3
     greedy_ms, greedy_n = ctx.timer("nm_greedy")
+    reorder_roof = reorder_rooflines(ctx, coo.nnz, coo.nrows, coo.nrows // M)
     print(f"[bench] reorder done: {reorder_wall_ms:.1f} ms, nm_rounds={hs.nm_rounds} "
           f"vnm_rounds={hs.vnm_rounds}", file=sys.stderr, flush=True)
     retained = g.retained_nnz(V)
@@ -530,7 +681,8 @@ def main():
                     "nm_rounds": hs.nm_rounds, "vnm_rounds": hs.vnm_rounds,
                     "init_cnt": hs.init_cnt, "final_cnt": hs.final_cnt,
                     "init_vblocks": hs.init_mb, "final_vblocks": hs.final_mb,
-                    "retained_nnz": retained, "nnz_value_nonzero": int((out_coo.val != 0).sum())},
+                    "retained_nnz": retained, "nnz_value_nonzero": int((out_coo.val != 0).sum()),
+                    "rooflines": reorder_roof},
         "variants": {"bf16_tflops": round(flops / (ms_bf * 1e-3) / 1e12, 4),
                      "bf16_kernel_gbs": round(bf_bytes / (k_bf * 1e-3) / 1e9, 1),
                      "fp8_tflops": round(flops / (ms_8 * 1e-3) / 1e12, 4)},
@@ -542,6 +694,8 @@ def main():
 
     if not a.no_cpu:
         import oracle
+        res["cpu_reference_path"] = cpu_reference_path_c1(vnm, ctx)
+        print(f"[bench] reference CPU path done", file=sys.stderr, flush=True)
         om = oracle.Matrix(out_coo.nrows, out_coo.ncols, out_coo.row, out_coo.col, out_coo.val)
         t = time.perf_counter()
         ref = oracle.spmm_vnm_f64(om, V, M, B)
@@ -558,10 +712,18 @@ def main():
                                 "fp8_rel_err_fro": e8[0], "fp8_rel_err_max": e8[1]})
         # bounded CPU sample of the reorder: the first 40 rounds of the first
         # N:M run of the same workload
-        om0 = oracle.Matrix(coo.nrows, coo.ncols, coo.row, coo.col, coo.val)
+        om0 = oracle.Matrix(coo.nrows, coo.ncols, coo.row.copy(), coo.col.copy(), coo.val.copy())
         t = time.perf_counter()
         st = oracle.nm_reorder_sample(om0, N, M, 0, 10, 40)
         cpu_round_ms = (time.perf_counter() - t) * 1e3 / max(int(st[4]), 1)
+        ncores_r = max(1, min(16, os.cpu_count() or 1))
+        oracle.set_threads(ncores_r)
+        om1 = oracle.Matrix(coo.nrows, coo.ncols, coo.row.copy(), coo.col.copy(), coo.val.copy())
+        t = time.perf_counter()
+        st1 = oracle.nm_reorder_sample(om1, N, M, 0, 10, 40)
+        cpu_round_ms_mt = (time.perf_counter() - t) * 1e3 / max(int(st1[4]), 1)
+        oracle.set_threads(1)
+        assert list(st1) == list(st) and np.array_equal(om1.row, om0.row) and np.array_equal(om1.col, om0.col)
         # the same product on all host cores (OpenMP; bit-identical result),
         # timed after one untimed call that starts the thread pool
         ncores = max(1, min(16, os.cpu_count() or 1))
@@ -580,6 +742,7 @@ def main():
                       f"({cpu_round_ms:.1f} ms/round on the CPU vs "
                       f"{greedy_ms / max(hs.nm_rounds + greedy_n, 1):.3f} ms/round GPU)",
             "cpu_reorder_ms_per_round": round(cpu_round_ms, 3),
+            "cpu_reorder_ms_per_round_all_cores": round(cpu_round_ms_mt, 3),
             "host_cpus": os.cpu_count(),
         }
     print(json.dumps(res), flush=True)

@@ -109,8 +109,10 @@ void *vnm_ctx_stream(vnm_ctx *ctx);
 int64_t vnm_ctx_launches(vnm_ctx *ctx);
 /* Kernel timers (the reference's GpuTimer / START_TIMER, util/utility.h:
  * 177-186,267-301): when enabled, HIP events bracket each launch of
- * k_spmm, k_score_rows and k_locate_all, and each whole N:M greedy run
- * ("nm_greedy"), on the context stream.
+ * k_spmm, k_spmm_dense, k_spmm_batch, k_gemm_xw, k_score_rows, k_locate_all
+ * and k_relabel, each VENOM iteration ("venom_iteration"), each proposer
+ * forward pass ("lp_forward") and pair scoring ("k_lp_pair_gain"), and each
+ * whole N:M greedy run ("nm_greedy"), on the context stream.
  * Enabling or disabling resets the totals; reading waits for the events. */
 int vnm_ctx_timing(vnm_ctx *ctx, int enable);
 int vnm_ctx_timer_read(vnm_ctx *ctx, const char *kernel, double *total_ms, int64_t *launches);

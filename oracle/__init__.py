@@ -76,6 +76,8 @@ def lib():
                                      i32p, P(SwapRec), ctypes.c_int64, P(ctypes.c_int64)]
         L.orc_venom_eval.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, i32p, i32p]
         L.orc_venom_reorder.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = None
         L.orc_hybrid.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, P(HybridStats)]
         L.orc_find_best_sched.argtypes = [P(Coo), ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p]
@@ -228,6 +230,11 @@ def venom_reorder(m, M, V, maxiter=10):
            "venom_reorder")
     m.sync(c)
     return st
+
+
+def set_threads(n):
+    """Host threads of the reorder path (1 = sequential; results identical)."""
+    lib().orc_set_threads(int(n))
 
 
 def hybrid(m, N, M, V, rounds, sched):

@@ -109,6 +109,9 @@ typedef struct {
 /* Runs the hybrid loop in memory on *coo (already loaded), emulating the
  * temp.mtx round trips.  On return *coo holds the final matrix in the order
  * the reference writes --outmtxfile. */
+/* Host threads of the reorder path (1: the reference's sequential loops;
+ * more: OpenMP over its row and segment loops, same results). */
+void orc_set_threads(int threads);
 int orc_hybrid(orc_coo *coo, int N, int M, int V, int rounds, int sched,
                orc_hybrid_stats *st);
 /* find_best_2toX (main.cu:72-114) over schedules 0..nsched-1 on copies of

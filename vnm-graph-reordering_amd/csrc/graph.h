@@ -131,6 +131,9 @@ struct KernelTimer {
     hipEvent_t a = nullptr, b = nullptr;
     KernelTimer(vnm_ctx *c, const char *n);
     void stop();
+    ~KernelTimer() { stop(); }  // a scope left early still records its end
+    KernelTimer(const KernelTimer &) = delete;
+    KernelTimer &operator=(const KernelTimer &) = delete;
 };
 
 }  // namespace vnm

@@ -1399,8 +1399,10 @@ int max_ids(vnm_graph *g, const int32_t *row, const int32_t *col, int32_t *max_r
 int relabel(vnm_graph *g, const int32_t *d_perm) {
     hipStream_t st = g->ctx->stream;
     g->epoch++;
+    KernelTimer kt(g->ctx, "k_relabel");
     hipLaunchKernelGGL(k_relabel, dim3(grid_for(g->nnz, 256)), dim3(256), 0, st, g->row.p,
                        g->col.p, g->nnz, d_perm, g->nrows);
+    kt.stop();
     hipLaunchKernelGGL(k_compose, dim3(grid_for((int64_t)g->perm_total.n, 256)), dim3(256), 0, st,
                        g->perm_total.p, (int64_t)g->perm_total.n, d_perm, g->nrows);
     g->ctx->launches += 2;

@@ -507,6 +507,9 @@ int venom_reorder(vnm_graph *g, int V, int maxiter, vnm_venom_stats *st) {
     int32_t *cur_r = g->row.p, *cur_c = g->col.p, *nxt_r = w.row2.p, *nxt_c = w.col2.p;
     int final_round = -1;
     while (inv > 0 && round <= MAXITER) {
+        // one iteration of reordering.py's loop: row keys, stable sort,
+        // relabel, recount (the host reads the counts)
+        KernelTimer kt(g->ctx, "venom_iteration");
         if ((rc = w.sort_rows())) return rc;
         hipLaunchKernelGGL(k_venom_relabel, dim3(grid_of(g->nnz, 256)), dim3(256), 0, s, cur_r,
                            cur_c, g->nnz, w.order.p, nxt_r, nxt_c);
@@ -529,7 +532,9 @@ int venom_reorder(vnm_graph *g, int V, int maxiter, vnm_venom_stats *st) {
             }
         }
         int strict;
-        if ((rc = w.count(nxt_r, nxt_c, &inv, &strict))) return rc;
+        rc = w.count(nxt_r, nxt_c, &inv, &strict);
+        kt.stop();
+        if (rc) return rc;
         if (inv > prev) {  // reordering.py:242-246: keep the previous positions
             final_round = round;
             break;
