@@ -199,6 +199,7 @@ def timed_spmm(ctx, packed, Bt, bdt, C, n, steps, warmup, scales=None):
     """(wall ms per call, k_spmm ms per launch) over `steps` calls."""
     import torch
     sp = scales.data_ptr() if scales is not None else None
+    torch.cuda.synchronize()  # B and C were made on torch's stream; the library runs on its own
     for _ in range(warmup):
         packed.spmm(Bt.data_ptr(), n, n, bdt, C.data_ptr(), n, sp)
     torch.cuda.synchronize()
@@ -340,6 +341,7 @@ def sparse_mfma_subset(vnm, ctx, coo, V, M, Bt, n, steps):
         res = {"values": int(np.sum(kept_rs[bkept[binv] >= d])), "blocks": int(np.sum(bkept >= d))}
         for name, dmin in (("row_entries", 0), ("sparse_mfma", 1)):
             p = g.pack(V, vnm.DT_BF16, dense_min=dmin)
+            torch.cuda.synchronize()  # C was made on torch's stream
             for _ in range(3):
                 p.spmm(Bt.data_ptr(), n, n, vnm.DT_BF16, C.data_ptr(), n)
             torch.cuda.synchronize()
@@ -417,6 +419,7 @@ def bench_batch64(a):
     Xs = [torch.from_numpy(vnm.gen_normal(5000 + i, (m.ncols, NC))).to(torch.bfloat16).to(dev)
           for i, m in enumerate(coos)]
     Ys = [torch.empty((m.nrows, NC), dtype=torch.float32, device=dev) for m in coos]
+    torch.cuda.synchronize()  # X and Y were made on torch's stream
     t = time.perf_counter()
     batch = vnm.Batch(ctx, packed, [x.data_ptr() for x in Xs], [NC] * 64, NC, vnm.DT_BF16,
                       [y.data_ptr() for y in Ys], [NC] * 64)
@@ -609,6 +612,7 @@ def main():
     C = torch.empty((coo.nrows, NCOLS), dtype=torch.float32, device=dev)
 
     def run(packed, Bt, bdt, steps, scales=None):
+        torch.cuda.synchronize()  # B and C were made on torch's stream; the library runs on its own
         for _ in range(a.warmup):
             packed.spmm(Bt.data_ptr(), NCOLS, NCOLS, bdt, C.data_ptr(), NCOLS,
                         scales.data_ptr() if scales is not None else None)

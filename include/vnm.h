@@ -4,7 +4,11 @@
  * Plain C types only.  Every call returns an int status: VNM_OK (0) or a
  * negative VNM_E_* code; vnm_last_error() gives the message of the last
  * failure on the calling thread.  Nothing is swallowed: HIP failures surface
- * as VNM_E_HIP.  Device work runs on the context's own HIP stream.
+ * as VNM_E_HIP.  Device work runs on the context's own HIP stream, created
+ * non-blocking: it does not wait for work on other streams (torch's current
+ * stream included).  Device buffers handed to a call must be complete when
+ * the call is made -- synchronize the stream that wrote them, or make the
+ * context stream (vnm_ctx_stream) wait on an event recorded after the writes.
  *
  * Each entry point names the reference interface it replaces
  * (PictureResearch/vnm-graph-reordering, file:line).

@@ -57,6 +57,7 @@ def test_batch_replays_match_single_and_oracle(ctx, bdt, n):
     Bs = [torch.from_numpy(vnm.gen_normal(7 + i, (m.ncols, n))).to(tdt).cuda() for i, m in enumerate(coos)]
     Cs = [torch.empty((m.nrows, n), dtype=torch.float32, device="cuda") for m in coos]
     bias = [torch.linspace(-1, 1, n, device="cuda") if i % 2 else None for i in range(len(coos))]
+    torch.cuda.synchronize()  # inputs made on torch's stream are complete
     batch = vnm.Batch(ctx, packed, [b.data_ptr() for b in Bs], [n] * len(coos), n, bdt,
                       [c.data_ptr() for c in Cs], [n] * len(coos),
                       bias=[b.data_ptr() if b is not None else 0 for b in bias])
@@ -66,6 +67,7 @@ def test_batch_replays_match_single_and_oracle(ctx, bdt, n):
     single = []
     for p, B, bi in zip(packed, Bs, bias):
         C1 = torch.empty((p.info().nrows, n), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()  # inputs made on torch's stream are complete
         p.spmm(B.data_ptr(), n, n, bdt, C1.data_ptr(), n)
         torch.cuda.synchronize()
         single.append(C1.cpu().numpy() + (bi.cpu().numpy() if bi is not None else 0))
@@ -98,6 +100,7 @@ def test_gcn_layer_matches_oracle(ctx):
     W = (torch.from_numpy(vnm.gen_normal(2, (K, Nout))) / np.sqrt(K)).to(torch.bfloat16).cuda()
     b = torch.linspace(-0.5, 0.5, Nout, device="cuda")
     Y = torch.full((m.nrows, Nout), float("nan"), device="cuda")
+    torch.cuda.synchronize()  # inputs made on torch's stream are complete
     p.gcn_layer(X.data_ptr(), K, K, W.data_ptr(), Nout, Y.data_ptr(), Nout, b.data_ptr())
     torch.cuda.synchronize()
     H = (X.double() @ W.double()).to(torch.bfloat16).float().cpu().numpy()

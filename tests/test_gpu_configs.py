@@ -69,6 +69,7 @@ def _e4m3_rows(Bd):
 
 def _spmm(p, Bt, bdt, n, nrows, sc=None):
     C = torch.full((nrows, n), float("nan"), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()  # inputs made on torch's stream are complete
     p.spmm(Bt.data_ptr(), n, n, bdt, C.data_ptr(), n, sc.data_ptr() if sc is not None else None)
     torch.cuda.synchronize()
     return C.cpu().numpy()
@@ -236,6 +237,7 @@ def test_chunglu64_learned_batch_gcn_aggregation(ctx):
     Xs = [torch.from_numpy(vnm.gen_normal(5000 + i, (m.ncols, n))).to(torch.bfloat16).cuda()
           for i, m in enumerate(coos)]
     Ys = [torch.full((m.nrows, n), float("nan"), dtype=torch.float32, device="cuda") for m in coos]
+    torch.cuda.synchronize()  # inputs made on torch's stream are complete
     batch = vnm.Batch(ctx, packed, [x.data_ptr() for x in Xs], [n] * 64, n, vnm.DT_BF16,
                       [y.data_ptr() for y in Ys], [n] * 64)
     for y in Ys:

@@ -45,6 +45,7 @@ def test_sort_pairs_matches_stable_argsort(ctx, n):
             vin = torch.arange(n, dtype=torch.int32, device="cuda")
             kout = torch.empty_like(kin)
             vout = torch.empty_like(vin)
+            torch.cuda.synchronize()  # inputs made on torch's stream are complete
             ctx.sort_pairs(kin.data_ptr(), kout.data_ptr(), n, end_bit, vin.data_ptr(), vout.data_ptr())
             torch.cuda.synchronize()
             assert np.array_equal(kout.cpu().numpy().view(np.uint64), keys[order]), (kind, end_bit)
@@ -52,6 +53,7 @@ def test_sort_pairs_matches_stable_argsort(ctx, n):
             assert np.array_equal(kin.cpu().numpy().view(np.uint64), keys)  # input untouched
             if end_bit in (13, 64):
                 k2 = torch.empty_like(kin)
+                torch.cuda.synchronize()  # inputs made on torch's stream are complete
                 ctx.sort_pairs(kin.data_ptr(), k2.data_ptr(), n, end_bit)
                 torch.cuda.synchronize()
                 assert np.array_equal(k2.cpu().numpy().view(np.uint64), keys[order]), (kind, end_bit)

@@ -100,6 +100,7 @@ def _run(ctx, m, M, V, n, adt, bdt, seed=5, vec=None):
         Bd = q.to(dev)
         scales_d = torch.from_numpy(sc).to(dev)
     C = torch.full((m.nrows, n), float("nan"), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()  # inputs made on torch's stream are complete
     p.spmm(Bd.data_ptr(), n, n, bdt, C.data_ptr(), n,
            scales_d.data_ptr() if scales_d is not None else None)
     torch.cuda.synchronize()
@@ -183,6 +184,7 @@ def test_spmm_tuner_picks_a_usable_shape(ctx):
     assert p.get_shape(vnm.DT_BF16, 128) == 0
     B = torch.zeros((m.ncols, 128), dtype=torch.bfloat16, device="cuda")
     C = torch.empty((m.nrows, 128), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()  # inputs made on torch's stream are complete
     p.spmm(B.data_ptr(), 128, 128, vnm.DT_BF16, C.data_ptr(), 128)
     assert p.get_shape(vnm.DT_BF16, 128) in _vecs(vnm.DT_BF16, 128)
     with pytest.raises(vnm.VnmError):
@@ -206,6 +208,7 @@ def test_fp8_decode_every_code(ctx):
     ref = Bd.to(torch.float32)
     for v in _vecs(vnm.DT_FP8, n):
         p.set_shape(vnm.DT_FP8, n, v)
+        torch.cuda.synchronize()  # inputs made on torch's stream are complete
         p.spmm(Bd.data_ptr(), n, n, vnm.DT_FP8, C.data_ptr(), n, sc.data_ptr())
         assert torch.equal(C, ref), v
 
@@ -236,6 +239,7 @@ def test_spmm_bit_identical_across_runs_and_shapes(ctx, adt, bdt, n):
         p.set_shape(bdt, n, v)
         for rep in range(2):
             C = torch.full((m.nrows, n), float("nan"), dtype=torch.float32, device=dev)
+            torch.cuda.synchronize()  # inputs made on torch's stream are complete
             p.spmm(Bd.data_ptr(), n, n, bdt, C.data_ptr(), n, sc.data_ptr() if sc is not None else None)
             torch.cuda.synchronize()
             b = _bits(C)
@@ -255,6 +259,7 @@ def _bf16_run(p, m, n, B, vec=None, bias=None):
         p.set_shape(vnm.DT_BF16, n, vec)
     Bd = torch.from_numpy(B).to(torch.bfloat16).cuda()
     C = torch.full((m.nrows, n), float("nan"), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()  # inputs made on torch's stream are complete
     p.spmm(Bd.data_ptr(), n, n, vnm.DT_BF16, C.data_ptr(), n)
     torch.cuda.synchronize()
     return C.cpu().numpy()
@@ -322,6 +327,7 @@ def test_spmm_sparse_mfma_gcn_bias_and_noalloc(ctx):
     W = (torch.from_numpy(vnm.gen_normal(37, (K, Nout))) / 8.0).to(torch.bfloat16).cuda()
     b = torch.linspace(-0.5, 0.5, Nout, device="cuda")
     Y = torch.full((m.nrows, Nout), float("nan"), device="cuda")
+    torch.cuda.synchronize()  # inputs made on torch's stream are complete
     p.gcn_layer(X.data_ptr(), K, K, W.data_ptr(), Nout, Y.data_ptr(), Nout, b.data_ptr())
     torch.cuda.synchronize()
     H = (X.double() @ W.double()).to(torch.bfloat16).float().cpu().numpy()
@@ -333,6 +339,7 @@ def test_spmm_sparse_mfma_gcn_bias_and_noalloc(ctx):
     C1 = _bf16_run(p, m, n, B)
     Bd = torch.from_numpy(B).to(torch.bfloat16).cuda()
     C2 = torch.full((m.nrows, n), float("nan"), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()  # inputs made on torch's stream are complete
     p.spmm_noalloc(Bd.data_ptr(), n, n, vnm.DT_BF16, C2.data_ptr(), n)
     torch.cuda.synchronize()
     assert np.array_equal(C1.view(np.int32), C2.cpu().numpy().view(np.int32))

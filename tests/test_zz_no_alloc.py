@@ -42,6 +42,7 @@ def test_noalloc_matches_spmm_two_calls():
         Bd = torch.from_numpy(B).to(tdt).cuda()
         C_ref = torch.full((m.nrows, n), float("nan"), device="cuda")
         C = torch.full((m.nrows, n), float("nan"), device="cuda")
+        torch.cuda.synchronize()  # inputs made on torch's stream are complete
         p.spmm(Bd.data_ptr(), n, n, dt, C_ref.data_ptr(), n)  # tunes, loads every kernel
         ref64 = oracle.spmm_vnm_f64(_om(m), V, M, Bd.float().cpu().numpy())
         prepared.append((g, p, info, Bd, C_ref, C, n, dt, tol, ref64))
@@ -50,6 +51,7 @@ def test_noalloc_matches_spmm_two_calls():
     for k, (g, p, info, Bd, C_ref, C, n, dt, tol, ref64) in enumerate(prepared):
         print(f"call {k}: nrows {info.nrows} items {info.items} slots {info.slots} n {n} dtype {dt}", flush=True)
         free0 = torch.cuda.mem_get_info()[0]
+        torch.cuda.synchronize()  # inputs made on torch's stream are complete
         p.spmm_noalloc(Bd.data_ptr(), n, n, dt, C.data_ptr(), n)
         torch.cuda.synchronize()
         assert torch.cuda.mem_get_info()[0] == free0

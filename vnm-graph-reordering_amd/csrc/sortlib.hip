@@ -26,6 +26,8 @@
 // Scan: tiles of 4,096 elements; per-tile totals (k_scan_reduce), one
 // workgroup scanning the totals (k_scan_totals), then each tile scanned in
 // LDS with its offset (k_scan_tile).  Sum and max, int32 or uint8 input.
+// Inputs of at most one tile are sorted (k_radix_small) or scanned
+// (k_scan_small) by one workgroup in one launch.
 // Compaction: exclusive sum of the flags, then a scatter of flagged items.
 #include "sortlib.h"
 
@@ -210,6 +212,92 @@ __global__ __launch_bounds__(WG) void k_radix_scatter(const uint64_t *__restrict
     }
 }
 
+// A whole sort of at most TILE keys in one workgroup: every pass ranks the
+// keys as k_radix_scatter does (each wave its 1,024 keys in 16 rounds of
+// 64), places them in LDS in digit order and reads them back in the rank
+// layout for the next pass -- one launch instead of five per pass, for the
+// many small sorts of the VENOM refinement levels and the proposer.
+template <bool VALS>
+__global__ __launch_bounds__(WG) void k_radix_small(const uint64_t *__restrict__ kin, const int32_t *__restrict__ vin,
+                                                    uint64_t *__restrict__ kout, int32_t *__restrict__ vout, int n,
+                                                    int end_bit) {
+    __shared__ int32_t run[NWAVE][RADIX];
+    __shared__ int32_t start[RADIX];
+    __shared__ int32_t sh[NWAVE];
+    __shared__ uint64_t sk[TILE];
+    __shared__ int32_t sv[VALS ? TILE : 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int base = w * (64 * ROUNDS);
+    uint64_t key[ROUNDS];
+    int32_t val[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int i = base + r * 64 + lane;
+        key[r] = i < n ? kin[i] : 0ull;
+        if constexpr (VALS) val[r] = i < n ? vin[i] : 0;
+    }
+    for (int shift = 0; shift < end_bit; shift += 8) {
+        const int bits = min(8, end_bit - shift);
+        for (int i = threadIdx.x; i < NWAVE * RADIX; i += WG) (&run[0][0])[i] = 0;
+        __syncthreads();
+        int dig[ROUNDS], rnk[ROUNDS];
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            const bool ok = base + r * 64 + lane < n;
+            const int d = ok ? digit_of(key[r], shift, bits) : 0;
+            uint64_t peers = __ballot(ok);
+            for (int b = 0; b < bits; ++b) {
+                const uint64_t bal = __ballot((d >> b) & 1);
+                peers &= ((d >> b) & 1) ? bal : ~bal;
+            }
+            const int below = __popcll(peers & lanemask_lt());
+            int r0 = 0;
+            if (ok) r0 = run[w][d];
+            if (ok && below == 0) run[w][d] = r0 + __popcll(peers);
+            dig[r] = ok ? d : -1;
+            rnk[r] = r0 + below;
+        }
+        __syncthreads();
+        const int d = threadIdx.x;
+        int32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < NWAVE; ++i) {
+            const int32_t x = run[i][d];
+            run[i][d] = c;
+            c += x;
+        }
+        int32_t total;
+        start[d] = wg_scan_excl<int32_t>(c, OpSum(), 0, sh, &total);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            if (dig[r] >= 0) {
+                const int p = start[dig[r]] + run[w][dig[r]] + rnk[r];
+                sk[p] = key[r];
+                if constexpr (VALS) sv[p] = val[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            const int i = base + r * 64 + lane;
+            if (i < n) {
+                key[r] = sk[i];
+                if constexpr (VALS) val[r] = sv[i];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int i = base + r * 64 + lane;
+        if (i < n) {
+            kout[i] = key[r];
+            if constexpr (VALS) vout[i] = val[r];
+        }
+    }
+}
+
 // ------------------------------------------------------------ scan
 
 // Scan input as int32: flags count as 0 or 1.
@@ -284,6 +372,40 @@ __global__ __launch_bounds__(WG) void k_scan_tile(const In *__restrict__ in, int
     }
 }
 
+template <typename In, typename Op, bool EXCL>
+// A whole scan of at most TILE elements in one workgroup (one launch).
+__global__ __launch_bounds__(WG) void k_scan_small(const In *__restrict__ in, int32_t *__restrict__ out, int n) {
+    __shared__ int32_t s[TILE + TILE / 32];
+    __shared__ int32_t sh[NWAVE];
+    const Op op;
+    auto at = [](int i) { return i + (i >> 5); };
+    for (int j = 0; j < ROUNDS; ++j) {
+        const int i = j * WG + threadIdx.x;
+        s[at(i)] = i < n ? load_as(in, i) : Op::id;
+    }
+    __syncthreads();
+    int32_t v[ROUNDS];
+    int32_t acc = Op::id;
+#pragma unroll
+    for (int j = 0; j < ROUNDS; ++j) {
+        v[j] = s[at(threadIdx.x * ROUNDS + j)];
+        acc = op(acc, v[j]);
+    }
+    int32_t total;
+    int32_t run = wg_scan_excl<int32_t>(acc, op, Op::id, sh, &total);
+#pragma unroll
+    for (int j = 0; j < ROUNDS; ++j) {
+        const int32_t next = op(run, v[j]);
+        s[at(threadIdx.x * ROUNDS + j)] = EXCL ? run : next;
+        run = next;
+    }
+    __syncthreads();
+    for (int j = 0; j < ROUNDS; ++j) {
+        const int i = j * WG + threadIdx.x;
+        if (i < n) out[i] = s[at(i)];
+    }
+}
+
 __global__ void k_select_scatter(const int32_t *__restrict__ in, const uint8_t *__restrict__ flags,
                                  const int32_t *__restrict__ pos, int64_t n, int32_t *__restrict__ out,
                                  int32_t *__restrict__ count) {
@@ -299,6 +421,12 @@ int tiles_of(int64_t n) { return (int)((n + TILE - 1) / TILE); }
 // Scan of n elements with `totals` (>= tiles_of(n) ints) as scratch.
 template <typename In, typename Op, bool EXCL>
 int scan_with(const In *in, int32_t *out, int64_t n, int32_t *totals, hipStream_t st) {
+    if (n <= TILE) {
+        hipLaunchKernelGGL((k_scan_small<In, Op, EXCL>), dim3(1), dim3(WG), 0, st, in, out, (int)n);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return set_error(VNM_E_HIP, "scan launch failed: %s", hipGetErrorString(e));
+        return VNM_OK;
+    }
     const int tiles = tiles_of(n);
     hipLaunchKernelGGL((k_scan_reduce<In, Op>), dim3(tiles), dim3(WG), 0, st, in, n, totals);
     hipLaunchKernelGGL((k_scan_totals<Op>), dim3(1), dim3(WG), 0, st, totals, tiles);
@@ -314,6 +442,12 @@ int radix_sort(const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t 
     if (n <= 0) return VNM_OK;
     if (n > (int64_t)INT32_MAX) return set_error(VNM_E_ARG, "radix sort of %lld keys", (long long)n);
     end_bit = std::max(1, std::min(end_bit, 64));
+    if (n <= TILE) {
+        hipLaunchKernelGGL((k_radix_small<VALS>), dim3(1), dim3(WG), 0, st, kin, vin, kout, vout, (int)n, end_bit);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return set_error(VNM_E_HIP, "radix sort launch failed: %s", hipGetErrorString(e));
+        return VNM_OK;
+    }
     const int passes = (end_bit + 7) / 8;
     const int tiles = tiles_of(n);
     const int64_t nh = (int64_t)RADIX * tiles;
