@@ -214,6 +214,45 @@ def timed_spmm(ctx, packed, Bt, bdt, C, n, steps, warmup, scales=None):
     return wall * 1e3, kms / max(kn, 1)
 
 
+def timed_gcn(ctx, packed, nrows, ncols, k_in, n_out, steps, warmup):
+    """One GCN layer Y = A (X W) (X ncols x k_in, W k_in x n_out, BF16
+    N(0,1); Y FP32), the reference's GCNConv aggregation (gnn/single/
+    pyg_gcn.py:41-74): per-call wall ms, the X W GEMM's kernel ms and TFLOP/s,
+    and its relative error against an FP32 torch product."""
+    import torch
+    import vnm
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    X = torch.randn(ncols, k_in, device="cuda", generator=gen).to(torch.bfloat16)
+    W = (torch.randn(k_in, n_out, device="cuda", generator=gen) / k_in ** 0.5).to(torch.bfloat16)
+    Y = torch.empty(nrows, n_out, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()  # X, W and Y were made on torch's stream
+    for _ in range(warmup):
+        packed.gcn_layer(X.data_ptr(), k_in, k_in, W.data_ptr(), n_out, Y.data_ptr(), n_out)
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    t = time.perf_counter()
+    for _ in range(steps):
+        packed.gcn_layer(X.data_ptr(), k_in, k_in, W.data_ptr(), n_out, Y.data_ptr(), n_out)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t) / steps
+    gms, gn = ctx.timer("k_gemm_xw")
+    g_ms = gms / max(gn, 1)
+    # the GEMM's own output (BF16) is internal: check Y's rows through the
+    # product XW on torch (FP32) and the same A applied by the library
+    H = (X.float() @ W.float()).to(torch.bfloat16)
+    Y2 = torch.empty_like(Y)
+    torch.cuda.synchronize()
+    packed.spmm(H.data_ptr(), n_out, n_out, vnm.DT_BF16, Y2.data_ptr(), n_out)
+    torch.cuda.synchronize()
+    rel = float((Y - Y2).norm() / Y2.norm().clamp_min(1e-30))
+    flops = 2.0 * ncols * k_in * n_out
+    return {"ms_per_layer": round(wall * 1e3, 5), "k_gemm_xw_ms": round(g_ms, 5),
+            "k_gemm_xw_tflops": round(flops / (g_ms * 1e-3) / 1e12, 2),
+            "k_gemm_xw_hbm_gbs": round((ncols * k_in * 2 + k_in * n_out * 2 + ncols * n_out * 2)
+                                       / (g_ms * 1e-3) / 1e9, 1),
+            "rel_err_vs_torch_xw": rel, "k_in": k_in, "n_out": n_out}
+
+
 def fp8_rows(Bd, n):
     """OCP e4m3 copy of B with one FP32 scale per 128 rows (absmax / 448)."""
     import torch
@@ -281,6 +320,7 @@ def bench_precision_131k(a):
         outs[name] = C.cpu().numpy()
     head = res_dt["bf16"]
     dense_subset = sparse_mfma_subset(vnm, ctx, out_coo, V, M, Bd.to(torch.bfloat16), NC, a.steps)
+
     if a.copy_calib:
         copy_calibration(dev)
     traffic = measured_traffic("k_spmm_rmat131072_deg24_V64_M4_traffic.json", "k_spmm_bf16",
@@ -413,6 +453,21 @@ def bench_batch64(a):
     reorder_sum_ms = sum(st.ms for st in sts)
     cnt0 = sum(st.init_cnt for st in sts)
     cnt1 = sum(st.final_cnt for st in sts)
+    # the exact N:M reorder of all 64 (schedule 0, maxiter 10) in one launch
+    # chain: rounds captured as HIP graphs, grid y = the 64 graphs
+    gx = [vnm.Graph(ctx, m, M) for m in coos]
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    sx = vnm.Graph.nm_reorder_batch(gx, N, 0, 10)
+    torch.cuda.synchronize()
+    exact_batch_ms = (time.perf_counter() - t) * 1e3
+    exact_batch = {"method": "exact N:M 2:8 schedule 0, maxiter 10, 64 graphs in one captured launch chain",
+                   "gpu_ms_total": round(exact_batch_ms, 2),
+                   "init_cnt": int(sum(x[2] for x in sx)), "final_cnt": int(sum(x[3] for x in sx)),
+                   "rounds_max": int(max(x[4] for x in sx)),
+                   "retained_nnz": int(sum(g.retained_nnz(V) for g in gx))}
+    for g in gx:
+        g.close()
     packed = [g.pack(V, vnm.DT_BF16) for g in graphs]
     infos = [p.info() for p in packed]
     dev = torch.device("cuda:0")
@@ -472,7 +527,8 @@ def bench_batch64(a):
                   "separate_calls_ms_per_step": round(seq_ms, 5)},
         "reorder": {"method": "learned proposer + exact gains, 64 graphs on 8 worker streams",
                     "gpu_ms_total": round(reorder_ms, 2), "per_graph_ms_sum": round(reorder_sum_ms, 2),
-                    "init_cnt": cnt0, "final_cnt": cnt1, "retained_nnz": int(retained)},
+                    "init_cnt": cnt0, "final_cnt": cnt1, "retained_nnz": int(retained),
+                    "exact_batched": exact_batch},
         "gen_s": round(gen_s, 2),
     }
     if not a.no_cpu:
@@ -653,6 +709,7 @@ def main():
     B8 = (Bblk / bsc[:, None, None]).to(torch.float8_e4m3fn).view(-1, NCOLS).contiguous()
     ms_8, k_8 = run(p8, B8, vnm.DT_FP8, a.steps, bsc)
     C8 = C.cpu().numpy()
+    gcn = timed_gcn(ctx, pbf, coo.nrows, coo.ncols, 256, 256, 20, 3)
 
     if a.copy_calib:
         copy_calibration(dev)
@@ -689,7 +746,8 @@ def main():
                     "rooflines": reorder_roof},
         "variants": {"bf16_tflops": round(flops / (ms_bf * 1e-3) / 1e12, 4),
                      "bf16_kernel_gbs": round(bf_bytes / (k_bf * 1e-3) / 1e9, 1),
-                     "fp8_tflops": round(flops / (ms_8 * 1e-3) / 1e12, 4)},
+                     "fp8_tflops": round(flops / (ms_8 * 1e-3) / 1e12, 4),
+                     "gcn_layer_bf16": gcn},
         "gen_s": round(t_gen, 2),
     }
 

@@ -91,17 +91,21 @@ def test_batch_replays_match_single_and_oracle(ctx, bdt, n):
     batch.close()
 
 
-def test_gcn_layer_matches_oracle(ctx):
+# K 48 and 272 end on a half step of the 32-deep MFMA (272: in the third
+# 128-deep LDS stage); ldx > K reads X through a row pitch
+@pytest.mark.parametrize("K,ldx", [(96, 96), (48, 52), (272, 272), (256, 260)])
+def test_gcn_layer_matches_oracle(ctx, K, ldx):
     m = vnm.gen_chung_lu(3000, 2.1, 12.0, 77)
     g = vnm.Graph(ctx, m, M)
     p = g.pack(64, vnm.DT_BF16)
-    K, Nout = 96, 128
-    X = torch.from_numpy(vnm.gen_normal(1, (m.ncols, K))).to(torch.bfloat16).cuda()
+    Nout = 128
+    Xfull = torch.from_numpy(vnm.gen_normal(1, (m.ncols, ldx))).to(torch.bfloat16).cuda()
+    X = Xfull[:, :K]
     W = (torch.from_numpy(vnm.gen_normal(2, (K, Nout))) / np.sqrt(K)).to(torch.bfloat16).cuda()
     b = torch.linspace(-0.5, 0.5, Nout, device="cuda")
     Y = torch.full((m.nrows, Nout), float("nan"), device="cuda")
     torch.cuda.synchronize()  # inputs made on torch's stream are complete
-    p.gcn_layer(X.data_ptr(), K, K, W.data_ptr(), Nout, Y.data_ptr(), Nout, b.data_ptr())
+    p.gcn_layer(Xfull.data_ptr(), ldx, K, W.data_ptr(), Nout, Y.data_ptr(), Nout, b.data_ptr())
     torch.cuda.synchronize()
     H = (X.double() @ W.double()).to(torch.bfloat16).float().cpu().numpy()
     ref = oracle.spmm_vnm_f64(_om(m), 64, M, H) + b.double().cpu().numpy()

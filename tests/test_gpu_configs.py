@@ -217,6 +217,30 @@ def test_chunglu64_exact_sample(ctx):
         assert g.retained_nnz(64) == gd["retained_after"]
 
 
+def test_chunglu64_exact_batch_one_chain(ctx):
+    """All 64 graphs reordered exactly (N:M 2:8, schedule 0, maxiter 10) in
+    one launch chain of captured rounds (grid y = the 64 graphs): the sample's
+    stats and COO match the oracle goldens, and every graph's result is the
+    one its own single-graph run gives (checked on 8 graphs across the size
+    range)."""
+    coos = [_chunglu(i) for i in range(64)]
+    gs = [vnm.Graph(ctx, m, 8) for m in coos]
+    sts = vnm.Graph.nm_reorder_batch(gs, N, 0, 10)
+    for i in CHUNGLU_SAMPLE:
+        gd = GOLD[f"chunglu{i}_sched0"]
+        assert list(sts[i]) == gd["stats"], i
+        assert _digest(gs[i].download()) == gd["sha256"], i
+    order = np.argsort([m.nrows for m in coos])
+    for i in order[::9]:
+        g1 = vnm.Graph(ctx, coos[i], 8)
+        s1 = g1.nm_reorder(N, 0)
+        assert s1 == list(sts[i]), i
+        assert _digest(g1.download()) == _digest(gs[i].download()), i
+        g1.close()
+    for g in gs:
+        g.close()
+
+
 def test_chunglu64_learned_batch_gcn_aggregation(ctx):
     """All 64 graphs: learned reorder on worker streams (each graph's result
     its own single run's, checked on a sample), V:N:M 64:2:8 BF16 packing,

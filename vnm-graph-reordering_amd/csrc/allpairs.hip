@@ -11,10 +11,12 @@
 // double swap applies its pairtab entry, which for moves 33..35 is not the
 // move that was scored.  Because singles are never dropped, the reference can
 // swap a node that was already swapped in the round; its relabel is then not
-// a permutation (two nodes take one id) and the matrix loses positions.  This
-// engine keeps the relabel a permutation, so in compat mode it stops with
-// VNM_E_UB_PATH at the first such swap.  clean drops every move that touches
-// a swapped node and applies the move that was scored.
+// a permutation (two nodes take one id) and the matrix loses positions.
+// Such a round is applied literally, as the reference does: the COO is
+// brought to its current ids, each id the round's newid table names is
+// replaced by it, and the run continues from that COO, its CSC holding every
+// position once.  clean drops every move that touches a swapped node and
+// applies the move that was scored.
 //
 // Shape: while no node is swapped twice, each visit of a primary segment
 // swaps at least one of its 4 unswapped columns, so a round touches the
@@ -22,16 +24,21 @@
 // GPU scores every ordered pair of those <= 5 segments at once: a table pass
 // gives each row's bitmaps in the 5 segments, a histogram counts (l, r)
 // bitmap pairs per segment pair (256 bins), and each of the 52 moves' gains is
-// the histogram weighted by the per-row gain of that move.  The greedy itself
-// (the reference's priority queues, same comparators, same libstdc++ heap)
-// runs on the host; swaps are committed to perm/inv and the counts recomputed
-// on the device.
+// the histogram weighted by the per-row gain of that move.  A pair outside
+// those (a compat round that swaps nodes twice can reach further targets) is
+// scored the same way on demand; scores are those of the round's start, as
+// the reference's bitmaps change only at its end.  The greedy itself (the
+// reference's priority queues, same comparators, same libstdc++ heap) runs on
+// the host; swaps are committed to perm/inv and the counts recomputed on the
+// device.
 #include "graph.h"
 #include "hip_common.h"
 #include "tables.h"
 
 #include <algorithm>
+#include <array>
 #include <functional>
+#include <map>
 #include <queue>
 #include <vector>
 
@@ -185,6 +192,16 @@ __global__ void k_ap_gains(int K, int N, const int32_t *__restrict__ hist, int32
     score[(size_t)ab * 104 + 2 * m + 1] = sr;
 }
 
+// The literal relabel of a compat round: a[i] = newid[a[i]] where that is
+// set (reorder/reorder.h:23-35 with the round's newid, which need not be a
+// permutation).
+__global__ void k_ap_literal(int32_t *__restrict__ a, int64_t n, const int32_t *__restrict__ newid, int32_t R) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t x = a[i];
+        if (x >= 0 && x < R && newid[x] != -1) a[i] = newid[x];
+    }
+}
+
 // Commit swaps of current ids (pairs (v1, v2)) to perm / inv.
 __global__ void k_ap_commit(int n, const int32_t *__restrict__ sw, int32_t *__restrict__ perm, int32_t *__restrict__ inv) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -256,8 +273,10 @@ int nm_reorder_allpairs(vnm_graph *g, int N, int sched, int maxiter, int clean, 
         while (!l.empty()) { c += l.top().second; l.pop(); }
         return c;
     };
-    // score every ordered pair of up to AP_K segments
+    // score every ordered pair of up to AP_K segments; the gains of ordered
+    // segment pairs (a, b), 52 moves x {lgain, rgain}, kept for the round
     std::vector<int> ksegs;
+    std::map<std::pair<int, int>, std::array<int32_t, 104>> scored;
     auto score_round = [&]() -> int {
         const int K = (int)ksegs.size();
         VNM_HIP_TRY(hipMemcpyAsync(dsegs.p, ksegs.data(), K * 4, hipMemcpyHostToDevice, st));
@@ -271,18 +290,25 @@ int nm_reorder_allpairs(vnm_graph *g, int N, int sched, int maxiter, int clean, 
         VNM_LAUNCH_CHECK("k_ap_gains");
         VNM_HIP_TRY(hipMemcpyAsync(score.data(), dscore.p, (size_t)K * K * 104 * 4, hipMemcpyDeviceToHost, st));
         VNM_HIP_TRY(hipStreamSynchronize(st));
+        for (int i = 0; i < K; ++i)
+            for (int j = 0; j < K; ++j) {
+                if (i == j) continue;
+                // a segment scored against itself keeps its (0, 1) slot pair
+                auto ins = scored.try_emplace({ksegs[i], ksegs[j]});
+                if (ins.second) std::copy_n(&score[(size_t)(i * K + j) * 104], 104, ins.first->second.begin());
+            }
         return VNM_OK;
     };
-    auto kidx = [&](int s) {
-        for (size_t i = 0; i < ksegs.size(); ++i)
-            if (ksegs[i] == s) return (int)i;
-        return -1;
-    };
-    // the moves of the round's segments ia x ib in push order
-    // (get_segment_swap_score_all)
-    auto moves = [&](int ia, int ib, std::vector<ApMove> &out) {
-        const int a = ksegs[ia], b = ksegs[ib], K = (int)ksegs.size();
-        const int32_t *sc = &score[(size_t)(ia * K + ib) * 104];
+    // the moves of segments a x b in push order (get_segment_swap_score_all)
+    auto moves = [&](int a, int b, std::vector<ApMove> &out) -> int {
+        auto it = scored.find({a, b});
+        if (it == scored.end()) {
+            ksegs = {a, b};
+            int r = score_round();
+            if (r) return r;
+            it = scored.find({a, b});
+        }
+        const int32_t *sc = it->second.data();
         out.resize(52);
         for (int i = 0; i < 16; ++i) out[i] = ApMove{a * 4 + i / 4, b * 4 + i % 4, -1, -1, sc[2 * i], sc[2 * i + 1]};
         for (int i = 0; i < 36; ++i) {
@@ -302,12 +328,13 @@ int nm_reorder_allpairs(vnm_graph *g, int N, int sched, int maxiter, int clean, 
             }
             out[16 + i] = ApMove{a * 4 + x, b * 4 + y, a * 4 + u, b * 4 + w, sc[2 * (16 + i)], sc[2 * (16 + i) + 1]};
         }
+        return VNM_OK;
     };
     std::vector<int> newid(R, -1);
     std::vector<int32_t> swaps;
-    bool ub = false;
+    bool literal = false;  // this round swapped a node twice: newid is not a permutation
     auto swap_nodes = [&](int a, int b) {
-        if (newid[a] != -1 || newid[b] != -1) ub = true;  // the reference relabel stops being a permutation
+        if (newid[a] != -1 || newid[b] != -1) literal = true;
         newid[a] = b;
         newid[b] = a;
         swaps.push_back(a);
@@ -335,7 +362,7 @@ int nm_reorder_allpairs(vnm_graph *g, int N, int sched, int maxiter, int clean, 
             // bitmaps of a row being independent copies (two slots)
             ksegs = {top, minseg};
             if ((rc = score_round())) return rc;
-            moves(0, 1, mv);
+            if ((rc = moves(top, minseg, mv))) return rc;
             std::priority_queue<const ApMove *, std::vector<const ApMove *>, std::function<bool(const ApMove *, const ApMove *)>>
                 pq(ap_less);
             for (auto &m : mv) pq.push(&m);
@@ -352,7 +379,7 @@ int nm_reorder_allpairs(vnm_graph *g, int N, int sched, int maxiter, int clean, 
             while (!trav.empty() && (int)ksegs.size() < AP_K) { ksegs.push_back(trav.top().first); trav.pop(); }
             if ((rc = score_round())) return rc;
         }
-        while (lst.size() >= 2 && !ub) {
+        while (lst.size() >= 2) {
             int pseg = lst.top().first, pcnt = lst.top().second;
             lst.pop();
             SegList temp(rev ? seg_cmp_rev : seg_cmp);
@@ -364,15 +391,13 @@ int nm_reorder_allpairs(vnm_graph *g, int N, int sched, int maxiter, int clean, 
                 temp = lst;
             }
             SegList upd(rev ? seg_cmp_rev : seg_cmp);
-            while (!temp.empty() && !ub) {
+            while (!temp.empty()) {
                 bool all_vis = true;
                 for (int i = 0; i < 4; ++i) all_vis = all_vis && visited(pseg * 4 + i);
                 if (pcnt <= 0 || all_vis) break;
                 const int tseg = temp.top().first;
                 int tcnt = temp.top().second;
-                if (kidx(pseg) < 0 || kidx(tseg) < 0)
-                    return set_error(VNM_E_UB_PATH, "allpairs: a round visited more than %d segments", AP_K);
-                moves(kidx(pseg), kidx(tseg), mv);
+                if ((rc = moves(pseg, tseg, mv))) return rc;
                 std::priority_queue<const ApMove *, std::vector<const ApMove *>, std::function<bool(const ApMove *, const ApMove *)>>
                     pq(ap_less);
                 for (auto &m : mv) pq.push(&m);
@@ -396,19 +421,36 @@ int nm_reorder_allpairs(vnm_graph *g, int N, int sched, int maxiter, int clean, 
             }
             lst = upd;
         }
-        if (ub)
-            return set_error(VNM_E_UB_PATH, "allpairs (reference filter): a node was swapped twice in round %d, "
-                                            "so the reference's relabel is not a permutation", round);
         // reorder(): commit the round's swaps, recount
         const int nsw = (int)swaps.size() / 2;
-        for (int base_i = 0; base_i < nsw; base_i += 64) {
-            const int n = std::min(64, nsw - base_i);
-            VNM_HIP_TRY(hipMemcpyAsync(dsw.p, swaps.data() + 2 * base_i, (size_t)n * 8, hipMemcpyHostToDevice, st));
-            hipLaunchKernelGGL(k_ap_commit, dim3(1), dim3(64), 0, st, n, dsw.p, g->perm.p, g->inv.p);
-            VNM_HIP_TRY(hipStreamSynchronize(st));
+        if (literal) {
+            // the COO at its current ids, then newid applied as it stands
+            // (reorder/reorder.h:392-418); the run goes on from that COO
+            if ((rc = relabel(g, g->perm.p))) return rc;
+            DBuf<int32_t> dnew;
+            if ((rc = dnew.alloc(R))) return rc;
+            VNM_HIP_TRY(hipMemcpyAsync(dnew.p, newid.data(), (size_t)R * 4, hipMemcpyHostToDevice, st));
+            auto grid = [](int64_t n) { return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65535))); };
+            hipLaunchKernelGGL(k_ap_literal, grid(g->nnz), dim3(256), 0, st, g->row.p, g->nnz, dnew.p, R);
+            hipLaunchKernelGGL(k_ap_literal, grid(g->nnz), dim3(256), 0, st, g->col.p, g->nnz, dnew.p, R);
+            hipLaunchKernelGGL(k_ap_literal, grid((int64_t)g->perm_total.n), dim3(256), 0, st, g->perm_total.p,
+                               (int64_t)g->perm_total.n, dnew.p, R);
+            g->ctx->launches += 3;
+            VNM_LAUNCH_CHECK("k_ap_literal");
+            g->epoch++;
+            if ((rc = build_csc_unique(g))) return rc;  // synchronizes: dnew may go
+            literal = false;
+        } else {
+            for (int base_i = 0; base_i < nsw; base_i += 64) {
+                const int n = std::min(64, nsw - base_i);
+                VNM_HIP_TRY(hipMemcpyAsync(dsw.p, swaps.data() + 2 * base_i, (size_t)n * 8, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(k_ap_commit, dim3(1), dim3(64), 0, st, n, dsw.p, g->perm.p, g->inv.p);
+                VNM_HIP_TRY(hipStreamSynchronize(st));
+            }
+            g->ctx->launches++;
         }
-        g->ctx->launches++;
         swaps.clear();
+        scored.clear();
         std::fill(newid.begin(), newid.end(), -1);
         if ((rc = recount())) return rc;
         lst = make_list();

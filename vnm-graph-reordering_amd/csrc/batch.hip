@@ -64,8 +64,9 @@ __global__ __launch_bounds__(256) void k_spmm_batch(const SpmmArgs *__restrict__
 
 // Fix-up of the listed (matrix, row) pairs: cut rows get the sum of their
 // slots in item order (rows of a matrix without work items, the bias or zero).
-__global__ void k_fix_batch(const int4 *__restrict__ list, int nf, int n, const SpmmArgs *__restrict__ descs) {
-    const int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256) void k_fix_batch(const int4 *__restrict__ list, int nf, int n,
+                                                   const SpmmArgs *__restrict__ descs) {
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (w >= nf) return;
     const int4 e = list[w];
     const SpmmArgs &a = descs[e.x];
@@ -237,55 +238,115 @@ static int batch_create(vnm_ctx *ctx, vnm_packed *const *ps, int count, const vo
 
 // ------------------------------------------------------------ GCN layer
 
-// H[R x N] (BF16) = X[R x K] (BF16, leading dimension ldx) W[K x N] (BF16,
-// row-major); K a multiple of 16, N of 64.  A workgroup computes 64 rows x
-// 64 columns (four waves of 16 rows): W's 64-column panel is staged in LDS
-// transposed (128 k per stage), and each wave issues
-// v_mfma_f32_16x16x16_bf16 on fragments lane l = (row l%16, k 4(l/16)..+3)
-// of X and (k 4(l/16)..+3, column l%16) of W, accumulating FP32.
-constexpr int GK = 128;
+// Wt[N x K] = W[K x N]^T (BF16), so that the GEMM's LDS stage is a plain
+// 16-byte copy.  Tiles of 64 x 64 through LDS, both sides coalesced.
+__global__ __launch_bounds__(256) void k_transpose_w(int K, int N, const uint16_t *__restrict__ W,
+                                                     uint16_t *__restrict__ Wt) {
+    __shared__ uint16_t t[64][65];
+    const int k0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+        const int r = i / 64, c = i % 64;
+        if (k0 + r < K && n0 + c < N) t[r][c] = W[(size_t)(k0 + r) * N + n0 + c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+        const int r = i / 64, c = i % 64;  // r: column of W, c: k
+        if (n0 + r < N && k0 + c < K) Wt[(size_t)(n0 + r) * K + k0 + c] = t[c][r];
+    }
+}
+
+// H[R x N] (BF16) = X[R x K] (BF16, leading dimension ldx) W[K x N], with
+// W given transposed (Wt, N x K); K a multiple of 16, N of 64.  A workgroup
+// computes 128 rows x 64 columns, a wave 32 rows x 64 columns (two row
+// fragments share every W fragment), GK k at a time: Wt's 64-row panel
+// staged in LDS (rows padded by 16 bytes, so the 16 lanes reading one k
+// range hit different banks), gfx950's v_mfma_f32_16x16x32_bf16 on
+// fragments lane l = (row l%16, 8 k) of X and (the same 8 k, column l%16) of
+// W, accumulating FP32: in step s, lane group g = l/16 takes k 32s + 8g ..
+// +7, so one load covers 64 contiguous bytes of each of 16 rows, and a
+// stage's loads are all issued before the first MFMA.  Groups past the
+// stage's k count (a K that is an odd multiple of 16) hold zeros in both
+// operands.  The N/64 column panels of
+// a row tile are consecutive workgroups of one XCD, so the X tile is fetched
+// from HBM once and hit in that XCD's L2 by the other panels.
+constexpr int GK = 256;
+constexpr int GKP = GK + 8;
+constexpr int GS = GK / 32;  // MFMA steps of a full stage
 
 __global__ __launch_bounds__(256) void k_gemm_xw(int R, int K, int N, const uint16_t *__restrict__ X, int ldx,
-                                                 const uint16_t *__restrict__ W, uint16_t *__restrict__ H) {
+                                                 const uint16_t *__restrict__ Wt, uint16_t *__restrict__ H) {
     typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+    typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
     typedef float f32x4_t __attribute__((ext_vector_type(4)));
-    __shared__ uint16_t w[64 * (GK + 4)];
+    __shared__ __align__(16) uint16_t w[64 * GKP];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int n0 = blockIdx.y * 64;
-    const int r0 = (blockIdx.x * 4 + wv) * 16;
-    const int m = lane & 15, kq = (lane >> 4) * 4;
-    const int row = r0 + m;
-    f32x4_t acc[4];
+    // workgroups are dealt round-robin over the 8 XCDs: b -> XCD b % 8
+    const int P = N / 64;
+    const int xcd = blockIdx.x % 8, jx = blockIdx.x / 8;
+    const int panel = jx % P, tile = (jx / P) * 8 + xcd;
+    const int n0 = panel * 64;
+    const int r0 = (tile * 4 + wv) * 32;
+    const int m = lane & 15, g = lane >> 4;
+    f32x4_t acc[2][4];
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) acc[nt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[h][nt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
     for (int kc = 0; kc < K; kc += GK) {
-        const int kn = min(GK, K - kc);
+        const int kn = min(GK, K - kc);  // a multiple of 16
+        const int S = (kn + 31) / 32;    // steps of 32 k
+        // this wave's X spans first: in flight while the panel is staged
+        bf16x8_t a[2][GS];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int row = r0 + h * 16 + m;
+#pragma unroll
+            for (int st = 0; st < GS; ++st) {
+                bf16x4_t lo = bf16x4_t{0, 0, 0, 0}, hi = bf16x4_t{0, 0, 0, 0};
+                const int k0 = 32 * st + 8 * g;
+                if (st < S && k0 < kn && row < R) {
+                    const uint16_t *xp = X + (size_t)row * ldx + kc + k0;
+                    lo = *(const bf16x4_t *)xp;  // ldx % 4 == 0: 8-byte aligned
+                    hi = *(const bf16x4_t *)(xp + 4);
+                }
+                a[h][st] = __builtin_bit_cast(bf16x8_t, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+            }
+        }
         __syncthreads();
-        for (int i = threadIdx.x; i < 64 * kn; i += 256) {
-            const int k = i / 64, c = i % 64;  // coalesced over c
-            w[c * (GK + 4) + k] = W[(size_t)(kc + k) * N + n0 + c];
+        // 64 rows of kn k each, 16 bytes (8 k) per copy
+        const int per_row = kn / 8;
+        for (int i = threadIdx.x; i < 64 * per_row; i += 256) {
+            const int c = i / per_row, k8 = (i % per_row) * 8;
+            *(uint4 *)&w[c * GKP + k8] = *(const uint4 *)(Wt + (size_t)(n0 + c) * K + kc + k8);
         }
         __syncthreads();
         if (r0 < R) {
-            for (int ks = 0; ks < kn; ks += 16) {
-                bf16x4_t a = bf16x4_t{0, 0, 0, 0};
-                if (row < R) a = *(const bf16x4_t *)(X + (size_t)row * ldx + kc + ks + kq);
+#pragma unroll
+            for (int st = 0; st < GS; ++st) {
+                if (st >= S) break;
+                const int k0 = 32 * st + 8 * g;
+                const bool live = k0 < kn;
 #pragma unroll
                 for (int nt = 0; nt < 4; ++nt) {
-                    const bf16x4_t bb = *(const bf16x4_t *)&w[(nt * 16 + m) * (GK + 4) + ks + kq];
-                    acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[nt], 0, 0, 0);
+                    bf16x8_t bb = bf16x8_t{};
+                    if (live) bb = *(const bf16x8_t *)&w[(nt * 16 + m) * GKP + k0];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+                        acc[h][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h][st], bb, acc[h][nt], 0, 0, 0);
                 }
             }
         }
     }
     if (r0 >= R) return;
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt)
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int rr = r0 + kq + i;
-            if (rr < R) H[(size_t)rr * N + n0 + nt * 16 + m] = f32_to_bf16(acc[nt][i]);
-        }
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int rr = r0 + h * 16 + g * 4 + i;
+                if (rr < R) H[(size_t)rr * N + n0 + nt * 16 + m] = f32_to_bf16(acc[h][nt][i]);
+            }
 }
 
 static int gcn_layer(vnm_packed *p, const uint16_t *X, int ldx, int K, const uint16_t *W, int N,
@@ -298,12 +359,19 @@ static int gcn_layer(vnm_packed *p, const uint16_t *X, int ldx, int K, const uin
     const int R = p->ncols;  // X W has one row per column of A
     hipStream_t st = p->ctx->stream;
     int rc;
-    if (p->gcn_h.n < (size_t)R * N) {
+    if (p->gcn_h.n < (size_t)R * N + (size_t)K * N) {  // X W, then W transposed
         p->gcn_h.release();
-        if ((rc = p->gcn_h.alloc((size_t)R * N))) return rc;
+        if ((rc = p->gcn_h.alloc((size_t)R * N + (size_t)K * N))) return rc;
     }
+    uint16_t *Wt = p->gcn_h.p + (size_t)R * N;  // 16-byte aligned: R * N is a multiple of 64
+    hipLaunchKernelGGL(k_transpose_w, dim3((K + 63) / 64, N / 64), dim3(256), 0, st, K, N, W, Wt);
+    p->ctx->launches++;
+    VNM_LAUNCH_CHECK("k_transpose_w");
     KernelTimer kt(p->ctx, "k_gemm_xw");
-    hipLaunchKernelGGL(k_gemm_xw, dim3((R + 63) / 64, N / 64), dim3(256), 0, st, R, K, N, X, ldx, W, p->gcn_h.p);
+    // row tiles of 128, rounded up to whole rounds of the 8 XCDs; every
+    // (tile, panel) pair one workgroup
+    const int tiles = (R + 127) / 128, tiles8 = (tiles + 7) / 8 * 8;
+    hipLaunchKernelGGL(k_gemm_xw, dim3(tiles8 * (N / 64)), dim3(256), 0, st, R, K, N, X, ldx, Wt, p->gcn_h.p);
     kt.stop();
     p->ctx->launches++;
     VNM_LAUNCH_CHECK("k_gemm_xw");

@@ -104,6 +104,7 @@ namespace vnm {
 
 int graph_scratch(vnm_graph *g);
 int build_csc(vnm_graph *g);
+int build_csc_unique(vnm_graph *g);
 int locate_all(vnm_graph *g, int N, int32_t *d_cnt, int32_t *d_nzr);
 int relabel(vnm_graph *g, const int32_t *d_perm);
 int max_ids(vnm_graph *g, const int32_t *row, const int32_t *col, int32_t *max_row, int32_t *max_col);

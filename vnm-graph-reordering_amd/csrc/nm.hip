@@ -30,6 +30,7 @@
 //                   section below)
 //   k_relabel     : the Reorder kernel (reorder/reorder.h:23-35), applied once
 #include "graph.h"
+#include "sortlib.h"
 #include "hip_common.h"
 #include "tables.h"
 
@@ -1356,6 +1357,70 @@ int build_csc(vnm_graph *g) {
     hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(R, 256)), dim3(256), 0, st, g->inv.p, (int64_t)R);
     g->ctx->launches += 3;
     VNM_LAUNCH_CHECK("k_col_scatter");
+    return VNM_OK;
+}
+
+// Keys (column << 32 | row) of the entries build_csc takes; the others get
+// the largest key and sort last.
+__global__ void k_csc_keys(const int32_t *__restrict__ row, const int32_t *__restrict__ col,
+                           const float *__restrict__ val, int64_t nnz, int32_t R, uint64_t *__restrict__ keys) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = col[i], r = row[i];
+        keys[i] = val[i] != 0.0f && c < R && r < R ? ((uint64_t)(uint32_t)c << 32) | (uint32_t)r : ~0ull;
+    }
+}
+
+// First occurrence of each sorted key: its row, a flag, and a count for its
+// column.
+__global__ void k_csc_unique(const uint64_t *__restrict__ keys, int64_t n, int32_t *__restrict__ rows,
+                             uint8_t *__restrict__ flags, int32_t *__restrict__ cnt1) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        const bool f = k != ~0ull && (i == 0 || keys[i - 1] != k);
+        rows[i] = (int32_t)(uint32_t)k;
+        flags[i] = f;
+        if (f) atomicAdd(&cnt1[(int)(k >> 32) + 1], 1);
+    }
+}
+
+// build_csc for a COO that may hold one position more than once (after the
+// allpairs schedules' literal relabel, which is not a permutation): each
+// position enters the CSC once, as a bitmap holds it once.  Rows within a
+// column come out in ascending order.
+int build_csc_unique(vnm_graph *g) {
+    int rc = graph_scratch(g);
+    if (rc) return rc;
+    hipStream_t st = g->ctx->stream;
+    const int R = g->nrows;
+    const int64_t n = g->nnz;
+    DBuf<uint64_t> k0, k1;
+    DBuf<int32_t> rows, cnt;
+    DBuf<uint8_t> flags;
+    const size_t n1 = (size_t)std::max<int64_t>(n, 1);
+    if ((rc = k0.alloc(n1)) || (rc = k1.alloc(n1)) || (rc = rows.alloc(n1)) || (rc = flags.alloc(n1)) ||
+        (rc = cnt.alloc(1)))
+        return rc;
+    VNM_HIP_TRY(hipMemsetAsync(g->cptr.p, 0, (R + 1) * 4, st));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_csc_keys, dim3(grid_for(n, 256)), dim3(256), 0, st, g->row.p, g->col.p, g->val.p, n, R,
+                           k0.p);
+        g->ctx->launches++;
+        VNM_LAUNCH_CHECK("k_csc_keys");
+        if ((rc = sort_keys_u64(k0.p, k1.p, n, 64, st))) return rc;
+        hipLaunchKernelGGL(k_csc_unique, dim3(grid_for(n, 256)), dim3(256), 0, st, k1.p, n, rows.p, flags.p,
+                           g->cptr.p);
+        g->ctx->launches++;
+        VNM_LAUNCH_CHECK("k_csc_unique");
+        if ((rc = select_flagged_i32(rows.p, flags.p, g->crow.p, cnt.p, n, st))) return rc;
+    }
+    rc = scan_inclusive(g, g->cptr.p, R + 1, g->ws.p);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(R, 256)), dim3(256), 0, st, g->perm.p, (int64_t)R);
+    hipLaunchKernelGGL(k_iota_i32, dim3(grid_for(R, 256)), dim3(256), 0, st, g->inv.p, (int64_t)R);
+    g->ctx->launches += 2;
+    VNM_LAUNCH_CHECK("k_iota_i32");
+    // the scratch is freed on return: finish with it first
+    VNM_HIP_TRY(hipStreamSynchronize(st));
     return VNM_OK;
 }
 

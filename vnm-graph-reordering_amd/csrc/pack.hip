@@ -399,10 +399,12 @@ __global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int T, int items, int 
 // Fix-up pass: rows cut into chunks get the sum of their partial-sum slots
 // in item order (one wave each); a matrix without work items, the bias or
 // zero in every row.
-__global__ void k_spmm_fix(const int4 *__restrict__ fix, int nfix, const float *__restrict__ part, int pn, int n,
-                           float *__restrict__ C, int ldc, const float *__restrict__ bias,
-                           const uint8_t *__restrict__ dvb, int V) {
-    const int w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256) void k_spmm_fix(const int4 *__restrict__ fix, int nfix, const float *__restrict__ part,
+                                                  int pn, int n, float *__restrict__ C, int ldc,
+                                                  const float *__restrict__ bias, const uint8_t *__restrict__ dvb,
+                                                  int V) {
+    // wave-uniform, so the list entry is a scalar load
+    const int w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (w >= nfix) return;
     const int4 f = fix[w];
     fix_row(C + (size_t)f.x * ldc, part, pn, f.y, f.z, n, bias, dvb != nullptr && dvb[f.x / V] != 0);
