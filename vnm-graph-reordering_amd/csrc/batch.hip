@@ -257,24 +257,44 @@ __global__ __launch_bounds__(256) void k_transpose_w(int K, int N, const uint16_
 
 // H[R x N] (BF16) = X[R x K] (BF16, leading dimension ldx) W[K x N], with
 // W given transposed (Wt, N x K); K a multiple of 16, N of 64.  A workgroup
-// computes 128 rows x 64 columns, a wave 32 rows x 64 columns (two row
-// fragments share every W fragment), GK k at a time: Wt's 64-row panel
-// staged in LDS (rows padded by 16 bytes, so the 16 lanes reading one k
-// range hit different banks), gfx950's v_mfma_f32_16x16x32_bf16 on
-// fragments lane l = (row l%16, 8 k) of X and (the same 8 k, column l%16) of
-// W, accumulating FP32: in step s, lane group g = l/16 takes k 32s + 8g ..
-// +7, so one load covers 64 contiguous bytes of each of 16 rows, and a
-// stage's loads are all issued before the first MFMA.  Groups past the
-// stage's k count (a K that is an odd multiple of 16) hold zeros in both
-// operands.  The N/64 column panels of
+// takes RT row tiles of 128 rows of one XCD for one 64-column panel; a wave
+// computes 32 rows x 64 columns of each (two row fragments share every W
+// fragment), GK k at a time.  Wt's 64-row panel is staged in LDS (rows
+// padded by 16 bytes, so the 16 lanes reading one k range hit different
+// banks), once per workgroup when K <= GK.  gfx950's v_mfma_f32_16x16x32_bf16
+// takes the W fragment as its first operand (lane l: column n = l%16 of the
+// panel, k 8(l/16)..+7) and the X fragment as its second (row l%16, the same
+// k), so it produces H transposed: lane l ends with 4 consecutive columns of
+// one row, stored as 8 bytes.  In step s lane group g = l/16 takes k
+// 32s + 8g .. +7, so one load covers 64 contiguous bytes of each of 16 rows;
+// a tile's loads are all issued before its first MFMA, and the X loads of
+// the first tile are in flight while the panel is staged (the barriers wait
+// for LDS only).  Groups past the stage's k count (a K that is an odd
+// multiple of 16) hold zeros in both operands.  The N/64 column panels of
 // a row tile are consecutive workgroups of one XCD, so the X tile is fetched
 // from HBM once and hit in that XCD's L2 by the other panels.
 constexpr int GK = 256;
 constexpr int GKP = GK + 8;
 constexpr int GS = GK / 32;  // MFMA steps of a full stage
+constexpr int GCP = 64 * (GK / 8) / 256;  // 16-byte panel copies per thread and stage
+
+// A barrier for LDS data only: pending global loads stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// f32_to_bf16 without branches (round to nearest even, NaN kept quiet).
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+    const uint32_t u = __float_as_uint(f);
+    const uint32_t r = (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+    return (u & 0x7FFFFFFFu) > 0x7F800000u ? ((u >> 16) | 0x40u) : r;
+}
 
 __global__ __launch_bounds__(256) void k_gemm_xw(int R, int K, int N, const uint16_t *__restrict__ X, int ldx,
-                                                 const uint16_t *__restrict__ Wt, uint16_t *__restrict__ H) {
+                                                 const uint16_t *__restrict__ Wt, uint16_t *__restrict__ H,
+                                                 int RT) {
     typedef short bf16x4_t __attribute__((ext_vector_type(4)));
     typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
     typedef float f32x4_t __attribute__((ext_vector_type(4)));
@@ -283,70 +303,98 @@ __global__ __launch_bounds__(256) void k_gemm_xw(int R, int K, int N, const uint
     // workgroups are dealt round-robin over the 8 XCDs: b -> XCD b % 8
     const int P = N / 64;
     const int xcd = blockIdx.x % 8, jx = blockIdx.x / 8;
-    const int panel = jx % P, tile = (jx / P) * 8 + xcd;
+    const int panel = jx % P, tgroup = jx / P;
     const int n0 = panel * 64;
-    const int r0 = (tile * 4 + wv) * 32;
     const int m = lane & 15, g = lane >> 4;
-    f32x4_t acc[2][4];
+    const bool once = K <= GK;  // the whole panel fits one stage: staged once
+    if (once) {
+        if (K == GK) {  // every copy's load issued first
+            uint4 cp[GCP];
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+            for (int j = 0; j < GCP; ++j) {
+                const int i = threadIdx.x + 256 * j;
+                cp[j] = *(const uint4 *)(Wt + (size_t)(n0 + i / (GK / 8)) * K + (i % (GK / 8)) * 8);
+            }
 #pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[h][nt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
-    for (int kc = 0; kc < K; kc += GK) {
-        const int kn = min(GK, K - kc);  // a multiple of 16
-        const int S = (kn + 31) / 32;    // steps of 32 k
-        // this wave's X spans first: in flight while the panel is staged
-        bf16x8_t a[2][GS];
+            for (int j = 0; j < GCP; ++j) {
+                const int i = threadIdx.x + 256 * j;
+                *(uint4 *)&w[(i / (GK / 8)) * GKP + (i % (GK / 8)) * 8] = cp[j];
+            }
+        } else {
+            const int per_row = K / 8;
+            for (int i = threadIdx.x; i < 64 * per_row; i += 256)
+                *(uint4 *)&w[(i / per_row) * GKP + (i % per_row) * 8] =
+                    *(const uint4 *)(Wt + (size_t)(n0 + i / per_row) * K + (i % per_row) * 8);
+        }
+        lds_barrier();
+    }
+    for (int t = 0; t < RT; ++t) {
+        const int tile = (tgroup * RT + t) * 8 + xcd;
+        const int r0 = (tile * 4 + wv) * 32;
+        if (once && (tile * 4) * 32 >= R) break;  // workgroup-uniform: no later tile has rows
+        f32x4_t acc[2][4];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int row = r0 + h * 16 + m;
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int st = 0; st < GS; ++st) {
-                bf16x4_t lo = bf16x4_t{0, 0, 0, 0}, hi = bf16x4_t{0, 0, 0, 0};
-                const int k0 = 32 * st + 8 * g;
-                if (st < S && k0 < kn && row < R) {
-                    const uint16_t *xp = X + (size_t)row * ldx + kc + k0;
-                    lo = *(const bf16x4_t *)xp;  // ldx % 4 == 0: 8-byte aligned
-                    hi = *(const bf16x4_t *)(xp + 4);
+            for (int nt = 0; nt < 4; ++nt) acc[h][nt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+        for (int kc = 0; kc < K; kc += GK) {
+            const int kn = min(GK, K - kc);  // a multiple of 16
+            const int S = (kn + 31) / 32;    // steps of 32 k
+            bf16x8_t a[2][GS];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int row = r0 + h * 16 + m;
+#pragma unroll
+                for (int st = 0; st < GS; ++st) {
+                    bf16x4_t lo = bf16x4_t{0, 0, 0, 0}, hi = bf16x4_t{0, 0, 0, 0};
+                    const int k0 = 32 * st + 8 * g;
+                    if (st < S && k0 < kn && row < R) {
+                        const uint16_t *xp = X + (size_t)row * ldx + kc + k0;
+                        lo = *(const bf16x4_t *)xp;  // ldx % 4 == 0: 8-byte aligned
+                        hi = *(const bf16x4_t *)(xp + 4);
+                    }
+                    a[h][st] = __builtin_bit_cast(bf16x8_t, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
                 }
-                a[h][st] = __builtin_bit_cast(bf16x8_t, __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+            }
+            if (!once) {  // this chunk of the panel, after its earlier readers
+                const int per_row = kn / 8;
+                lds_barrier();
+                for (int i = threadIdx.x; i < 64 * per_row; i += 256)
+                    *(uint4 *)&w[(i / per_row) * GKP + (i % per_row) * 8] =
+                        *(const uint4 *)(Wt + (size_t)(n0 + i / per_row) * K + kc + (i % per_row) * 8);
+                lds_barrier();
+            }
+            if (r0 < R) {
+#pragma unroll
+                for (int st = 0; st < GS; ++st) {
+                    if (st >= S) break;
+                    const int k0 = 32 * st + 8 * g;
+                    const bool live = k0 < kn;
+#pragma unroll
+                    for (int nt = 0; nt < 4; ++nt) {
+                        bf16x8_t bb = bf16x8_t{};
+                        if (live) bb = *(const bf16x8_t *)&w[(nt * 16 + m) * GKP + k0];
+#pragma unroll
+                        for (int h = 0; h < 2; ++h)
+                            acc[h][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bb, a[h][st], acc[h][nt], 0, 0, 0);
+                    }
+                }
             }
         }
-        __syncthreads();
-        // 64 rows of kn k each, 16 bytes (8 k) per copy
-        const int per_row = kn / 8;
-        for (int i = threadIdx.x; i < 64 * per_row; i += 256) {
-            const int c = i / per_row, k8 = (i % per_row) * 8;
-            *(uint4 *)&w[c * GKP + k8] = *(const uint4 *)(Wt + (size_t)(n0 + c) * K + kc + k8);
-        }
-        __syncthreads();
-        if (r0 < R) {
+        // lane (m, g): row r0 + 16h + m, columns n0 + 16nt + 4g .. +3
 #pragma unroll
-            for (int st = 0; st < GS; ++st) {
-                if (st >= S) break;
-                const int k0 = 32 * st + 8 * g;
-                const bool live = k0 < kn;
+        for (int h = 0; h < 2; ++h) {
+            const int rr = r0 + h * 16 + m;
+            if (rr < R) {
 #pragma unroll
                 for (int nt = 0; nt < 4; ++nt) {
-                    bf16x8_t bb = bf16x8_t{};
-                    if (live) bb = *(const bf16x8_t *)&w[(nt * 16 + m) * GKP + k0];
-#pragma unroll
-                    for (int h = 0; h < 2; ++h)
-                        acc[h][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h][st], bb, acc[h][nt], 0, 0, 0);
+                    const uint2 v = make_uint2(bf16_bits(acc[h][nt][0]) | (bf16_bits(acc[h][nt][1]) << 16),
+                                               bf16_bits(acc[h][nt][2]) | (bf16_bits(acc[h][nt][3]) << 16));
+                    *(uint2 *)(H + (size_t)rr * N + n0 + nt * 16 + g * 4) = v;
                 }
             }
         }
     }
-    if (r0 >= R) return;
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int rr = r0 + h * 16 + g * 4 + i;
-                if (rr < R) H[(size_t)rr * N + n0 + nt * 16 + m] = f32_to_bf16(acc[h][nt][i]);
-            }
 }
 
 static int gcn_layer(vnm_packed *p, const uint16_t *X, int ldx, int K, const uint16_t *W, int N,
@@ -368,10 +416,16 @@ static int gcn_layer(vnm_packed *p, const uint16_t *X, int ldx, int K, const uin
     p->ctx->launches++;
     VNM_LAUNCH_CHECK("k_transpose_w");
     KernelTimer kt(p->ctx, "k_gemm_xw");
-    // row tiles of 128, rounded up to whole rounds of the 8 XCDs; every
-    // (tile, panel) pair one workgroup
-    const int tiles = (R + 127) / 128, tiles8 = (tiles + 7) / 8 * 8;
-    hipLaunchKernelGGL(k_gemm_xw, dim3(tiles8 * (N / 64)), dim3(256), 0, st, R, K, N, X, ldx, Wt, p->gcn_h.p);
+    // row tiles of 128 per XCD, RT of them per workgroup and column panel:
+    // about 4 workgroups per CU, so each stages its W panel once for RT tiles
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p->ctx->device) != hipSuccess || ncu < 8)
+        ncu = 8;
+    const int P = N / 64;
+    const int tiles_x = ((R + 127) / 128 + 7) / 8;  // row tiles per XCD
+    const int RT = K <= GK ? std::max(1, std::min(tiles_x, tiles_x * P * 8 / (4 * ncu))) : 1;
+    const int groups = (tiles_x + RT - 1) / RT;
+    hipLaunchKernelGGL(k_gemm_xw, dim3(8 * P * groups), dim3(256), 0, st, R, K, N, X, ldx, Wt, p->gcn_h.p, RT);
     kt.stop();
     p->ctx->launches++;
     VNM_LAUNCH_CHECK("k_gemm_xw");

@@ -353,9 +353,7 @@ constexpr int KMAX = 9;          // primary + up to M targets (M <= 8)
 constexpr int MMAX = 8;
 constexpr int NPAIR_T = KMAX * (KMAX - 1) / 2 + 1;  // unordered pairs + one self pair
 constexpr int NQ = (KMAX - 1) * (KMAX - 1);         // ordered pairs the greedy can visit
-constexpr int POPP = 16;                            // pops precomputed per ordered pair
-constexpr int DEEP_AFTER = 8;                       // heap pops in a round's greedy that make the
-                                                    // next round precompute whole pop orders
+constexpr int POPP = 16;                            // fewest pops precomputed per ordered pair
 static_assert(NQ <= 64, "one heap thread per ordered pair, all in wave 0");
 constexpr int NACC = KMAX * MMAX + NPAIR_T * 2 * MMAX * MMAX;  // E and both, flattened
 
@@ -375,7 +373,7 @@ struct GreedyState {
     int32_t *trace;  // optional: 9 ints per swap
     // run state
     int round, L, totcnt, maxiter, init_size, init_cnt, status, done, pending;
-    int deep;  // the last round's greedy popped heaps often: precompute whole pop orders
+    int npre;  // pops to precompute per ordered pair this round (from the last round's need)
     int K, self_pair, size1;
     int sel[KMAX], selcnt[KMAX];
     int64_t trace_len, work;
@@ -414,6 +412,7 @@ struct GreedyShared {
     int qcur[NQ];
     int heapn[NQ];
     int npre, heap_pops;  // pops precomputed this round; pops the greedy took from heaps
+    int maxk;             // precomputed pops the greedy used, most over the ordered pairs
     int swv1[64], swv2[64], swlg[64], swrg[64], swp[64], swt[64], swpc[64], swtc[64];
     int nsw;
     uint64_t wtop[NW][KMAX + 1];
@@ -841,6 +840,7 @@ __device__ __forceinline__ int first_free(GreedyShared &S, int q, int a, int b, 
         if (m) {
             const int i = __builtin_ctzll(m);
             S.qcur[q] = k0 + i;
+            if (lane == 0) S.maxk = max(S.maxk, k0 + i + 1);  // precomputed pops this round used
             return __builtin_amdgcn_readlane(e, i);
         }
         S.qcur[q] = npre;
@@ -868,7 +868,8 @@ __device__ __forceinline__ bool list_before(int sa, int ca, int sb, int cb, bool
 // wave in lockstep: every lane holds the same values and writes the same
 // LDS words, and first_free spreads its search over the lanes.  Returns the
 // number of swaps.
-__device__ __forceinline__ int greedy_apply(GreedyShared &S, int M, bool rev, bool drain) {
+template <int M>  // 4 or 8: the per-element divisions by M become shifts
+__device__ __forceinline__ int greedy_apply(GreedyShared &S, bool rev, bool drain) {
     const int K = S.K;
     int nsw = 0;
     Visited vis;
@@ -1178,10 +1179,11 @@ __global__ __launch_bounds__(GT) void k_gr_greedy(NMArgs a0, const NMArgs *__res
         if (tid == 0) st->clk[11] += __builtin_amdgcn_s_memrealtime() - tD;
         // one thread per list, all in wave 0: the heap steps are bound by
         // instruction issue, so 64 lists share every instruction
-        const int npre = st->deep ? MM : min(POPP, MM);
+        const int npre = min(st->npre, MM);
         if (tid == 0) {
             S.npre = npre;
             S.heap_pops = 0;
+            S.maxk = 0;
         }
         if (tid < nq) {
             HeapRef H{&S.heap[0][tid], 0, NQ};
@@ -1196,10 +1198,15 @@ __global__ __launch_bounds__(GT) void k_gr_greedy(NMArgs a0, const NMArgs *__res
     const uint64_t tE = __builtin_amdgcn_s_memrealtime();
     // ---- phase E: the sequential greedy (wave 0 in lockstep)
     if (w == 0) {
-        const int nsw = greedy_apply(S, M, rev, drain);
+        const int nsw = M == 8 ? greedy_apply<8>(S, rev, drain) : greedy_apply<4>(S, rev, drain);
         if (lane == 0) {
             S.nsw = nsw;
-            st->deep = S.heap_pops > DEEP_AFTER;
+            st->clk[12] += nsw;           // swaps
+            st->clk[13] += S.heap_pops;   // pops taken from heaps by the greedy
+            st->clk[14] += S.npre;        // pops precomputed per ordered pair
+            // the next round precomputes what this one used, plus 8: a
+            // round that had to pop heaps precomputes whole pop orders next
+            st->npre = S.heap_pops > 0 ? MM : max(POPP, min(MM, (S.maxk + 15) / 8 * 8));
             const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
             st->clk[6] += tE - tD;
             st->clk[7] += t1 - tE;
@@ -1282,7 +1289,7 @@ __global__ __launch_bounds__(GT) void k_gr_init(NMArgs A, int R, int M, int N, i
         st->round = 1; st->L = sL; st->totcnt = sC;
         st->init_size = sL; st->init_cnt = sC;
         st->maxiter = maxiter > sL ? maxiter : sL;
-        st->status = 0; st->done = 0; st->pending = 0; st->deep = 0;
+        st->status = 0; st->done = 0; st->pending = 0; st->npre = POPP;
         st->K = 0; st->self_pair = 0; st->size1 = 0;
         st->trace_len = 0; st->work = 0;
         st->S.K = 0; st->S.total = 0; st->S.cb[0] = 0;
@@ -1585,6 +1592,8 @@ static void debug_print(int i, const GreedyState &h) {
     fprintf(stderr, "[vnm]   select: commit %llu, segments %llu, columns %llu; heaps: ranks %llu\n",
             (unsigned long long)(h.clk[8] / 100), (unsigned long long)(h.clk[9] / 100),
             (unsigned long long)(h.clk[10] / 100), (unsigned long long)(h.clk[11] / 100));
+    fprintf(stderr, "[vnm]   greedy: swaps %llu, heap pops %llu, precomputed pops per pair (sum) %llu\n",
+            (unsigned long long)h.clk[12], (unsigned long long)h.clk[13], (unsigned long long)h.clk[14]);
 }
 
 // The greedy of G graphs (schedules 0-3 or >= 8 each) as one chain of round
