@@ -33,7 +33,9 @@ def _keys(rng, n, kind):
         rng.integers(0, 1000, n).astype(np.uint64)  # (row, col) keys as the packer builds them
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 65, 4095, 4096, 4097, 70001, 1_000_003])
+# one tile (4,096 keys) and below: one launch in LDS; 2-4 tiles: one
+# workgroup over global buffers; more: three launches per pass
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 4095, 4096, 4097, 8192, 12289, 16384, 16385, 70001, 1_000_003])
 def test_sort_pairs_matches_stable_argsort(ctx, n):
     rng = np.random.default_rng(n)
     for kind in ("wide", "few", "rowcol"):

@@ -743,7 +743,7 @@ __global__ void k_lp_pick(int nseg, int M, int mode, uint64_t seed, const int32_
                           const int32_t *__restrict__ segcls, const float *__restrict__ Y,
                           uint64_t *__restrict__ dkey, uint64_t *__restrict__ akey,
                           int8_t *__restrict__ dbest, int8_t *__restrict__ abest,
-                          int32_t *__restrict__ ndonor) {
+                          int32_t *__restrict__ ndonor, int sbits) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nseg) return;
     const int4 S = *(const int4 *)&segcls[s * 4];
@@ -768,8 +768,10 @@ __global__ void k_lp_pick(int nseg, int M, int mode, uint64_t seed, const int32_
     }
     dbest[s] = (int8_t)bdi;
     abest[s] = (int8_t)bai;
-    dkey[s] = bdi >= 0 ? ((uint64_t)(~ord_f32(bd)) << 32) | (uint32_t)s : ~0ull;
-    akey[s] = ((uint64_t)(~ord_f32(ba)) << 32) | (uint32_t)s;
+    // (score descending, segment) in 32 + sbits bits, sbits the bits of the
+    // largest segment index: the keys' order, sorted on those bits only
+    dkey[s] = bdi >= 0 ? ((uint64_t)(~ord_f32(bd)) << sbits) | (uint32_t)s : ~0ull;
+    akey[s] = ((uint64_t)(~ord_f32(ba)) << sbits) | (uint32_t)s;
     // donors counted once per wave (lanes past nseg have returned)
     const uint64_t dm = __ballot(bdi >= 0);
     const uint64_t live = __ballot(1);
@@ -787,17 +789,17 @@ __global__ void k_lp_match(int nd, int W, int nseg, int stride, int M, int mode,
                            const uint64_t *__restrict__ sdkey, const uint64_t *__restrict__ sakey,
                            const int8_t *__restrict__ dbest, const int8_t *__restrict__ abest,
                            const int32_t *__restrict__ inv, const float *__restrict__ Y,
-                           Cand *__restrict__ cand) {
+                           Cand *__restrict__ cand, uint64_t smask) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nd) return;
-    const int p = (int)(uint32_t)sdkey[i];
+    const int p = (int)(sdkey[i] & smask);
     const int bu = dbest[p];
     const int u = inv[p * M + bu];
     float best = 0.f;
     int bw = -1;
     for (int w = 0; w < W; ++w) {
         const int j = (int)(((int64_t)i * stride + w) % nseg);
-        const int q = (int)(uint32_t)sakey[j];
+        const int q = (int)(sakey[j] & smask);
         const int bv = abest[q];
         Cand c{p, q == p ? -1 : q, bu, bv};
         if (all) {
@@ -823,7 +825,7 @@ __global__ void k_lp_match(int nd, int W, int nseg, int stride, int M, int mode,
             cand[i] = Cand{p, -1, bu, 0};
         } else {
             const int j = (int)(((int64_t)i * stride + bw) % nseg);
-            const int q = (int)(uint32_t)sakey[j];
+            const int q = (int)(sakey[j] & smask);
             cand[i] = Cand{p, q, bu, abest[q]};
         }
     }
@@ -1033,14 +1035,17 @@ __global__ __launch_bounds__(256) void k_lp_pair_gain(int R, int M, int N, const
 
 // ------------------------------------------------------------ selection
 
+// (gain descending, candidate) in gbits + ibits bits: ibits the bits of the
+// largest candidate index, gmax = 2^gbits - 1 at least the largest gain (a
+// gain counts rows of the pair's two segments, each row at most once per
+// segment, so it is at most twice the row count)
 __global__ void k_lp_gain_keys(int nc, const Cand *__restrict__ cand, const int4 *__restrict__ best,
-                               uint64_t *__restrict__ keys) {
+                               uint64_t *__restrict__ keys, int ibits, uint32_t gmax) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nc) return;
     const int4 b = best[i];
     const int gain = b.z + b.w;
-    keys[i] = (cand[i].q >= 0 && gain > 0) ? ((uint64_t)(uint32_t)(0x7FFFFFFF - gain) << 32) | (uint32_t)i
-                                            : ~0ull;
+    keys[i] = (cand[i].q >= 0 && gain > 0) ? ((uint64_t)(gmax - (uint32_t)gain) << ibits) | (uint32_t)i : ~0ull;
 }
 
 constexpr int SEL_T = 1024, SEL_B = 1024;
@@ -1061,7 +1066,7 @@ constexpr int32_t OWNER_NONE = 0x7F7F7F7F;    // empty owner (bytes 0x7F, set by
 __global__ __launch_bounds__(SEL_T) void k_lp_select(int nc, int nseg, const uint64_t *__restrict__ skeys,
                                                      const Cand *__restrict__ cand,
                                                      const int4 *__restrict__ best, int32_t *__restrict__ acc,
-                                                     int32_t *__restrict__ out, int32_t *owner) {
+                                                     int32_t *__restrict__ out, int32_t *owner, uint64_t imask) {
     extern __shared__ uint32_t used[];
     __shared__ int bp[SEL_B], bq[SEL_B], bg[SEL_B], bc[SEL_B];
     __shared__ uint8_t live[SEL_B], took[SEL_B];
@@ -1081,7 +1086,7 @@ __global__ __launch_bounds__(SEL_T) void k_lp_select(int nc, int nseg, const uin
         if (k < n) {
             const uint64_t key = skeys[base + k];
             if (key != ~0ull) {
-                const int ci = (int)(uint32_t)key;
+                const int ci = (int)(key & imask);
                 p = cand[ci].p;
                 q = cand[ci].q;
                 const int4 b = best[ci];
@@ -1579,6 +1584,11 @@ static int lp_reorder_run(vnm_graph *g, const vnm_lp_model *m, int N, int mode, 
     hipStream_t st = g->ctx->stream;
     const int ns = c->nseg, M = g->M;
     const int ncmax = std::max(topk * window, topk);
+    // sort keys hold a segment or candidate index in their low bits
+    auto bits_of = [](int64_t x) { int b = 1; while ((1ll << b) <= x) ++b; return b; };
+    const int sbits = bits_of(ns - 1), ibits = bits_of(ncmax - 1), gbits = bits_of(2 * (int64_t)g->nrows);
+    const uint32_t gmax = (uint32_t)((1ull << gbits) - 1);
+    const uint64_t smask = (1ull << sbits) - 1, imask = (1ull << ibits) - 1;
     LPRound w;
     if ((rc = w.dkey.alloc(ns)) || (rc = w.akey.alloc(ns)) || (rc = w.sdkey.alloc(ns)) ||
         (rc = w.sakey.alloc(ns)) || (rc = w.dbest.alloc(ns)) || (rc = w.abest.alloc(ns)) ||
@@ -1601,11 +1611,11 @@ static int lp_reorder_run(vnm_graph *g, const vnm_lp_model *m, int N, int mode, 
         VNM_HIP_TRY(hipMemsetAsync(w.out.p, 0, 32, st));
         hipLaunchKernelGGL(k_lp_pick, dim3(grid_of(ns, 256)), dim3(256), 0, st, ns, M, pick_mode,
                            seed + (uint64_t)it * 0x9E3779B97F4A7C15ull, g->inv.p, c->ptr.p, c->ccls.p,
-                           c->segcls.p, c->Y.p, w.dkey.p, w.akey.p, w.dbest.p, w.abest.p, w.out.p + 4);
+                           c->segcls.p, c->Y.p, w.dkey.p, w.akey.p, w.dbest.p, w.abest.p, w.out.p + 4, sbits);
         g->ctx->launches++;
         VNM_LAUNCH_CHECK("k_lp_pick");
-        if ((rc = sort_keys_u64(w.dkey.p, w.sdkey.p, ns, 64, st))) return rc;
-        if ((rc = sort_keys_u64(w.akey.p, w.sakey.p, ns, 64, st))) return rc;
+        if ((rc = sort_keys_u64(w.dkey.p, w.sdkey.p, ns, 32 + sbits, st))) return rc;
+        if ((rc = sort_keys_u64(w.akey.p, w.sakey.p, ns, 32 + sbits, st))) return rc;
         int32_t nd_h = 0;
         VNM_HIP_TRY(hipMemcpyAsync(&nd_h, w.out.p + 4, 4, hipMemcpyDeviceToHost, st));
         VNM_HIP_TRY(hipStreamSynchronize(st));
@@ -1616,18 +1626,18 @@ static int lp_reorder_run(vnm_graph *g, const vnm_lp_model *m, int N, int mode, 
         const int stride = std::max(1, ns / std::max(nd, 1));
         hipLaunchKernelGGL(k_lp_match, dim3(grid_of(nd, 256)), dim3(256), 0, st, nd, W, ns, stride, M, pick_mode,
                            all, seed ^ ((uint64_t)it << 20), w.sdkey.p, w.sakey.p, w.dbest.p, w.abest.p, g->inv.p,
-                           c->Y.p, w.cand.p);
+                           c->Y.p, w.cand.p, smask);
         g->ctx->launches++;
         VNM_LAUNCH_CHECK("k_lp_match");
         const int nc = all ? nd * W : nd;
         if ((rc = lp_score_cands(g, c, N, w.cand.p, w.best.p, nc, nullptr))) return rc;
         S.exact_pairs += nc;
         hipLaunchKernelGGL(k_lp_gain_keys, dim3(grid_of(nc, 256)), dim3(256), 0, st, nc, w.cand.p, w.best.p,
-                           w.gkey.p);
-        if ((rc = sort_keys_u64(w.gkey.p, w.sgkey.p, nc, 64, st))) return rc;
+                           w.gkey.p, ibits, gmax);
+        if ((rc = sort_keys_u64(w.gkey.p, w.sgkey.p, nc, gbits + ibits, st))) return rc;
         const size_t shm = (size_t)((ns + 31) / 32) * 4;
         hipLaunchKernelGGL(k_lp_select, dim3(1), dim3(SEL_T), shm, st, nc, ns, w.sgkey.p, w.cand.p, w.best.p,
-                           w.acc.p, w.out.p, w.owner.p);
+                           w.acc.p, w.out.p, w.owner.p, imask);
         g->ctx->launches += 2;
         VNM_LAUNCH_CHECK("k_lp_select");
         int32_t o[2];

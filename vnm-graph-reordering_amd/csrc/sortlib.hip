@@ -298,6 +298,115 @@ __global__ __launch_bounds__(WG) void k_radix_small(const uint64_t *__restrict__
     }
 }
 
+// Sorts of 2 to FUSED_TILES tiles: one launch per pass.  Each workgroup
+// derives its tile's offsets itself from the pass's (digit, tile) count
+// table (a scan of at most 256 * FUSED_TILES counts, in (digit, tile)
+// order), ranks and scatters its tile as k_radix_scatter does, and counts
+// the next pass's digits of the keys it writes by destination tile (an LDS
+// table, then one atomic add per nonzero count), so no separate histogram
+// or scan launch is needed between passes.
+constexpr int FUSED_TILES = 16;
+
+__global__ void k_fill_zero_i32(int32_t *p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+template <bool VALS>
+__global__ __launch_bounds__(WG) void k_radix_fused(const uint64_t *__restrict__ kin, const int32_t *__restrict__ vin,
+                                                    uint64_t *__restrict__ kout, int32_t *__restrict__ vout,
+                                                    int n, int shift, int bits, int tiles,
+                                                    const int32_t *__restrict__ hist, int next_shift,
+                                                    int next_bits, int32_t *__restrict__ next_hist) {
+    __shared__ int32_t run[NWAVE][RADIX];
+    __shared__ int32_t start[RADIX];
+    __shared__ int32_t offs[RADIX];
+    __shared__ int32_t sh[NWAVE];
+    __shared__ uint64_t sk[TILE];
+    __shared__ int32_t sv[VALS ? TILE : 1];
+    __shared__ int32_t nh[FUSED_TILES * RADIX];  // next pass: counts per (destination tile, digit)
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tile = blockIdx.x;
+    for (int i = threadIdx.x; i < NWAVE * RADIX; i += WG) (&run[0][0])[i] = 0;
+    if (next_hist)
+        for (int i = threadIdx.x; i < tiles * RADIX; i += WG) nh[i] = 0;
+    // this tile's offsets: digits before d over all tiles, then digit d in
+    // the tiles before this one (hist is digit-major: hist[d * tiles + t])
+    {
+        const int d = threadIdx.x;
+        int32_t row = 0, before = 0;
+        for (int t = 0; t < tiles; ++t) {
+            const int32_t x = hist[d * tiles + t];
+            before += t < tile ? x : 0;
+            row += x;
+        }
+        int32_t total;
+        offs[d] = wg_scan_excl<int32_t>(row, OpSum(), 0, sh, &total) + before;
+    }
+    __syncthreads();
+    const int base = tile * TILE + w * (64 * ROUNDS);
+    uint64_t key[ROUNDS];
+    int32_t val[ROUNDS];
+    int dig[ROUNDS], rnk[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int i = base + r * 64 + lane;
+        const bool ok = i < n;
+        key[r] = ok ? kin[i] : 0ull;
+        if constexpr (VALS) val[r] = ok ? vin[i] : 0;
+        const int d = ok ? digit_of(key[r], shift, bits) : 0;
+        uint64_t peers = __ballot(ok);
+        for (int b = 0; b < bits; ++b) {
+            const uint64_t bal = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? bal : ~bal;
+        }
+        const int below = __popcll(peers & lanemask_lt());
+        int r0 = 0;
+        if (ok) r0 = run[w][d];
+        if (ok && below == 0) run[w][d] = r0 + __popcll(peers);
+        dig[r] = ok ? d : -1;
+        rnk[r] = r0 + below;
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;
+        int32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < NWAVE; ++i) {
+            const int32_t x = run[i][d];
+            run[i][d] = c;
+            c += x;
+        }
+        int32_t total;
+        start[d] = wg_scan_excl<int32_t>(c, OpSum(), 0, sh, &total);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        if (dig[r] >= 0) {
+            const int p = start[dig[r]] + run[w][dig[r]] + rnk[r];
+            sk[p] = key[r];
+            if constexpr (VALS) sv[p] = val[r];
+        }
+    }
+    __syncthreads();
+    const int cnt = min(TILE, n - tile * TILE);
+    for (int p = threadIdx.x; p < cnt; p += WG) {
+        const uint64_t k = sk[p];
+        const int dd = digit_of(k, shift, bits);
+        const int o = offs[dd] + (p - start[dd]);
+        kout[o] = k;
+        if constexpr (VALS) vout[o] = sv[p];
+        if (next_hist) atomicAdd(&nh[(o / TILE) * RADIX + digit_of(k, next_shift, next_bits)], 1);
+    }
+    if (next_hist) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < tiles * RADIX; i += WG) {
+            const int32_t c = nh[i];
+            if (c) atomicAdd(&next_hist[(i % RADIX) * tiles + i / RADIX], c);
+        }
+    }
+}
+
 // ------------------------------------------------------------ scan
 
 // Scan input as int32: flags count as 0 or 1.
@@ -450,6 +559,35 @@ int radix_sort(const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t 
     }
     const int passes = (end_bit + 7) / 8;
     const int tiles = tiles_of(n);
+    if (tiles <= FUSED_TILES) {
+        // one count table per pass, zeroed up front; pass 0's by k_radix_hist,
+        // each later one by the pass before it
+        const size_t kb = align256((size_t)n * 8), vb = VALS ? align256((size_t)n * 4) : 0;
+        const int tab = RADIX * tiles;
+        int rc = g_tmp.ensure(kb + vb + align256((size_t)tab * passes * 4));
+        if (rc) return rc;
+        uint64_t *ktmp = (uint64_t *)g_tmp.p;
+        int32_t *vtmp = (int32_t *)((char *)g_tmp.p + kb);
+        int32_t *hists = (int32_t *)((char *)g_tmp.p + kb + vb);
+        hipLaunchKernelGGL(k_fill_zero_i32, dim3((tab * passes + 255) / 256), dim3(256), 0, st, hists, tab * passes);
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(WG), 0, st, kin, n, 0, std::min(8, end_bit), tiles, hists);
+        const uint64_t *ks = kin;
+        const int32_t *vs = vin;
+        for (int p = 0; p < passes; ++p) {
+            const int shift = 8 * p, bits = std::min(8, end_bit - shift);
+            const bool last = p + 1 == passes;
+            uint64_t *kd = ((passes - 1 - p) % 2 == 0) ? kout : ktmp;
+            int32_t *vd = ((passes - 1 - p) % 2 == 0) ? vout : vtmp;
+            hipLaunchKernelGGL((k_radix_fused<VALS>), dim3(tiles), dim3(WG), 0, st, ks, vs, kd, vd, (int)n, shift, bits,
+                               tiles, hists + (size_t)tab * p, last ? 0 : shift + 8,
+                               last ? 1 : std::min(8, end_bit - shift - 8), last ? nullptr : hists + (size_t)tab * (p + 1));
+            ks = kd;
+            vs = vd;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return set_error(VNM_E_HIP, "radix sort launch failed: %s", hipGetErrorString(e));
+        return VNM_OK;
+    }
     const int64_t nh = (int64_t)RADIX * tiles;
     // scratch: keys and values of the other buffer, digit counts, offsets,
     // scan totals
