@@ -734,7 +734,7 @@ def main():
                    "hybrid_rounds": a.rounds, "sched": 0, "parallelism": "none (1 GPU)"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"k_spmm<f32, f32, {vec32}>", "kernel_ms": round(k_ms, 5),
+                     "kernel": f"k_spmm<f32 B, {vec32} columns per lane>", "kernel_ms": round(k_ms, 5),
                      "alg_bytes": int(alg_bytes), "a_read_bytes": int(a_bytes),
                      "a_packed_bytes": int(info.bytes)},
         "reorder": {"gpu_ms": round(reorder_wall_ms, 2), "hybrid_ms": round(hs.milliseconds, 2),

@@ -268,16 +268,16 @@ typedef struct {
     int32_t nblocks;          /* stored V x M blocks                     */
     int32_t dtype;
     int64_t retained_nnz;
-    int64_t bytes;            /* device bytes of the packed format       */
-    int64_t spmm_bytes;       /* bytes of it one vnm_spmm reads: per entry
-                                 row, column, selector and two values
-                                 (plus, for FP8, the block index and the
-                                 block scales)                           */
+    int64_t bytes;            /* device bytes of the packed format, its
+                                 SpMM value stream included              */
+    int64_t spmm_bytes;       /* bytes of it one vnm_spmm reads: the value
+                                 stream (row, B row and FP32 weight per
+                                 kept value) and the work-item table     */
     int32_t items;            /* SpMM work items (whole rows of at most 64
-                                 row entries, or 64-entry chunks of longer
+                                 values, or 64-value chunks of longer
                                  rows)                                    */
     int32_t slots;            /* rows of partial sums: one per chunk of a
-                                 row longer than 64 row entries, plus V
+                                 row longer than 64 values, plus V
                                  per extra work unit of the sparse-MFMA
                                  part; a product with n columns needs
                                  slots * n floats of scratch             */

@@ -1,7 +1,7 @@
 // Batched V:N:M SpMM and the GCN aggregation layer (gfx950).
 //
 // Batch: many packed matrices of uneven size multiplied in one persistent
-// launch.  The work items (64 row entries x one column tile) of all matrices
+// launch.  The work items (64 values x one column tile) of all matrices
 // form one list (prefix offsets per matrix); a fixed grid of 8 workgroups per
 // CU walks it, each XCD taking a contiguous range so neighbouring items of
 // one matrix share that XCD's L2.  A device table holds every matrix's
@@ -40,7 +40,7 @@ struct vnm_batch {
 
 namespace vnm {
 
-template <int AT, int BT, int VEC>
+template <int BT, int VEC>
 __global__ __launch_bounds__(256) void k_spmm_batch(const SpmmArgs *__restrict__ descs,
                                                     const int32_t *__restrict__ unit_off, int count, int T,
                                                     int total, int per_xcd, int waves_per_xcd) {
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(256) void k_spmm_batch(const SpmmArgs *__restrict__
             else hi = mid - 1;
         }
         const int local = unit - unit_off[lo];
-        spmm_unit<AT, BT, VEC, true>(descs[lo], local / T, local % T);
+        spmm_unit<BT, VEC>(descs[lo], local / T, local % T, T);
     }
 }
 
@@ -78,29 +78,28 @@ __global__ void k_tag_fix(const int4 *__restrict__ fix, int nf, int g, int4 *__r
     if (i < nf) out[i] = make_int4(g, fix[i].x, fix[i].y, fix[i].z);
 }
 
-template <int AT, int BT, int VEC>
+template <int BT, int VEC>
 static void launch_batch_v(vnm_batch *b, hipStream_t st) {
-    hipLaunchKernelGGL((k_spmm_batch<AT, BT, VEC>), dim3(b->grid), dim3(256), 0, st, b->descs.p, b->unit_off.p,
+    hipLaunchKernelGGL((k_spmm_batch<BT, VEC>), dim3(b->grid), dim3(256), 0, st, b->descs.p, b->unit_off.p,
                        b->count, b->T, b->total_units, b->per_xcd, b->waves_per_xcd);
 }
 
-template <int AT>
-static void launch_batch_a(vnm_batch *b, hipStream_t st) {
+static void launch_batch_b(vnm_batch *b, hipStream_t st) {
     const int v = b->vec;
     if (b->bt == VNM_DT_F32) {
-        if (v == 4) launch_batch_v<AT, VNM_DT_F32, 4>(b, st);
-        else if (v == 2) launch_batch_v<AT, VNM_DT_F32, 2>(b, st);
-        else launch_batch_v<AT, VNM_DT_F32, 1>(b, st);
+        if (v == 4) launch_batch_v<VNM_DT_F32, 4>(b, st);
+        else if (v == 2) launch_batch_v<VNM_DT_F32, 2>(b, st);
+        else launch_batch_v<VNM_DT_F32, 1>(b, st);
     } else if (b->bt == VNM_DT_BF16) {
-        if (v == 8) launch_batch_v<AT, VNM_DT_BF16, 8>(b, st);
-        else if (v == 4) launch_batch_v<AT, VNM_DT_BF16, 4>(b, st);
-        else if (v == 2) launch_batch_v<AT, VNM_DT_BF16, 2>(b, st);
-        else launch_batch_v<AT, VNM_DT_BF16, 1>(b, st);
+        if (v == 8) launch_batch_v<VNM_DT_BF16, 8>(b, st);
+        else if (v == 4) launch_batch_v<VNM_DT_BF16, 4>(b, st);
+        else if (v == 2) launch_batch_v<VNM_DT_BF16, 2>(b, st);
+        else launch_batch_v<VNM_DT_BF16, 1>(b, st);
     } else {
-        if (v == 16) launch_batch_v<AT, VNM_DT_FP8, 16>(b, st);
-        else if (v == 8) launch_batch_v<AT, VNM_DT_FP8, 8>(b, st);
-        else if (v == 4) launch_batch_v<AT, VNM_DT_FP8, 4>(b, st);
-        else launch_batch_v<AT, VNM_DT_FP8, 2>(b, st);
+        if (v == 16) launch_batch_v<VNM_DT_FP8, 16>(b, st);
+        else if (v == 8) launch_batch_v<VNM_DT_FP8, 8>(b, st);
+        else if (v == 4) launch_batch_v<VNM_DT_FP8, 4>(b, st);
+        else launch_batch_v<VNM_DT_FP8, 2>(b, st);
     }
 }
 
@@ -121,12 +120,7 @@ static void batch_shape(vnm_batch *b, const std::vector<int> &items, int vec, st
 
 static int batch_enqueue(vnm_batch *b, hipStream_t st) {
     if (b->total_units > 0) {
-        switch (b->at) {
-            case VNM_DT_F32: launch_batch_a<VNM_DT_F32>(b, st); break;
-            case VNM_DT_BF16: launch_batch_a<VNM_DT_BF16>(b, st); break;
-            case VNM_DT_F16: launch_batch_a<VNM_DT_F16>(b, st); break;
-            default: launch_batch_a<VNM_DT_FP8>(b, st); break;
-        }
+        launch_batch_b(b, st);
     }
     if (b->nfix > 0)
         hipLaunchKernelGGL(k_fix_batch, dim3((b->nfix + 3) / 4), dim3(256), 0, st, b->fix.p, b->nfix, b->n,

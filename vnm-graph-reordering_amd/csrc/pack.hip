@@ -19,21 +19,22 @@
 // (rowptr; per row entry its row, block, selector byte and two values; per
 // block its segment, selectors and scale).
 //
-// SpMM: C[R x n] = A_vnm B, FP32 accumulation.  Row entries also carry the
-// absolute column of their first value (ent_col) and the in-segment offset of
-// the second (low nibble of ent_idx, 0x10 = present), so the SpMM reads no
-// block data (except the block scale of FP8 A).  A work item holds whole rows
-// of at most 64 row entries, or a 64-entry chunk of a longer row (item table
-// built once per packed matrix); a wave takes one item for one column tile of
-// 64 * VEC columns: lane k first resolves entry k (its two B rows and scaled
-// values), then the wave walks the entries with the entry data broadcast by
-// scalar reads, every lane gathering VEC consecutive values of each B row (2
-// to 16 bytes) and owning VEC columns of the output row, which is stored when
-// the row changes (rows that keep nothing are written in the gaps).  The
-// chunks of a long row store partial sums into slots, and a fix-up pass adds
-// each cut row's slots in item order, so C is bit-identical for every launch
-// shape and run: no atomics.  When the tile count T divides 8, tile t runs on its own
-// 8/T XCDs, so each XCD's L2 serves one slice of B.  VEC is chosen per (B
+// SpMM: C[R x n] = A_vnm B, FP32 accumulation.  Packing also writes the
+// SpMM's value stream: every kept value of the row entries, row-major, as
+// (row, B row, FP32 weight = value x block scale), 12 bytes per value, so the
+// SpMM reads no block data and branches on no selector.  A work item holds
+// whole rows of at most 64 values, or a 64-value chunk of a longer row (item
+// table built once per packed matrix); a wave takes one item for one column
+// tile of 64 * VEC columns and walks it row by row: the row's B rows and
+// weights arrive as scalar loads, every lane gathers VEC consecutive values of
+// each B row (2 to 16 bytes) into VEC FMAs with the weight as a scalar
+// operand, and owns VEC columns of the output row, which is stored when the
+// row ends (rows that keep nothing are written in the gaps).  The chunks of a
+// long row store partial sums into slots, and a fix-up pass adds each cut
+// row's slots in item order, so C is bit-identical for every launch shape and
+// run: no atomics.  The T column tiles interleave in 128-byte pieces of each
+// row, and when T divides 8, tile t runs on its own 8/T XCDs, so each XCD's L2
+// serves one 1/T slice of B spread over all of its channels.  VEC is chosen per (B
 // dtype, n) by timing the usable values on first use (vnm_spmm_set_shape
 // overrides).  FP8 B is decoded by the gfx950 e4m3 conversion instruction.
 #include "graph.h"
@@ -373,9 +374,71 @@ __global__ void k_emit(const int32_t *__restrict__ sel_runs, int E, const int32_
 
 // ------------------------------------------------------------ SpMM
 
+// ------------------------------------------------------------ value stream
+
+// Values per row entry (1 or 2), and entry E's count 0 (the scan's total).
+__global__ void k_val_counts(const uint8_t *__restrict__ ent_idx, int E, int32_t *__restrict__ nv) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < E) nv[e] = (ent_idx[e] & 0x10u) ? 2 : 1;
+    else if (e == E) nv[e] = 0;
+}
+
+template <int DT>
+__device__ __forceinline__ void emit_values(int e, int o, int M, const int32_t *__restrict__ ent_row,
+                                            const int32_t *__restrict__ ent_col, const uint8_t *__restrict__ ent_idx,
+                                            const uint8_t *__restrict__ ent_val, const int32_t *__restrict__ ent_blk,
+                                            const float *__restrict__ blk_scale, int32_t *__restrict__ vrow,
+                                            int32_t *__restrict__ vcol, float *__restrict__ vw) {
+    const int r = ent_row[e], c0 = ent_col[e];
+    const uint32_t code = ent_idx[e];
+    const float sc = DT == VNM_DT_FP8 ? blk_scale[ent_blk[e]] : 1.f;
+    vrow[o] = r;
+    vcol[o] = c0;
+    vw[o] = load_aval<DT>(ent_val, 2 * e) * sc;
+    if (code & 0x10u) {
+        vrow[o + 1] = r;
+        vcol[o + 1] = (c0 & ~(M - 1)) | (int)(code & 15u);
+        vw[o + 1] = load_aval<DT>(ent_val, 2 * e + 1) * sc;
+    }
+}
+
+// The SpMM's value stream: row entry e's values (its first, then its second)
+// at voff[e] as (row, B row, weight), the weight being the stored value times
+// the block scale (FP8; 1 otherwise) rounded to FP32 once.
+__global__ void k_values(int E, int M, int dtype, const int32_t *__restrict__ voff,
+                         const int32_t *__restrict__ ent_row, const int32_t *__restrict__ ent_col,
+                         const uint8_t *__restrict__ ent_idx, const uint8_t *__restrict__ ent_val,
+                         const int32_t *__restrict__ ent_blk, const float *__restrict__ blk_scale,
+                         int32_t *__restrict__ vrow, int32_t *__restrict__ vcol, float *__restrict__ vw) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int o = voff[e];
+    switch (dtype) {
+        case VNM_DT_F32:
+            emit_values<VNM_DT_F32>(e, o, M, ent_row, ent_col, ent_idx, ent_val, ent_blk, blk_scale, vrow, vcol, vw);
+            break;
+        case VNM_DT_BF16:
+            emit_values<VNM_DT_BF16>(e, o, M, ent_row, ent_col, ent_idx, ent_val, ent_blk, blk_scale, vrow, vcol, vw);
+            break;
+        case VNM_DT_F16:
+            emit_values<VNM_DT_F16>(e, o, M, ent_row, ent_col, ent_idx, ent_val, ent_blk, blk_scale, vrow, vcol, vw);
+            break;
+        default:
+            emit_values<VNM_DT_FP8>(e, o, M, ent_row, ent_col, ent_idx, ent_val, ent_blk, blk_scale, vrow, vcol, vw);
+            break;
+    }
+}
+
+// Row pointers over values: vrp[r] = voff[rowptr[r]], r <= R.
+__global__ void k_vrowptr(const int32_t *__restrict__ rowptr, const int32_t *__restrict__ voff, int R,
+                          int32_t *__restrict__ vrp) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r <= R) vrp[r] = voff[rowptr[r]];
+}
+
 // ------------------------------------------------------------ SpMM kernel
 
-template <int AT, int BT, int VEC>
+template <int BT, int VEC>
 __global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int T, int items, int per_g) {
     const int wv = threadIdx.x >> 6;
     // workgroups are dealt round-robin over the 8 XCDs
@@ -393,7 +456,7 @@ __global__ __launch_bounds__(256) void k_spmm(SpmmArgs a, int T, int items, int 
     // wave-uniform; readfirstlane lets the entry reads be scalar loads
     const int item = __builtin_amdgcn_readfirstlane(iwg * 4 + wv);
     if (item >= items) return;
-    spmm_unit<AT, BT, VEC>(a, item, __builtin_amdgcn_readfirstlane(tile));
+    spmm_unit<BT, VEC>(a, item, __builtin_amdgcn_readfirstlane(tile), T);
 }
 
 // Fix-up pass: rows cut into chunks get the sum of their partial-sum slots
@@ -427,19 +490,19 @@ static std::vector<uint32_t> subsets4(int M) {
     return out;
 }
 
-// SpMM work items from the row pointers (host pass, once per packed matrix):
-// consecutive whole rows while they fit in ITEM entries; a row longer than
-// ITEM starts a new item and is cut into ITEM-entry chunks, each with a
+// SpMM work items from the value row pointers (host pass, once per packed matrix):
+// consecutive whole rows while they fit in ITEM values; a row longer than
+// ITEM starts a new item and is cut into ITEM-value chunks, each with a
 // partial-sum slot of its own (consecutive slots), the last chunk staying
 // open for the rows that follow.  So an item's only cut row is its first.
 // The fix-up list holds the cut rows with their slot range and the rows of
 // runs of more than GAP_MAX rows that keep nothing (the SpMM's item walk
 // writes the shorter runs; without any item, the list holds every row).
-static int build_items(vnm_packed *p) {
+static int build_items(vnm_packed *p, const int32_t *d_vrp) {
     hipStream_t st = p->ctx->stream;
     const int R = p->nrows;
     std::vector<int32_t> rp(R + 1);
-    VNM_HIP_TRY(hipMemcpyAsync(rp.data(), p->rowptr.p, (size_t)(R + 1) * 4, hipMemcpyDeviceToHost, st));
+    VNM_HIP_TRY(hipMemcpyAsync(rp.data(), d_vrp, (size_t)(R + 1) * 4, hipMemcpyDeviceToHost, st));
     VNM_HIP_TRY(hipStreamSynchronize(st));
     std::vector<int32_t> beg, slot;
     std::vector<int4> fix;
@@ -662,7 +725,37 @@ int pack(vnm_graph *g, int V, int dtype, int dense_min, vnm_packed **out, int64_
                            M, dtype, p->ent_row.p, p->ent_col.p, p->ent_blk.p, p->ent_idx.p, p->ent_val.p);
     g->ctx->launches += 17;
     VNM_LAUNCH_CHECK("pack emit");
-    if ((rc = build_items(p))) { delete p; return rc; }
+    // the value stream and its row pointers
+    DBuf<int32_t> nv, voff, vrp;
+    if ((rc = nv.alloc((size_t)E + 1)) || (rc = voff.alloc((size_t)E + 1)) || (rc = vrp.alloc((size_t)R + 1))) {
+        delete p;
+        return rc;
+    }
+    hipLaunchKernelGGL(k_val_counts, dim3(grid_of((int64_t)E + 1, 256)), dim3(256), 0, st, p->ent_idx.p, E, nv.p);
+    if ((rc = scan_excl_i32(nv.p, voff.p, (int64_t)E + 1, st))) { delete p; return rc; }
+    int NV = 0;
+    VNM_HIP_TRY(hipMemcpyAsync(&NV, voff.p + E, 4, hipMemcpyDeviceToHost, st));
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    p->nval = NV;
+    // 8 zero records past the end: the SpMM reads B rows and weights 8 values
+    // at a time
+    const size_t vn = (size_t)NV + 8;
+    if ((rc = p->vrow.alloc(vn)) || (rc = p->vcol.alloc(vn)) || (rc = p->vw.alloc(vn))) {
+        delete p;
+        return rc;
+    }
+    VNM_HIP_TRY(hipMemsetAsync(p->vrow.p + NV, 0, 8 * 4, st));
+    VNM_HIP_TRY(hipMemsetAsync(p->vcol.p + NV, 0, 8 * 4, st));
+    VNM_HIP_TRY(hipMemsetAsync(p->vw.p + NV, 0, 8 * 4, st));
+    if (E > 0)
+        hipLaunchKernelGGL(k_values, dim3(grid_of(E, 256)), dim3(256), 0, st, E, M, dtype, voff.p, p->ent_row.p,
+                           p->ent_col.p, p->ent_idx.p, p->ent_val.p, p->ent_blk.p, p->blk_scale.p,
+                           p->vrow.p, p->vcol.p, p->vw.p);
+    hipLaunchKernelGGL(k_vrowptr, dim3(grid_of((int64_t)R + 1, 256)), dim3(256), 0, st, p->rowptr.p, voff.p, R,
+                       vrp.p);
+    g->ctx->launches += 4;
+    VNM_LAUNCH_CHECK("pack values");
+    if ((rc = build_items(p, vrp.p))) { delete p; return rc; }
     *out = p;
     return VNM_OK;
 }
@@ -694,19 +787,15 @@ SpmmArgs packed_args(const vnm_packed *p, const void *B, int ldb, const float *b
     SpmmArgs a;
     a.item_beg = p->item_beg.p;
     a.item_slot = p->item_slot.p;
-    a.ent_row = p->ent_row.p;
-    a.ent_col = p->ent_col.p;
-    a.ent_idx = p->ent_idx.p;
-    a.ent_val = p->ent_val.p;
-    a.ent_blk = p->ent_blk.p;
-    a.blk_scale = p->blk_scale.p;
+    a.vrow = p->vrow.p;
+    a.vcol = p->vcol.p;
+    a.vw = p->vw.p;
     a.B = (const uint8_t *)B;
     a.b_scales = bs;
     a.C = C;
     a.part = part;
     a.bias = bias;
     a.items = p->nitems;
-    a.M = p->M;
     a.ldb = ldb;
     a.ldc = ldc;
     a.pn = pn;
@@ -717,7 +806,7 @@ SpmmArgs packed_args(const vnm_packed *p, const void *B, int ldb, const float *b
     return a;
 }
 
-template <int AT, int BT, int VEC>
+template <int BT, int VEC>
 static void launch_v(const SpmmArgs &args, int n, hipStream_t st) {
     const int T = n / (64 * VEC);
     const int items = args.items;
@@ -729,25 +818,26 @@ static void launch_v(const SpmmArgs &args, int n, hipStream_t st) {
     } else {
         per_g = (iwg * T + 7) / 8;
     }
-    hipLaunchKernelGGL((k_spmm<AT, BT, VEC>), dim3(8 * per_g), dim3(256), 0, st, args, T, items, per_g);
+    hipLaunchKernelGGL((k_spmm<BT, VEC>), dim3(8 * per_g), dim3(256), 0, st, args, T, items, per_g);
 }
 
-template <int AT>
-static void launch_a(const SpmmArgs &args, int vec, int n, int bt, hipStream_t st) {
+// The value stream carries FP32 weights, so the kernel depends on B's dtype
+// and the values per lane only.
+static void launch_b(const SpmmArgs &args, int vec, int n, int bt, hipStream_t st) {
     if (bt == VNM_DT_F32) {
-        if (vec == 4) launch_v<AT, VNM_DT_F32, 4>(args, n, st);
-        else if (vec == 2) launch_v<AT, VNM_DT_F32, 2>(args, n, st);
-        else launch_v<AT, VNM_DT_F32, 1>(args, n, st);
+        if (vec == 4) launch_v<VNM_DT_F32, 4>(args, n, st);
+        else if (vec == 2) launch_v<VNM_DT_F32, 2>(args, n, st);
+        else launch_v<VNM_DT_F32, 1>(args, n, st);
     } else if (bt == VNM_DT_BF16) {
-        if (vec == 8) launch_v<AT, VNM_DT_BF16, 8>(args, n, st);
-        else if (vec == 4) launch_v<AT, VNM_DT_BF16, 4>(args, n, st);
-        else if (vec == 2) launch_v<AT, VNM_DT_BF16, 2>(args, n, st);
-        else launch_v<AT, VNM_DT_BF16, 1>(args, n, st);
+        if (vec == 8) launch_v<VNM_DT_BF16, 8>(args, n, st);
+        else if (vec == 4) launch_v<VNM_DT_BF16, 4>(args, n, st);
+        else if (vec == 2) launch_v<VNM_DT_BF16, 2>(args, n, st);
+        else launch_v<VNM_DT_BF16, 1>(args, n, st);
     } else {
-        if (vec == 16) launch_v<AT, VNM_DT_FP8, 16>(args, n, st);
-        else if (vec == 8) launch_v<AT, VNM_DT_FP8, 8>(args, n, st);
-        else if (vec == 4) launch_v<AT, VNM_DT_FP8, 4>(args, n, st);
-        else launch_v<AT, VNM_DT_FP8, 2>(args, n, st);
+        if (vec == 16) launch_v<VNM_DT_FP8, 16>(args, n, st);
+        else if (vec == 8) launch_v<VNM_DT_FP8, 8>(args, n, st);
+        else if (vec == 4) launch_v<VNM_DT_FP8, 4>(args, n, st);
+        else launch_v<VNM_DT_FP8, 2>(args, n, st);
     }
 }
 
@@ -762,12 +852,7 @@ int spmm_run(vnm_packed *p, int vec, const void *B, int ldb, int n, int bt, cons
     if (rc) return rc;
     if (p->nitems > 0) {
         KernelTimer kt(timed ? p->ctx : nullptr, "k_spmm");
-        switch (p->dtype) {
-            case VNM_DT_F32: launch_a<VNM_DT_F32>(args, vec, n, bt, st); break;
-            case VNM_DT_BF16: launch_a<VNM_DT_BF16>(args, vec, n, bt, st); break;
-            case VNM_DT_F16: launch_a<VNM_DT_F16>(args, vec, n, bt, st); break;
-            default: launch_a<VNM_DT_FP8>(args, vec, n, bt, st); break;
-        }
+        launch_b(args, vec, n, bt, st);
         kt.stop();
         p->ctx->launches++;
     }
@@ -891,9 +976,11 @@ int vnm_packed_info_get(const vnm_packed *p, vnm_packed_info *info) {
     const int blk = p->dtype == VNM_DT_FP8 ? 4 : 0;  // per-entry block index for the FP8 scale
     info->bytes = 4ll * (p->nrows + 1) + (int64_t)p->nent * (4 + 4 + 1 + 2 * vs + blk) +
                   (int64_t)p->nblocks * (4 + 2 + 4);
-    // what k_spmm reads: the entry arrays, and for FP8 A the scales of the
-    // blocks (rowptr is read at most twice per 64 entries)
-    info->spmm_bytes = (int64_t)p->nent * (4 + 4 + 1 + 2 * vs + blk) + (blk ? (int64_t)p->nblocks * 4 : 0);
+    // the SpMM's value stream: (row, B row, FP32 weight) per value
+    info->bytes += (int64_t)p->nval * 12;
+    // what k_spmm reads: the value stream and the item table (first value
+    // and slot of each item)
+    info->spmm_bytes = (int64_t)p->nval * 12 + (int64_t)p->nitems * 8;
     info->items = p->nitems;
     info->slots = (int32_t)scratch_rows(p);
     info->dense_blocks = p->ndense;

@@ -23,10 +23,15 @@ struct vnm_packed {
     vnm::DBuf<uint8_t> ent_idx, ent_val;                   // values: 2 per entry, dtype-sized
     vnm::DBuf<uint16_t> blk_sel;
     vnm::DBuf<float> blk_scale;
-    // SpMM work items: whole rows of at most ITEM row entries, and rows longer
-    // than ITEM cut into ITEM-entry chunks whose partial sums go to slots
+    // the SpMM's value stream: every value of the row entries, row-major, as
+    // (row, B row, FP32 weight), and 8 zero records past its end
+    int32_t nval = 0;
+    vnm::DBuf<int32_t> vrow, vcol;
+    vnm::DBuf<float> vw;
+    // SpMM work items: whole rows of at most ITEM values, and rows longer
+    // than ITEM cut into ITEM-value chunks whose partial sums go to slots
     int32_t nitems = 0, nslots = 0, nfix = 0;
-    vnm::DBuf<int32_t> item_beg;                           // nitems + 1 entry offsets
+    vnm::DBuf<int32_t> item_beg;                           // nitems + 1 value offsets
     vnm::DBuf<int32_t> item_slot;                          // slot of the item's first row, or -1
     vnm::DBuf<int4> fix;                                   // {row, first slot, slots, 0}: rows the
                                                            // fix-up pass writes (empty or cut)
@@ -45,7 +50,7 @@ struct vnm_packed {
 
 namespace vnm {
 
-constexpr int ITEM = 64;      // row entries per work item
+constexpr int ITEM = 64;      // values per work item
 constexpr int GAP_MAX = 8;    // runs of rows without entries up to this long are written by the
                               // item walk; longer runs by the fix-up pass (one wave per row)
 
@@ -184,17 +189,16 @@ __device__ __forceinline__ void load_vec(const float *__restrict__ cp, float (&v
 
 // One packed matrix as the SpMM kernels see it.
 struct SpmmArgs {
-    const int32_t *item_beg, *item_slot, *ent_row, *ent_col;
-    const uint8_t *ent_idx, *ent_val;
-    const int32_t *ent_blk;
-    const float *blk_scale;
+    const int32_t *item_beg, *item_slot;
+    const int32_t *vrow, *vcol;  // the value stream
+    const float *vw;
     const uint8_t *B;
     const float *b_scales;
     float *C;
     float *part;        // nslots x pn partial sums of cut rows
     const float *bias;  // nullptr or n floats added to every output row
     const uint8_t *dvb; // nullptr, or per V-row block 1 when C already holds the block's dense sums
-    int items, M, ldb, ldc, pn;
+    int items, ldb, ldc, pn;
     int b_rows;         // rows of B (columns of A); b_rows * ldb * sizeof(value) < 2^32
     int c_rows;         // rows of C (of A)
     int V;
@@ -234,78 +238,60 @@ __device__ __forceinline__ float lane_bcast(float v, int k) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-// C = A_vnm B for one work item (`cnt` <= ITEM row entries starting at
-// `beg`) and column tile `tile` (64 * VEC columns), one wave.  Lane k first
-// resolves entry k: its row, its scaled values and the byte offsets of its
-// B rows (column * ldb * sizeof(B value); -1 for an absent second value).
-// The wave then walks the entries, U at a time, with each entry's data
-// broadcast by readlane: every lane gathers VEC values of the entry's B rows
-// by buffer loads whose row offset is that broadcast scalar, and owns VEC
-// columns of the output row.  A row is stored when the row changes, and the
-// rows that keep nothing between it and the next (up to the next item's first
-// row `next_row`; item 0 also from row 0) get the bias or zero.  Items
-// hold whole rows, except chunks of rows longer than ITEM entries: such a
-// chunk is the first row of its item, and its sum goes to the item's
-// partial-sum slot (slot >= 0) instead of C; the fix-up pass adds a cut
-// row's slots in item order.  So every output value is one fixed sequence of
-// FP32 operations, whatever the launch shape.  PIPE: two register sets of U
-// entries, the gathers of the next group in flight while the current one is
-// folded in (the persistent batched kernel runs faster this way; the
-// one-matrix kernel does not).
-template <int AT, int BT, int VEC, bool PIPE = false>
-__device__ __forceinline__ void spmm_item(const int32_t *__restrict__ erow, const int32_t *__restrict__ ecol,
-                                          const uint8_t *__restrict__ eidx, const uint8_t *__restrict__ evals,
-                                          const int32_t *__restrict__ eblk, const float *__restrict__ blk_scale,
-                                          const float *__restrict__ bsc, const uint8_t *__restrict__ B,
-                                          float *__restrict__ C, float *__restrict__ part,
-                                          const float *__restrict__ bias, const uint8_t *__restrict__ dvb,
-                                          int V, int beg, int cnt, int slot, int lead_row, int next_row, int M,
-                                          int ldb, int ldc, int pn, int b_rows, int tile) {
+// C = A_vnm B for one work item (`cnt` <= ITEM values from `beg` on in the
+// value stream) and column tile `tile` of T (64 * VEC columns), one wave.  The
+// value stream holds every kept value of the row entries in row-major order
+// as (row, B row, weight); the weight is the stored value times its block's
+// scale (FP8 A), rounded to FP32 once at packing, as a product formed here
+// would be.  Lane k reads the row of value k and a ballot marks where rows
+// start; the wave then walks the item in blocks of 8 values.  A block's B
+// rows and weights are scalar loads, so per value the wave spends one scalar
+// multiply (the B row's byte offset), one buffer load (VEC consecutive values
+// of the B row per lane, 2 to 16 bytes) and VEC FMAs that take the weight as
+// a scalar operand: no per-value lane broadcasts or branches.  All 8 gathers
+// of a block are in flight before its FMAs, whatever the row lengths; the
+// FMAs go one row segment of the block at a time.  Each lane owns VEC
+// columns of the output row and stores them when the row ends; the rows that
+// keep nothing between it and the next (up to the next item's first row
+// `next_row`; item 0 also from row 0) get the bias or zero.  Items hold whole
+// rows, except chunks of rows longer than ITEM values: such a chunk is the
+// first row of its item, and its sum goes to the item's partial-sum slot
+// (slot >= 0) instead of C; the fix-up pass adds a cut row's slots in item
+// order.  So every output value is one fixed sequence of FP32 operations (the
+// row's values in order), whatever the launch shape.
+template <int BT, int VEC>
+__device__ __forceinline__ void spmm_item(const int32_t *__restrict__ vrow, const int32_t *__restrict__ vcol,
+                                          const float *__restrict__ vw, const float *__restrict__ bsc,
+                                          const uint8_t *__restrict__ B, float *__restrict__ C,
+                                          float *__restrict__ part, const float *__restrict__ bias,
+                                          const uint8_t *__restrict__ dvb, int V, int beg, int cnt, int slot,
+                                          int lead_row, int next_row, int ldb, int ldc, int pn, int b_rows,
+                                          int tile, int T) {
     constexpr int SB = bsize<BT>();
-    // entries per group: up to 8; pipelined, 16 VGPRs of gathered values
-    // per register set
-    constexpr int NW = BRaw<BT, VEC>::NW;
-    constexpr int U = PIPE ? (NW >= 4 ? 2 : 8 / NW) : (BRaw<BT, VEC>::BYTES >= 16 ? 4 : 8);
     const int lane = threadIdx.x & 63;
     const int rowb = ldb * SB;
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void *)B, 0, (int)((uint32_t)b_rows * (uint32_t)rowb), 0x00020000);
-    int rr = 0, off0 = 0, off1 = -1;
-    float a0 = 0.f, a1 = 0.f;
-    if (lane < cnt) {
-        const int e = beg + lane;
-        rr = erow[e];
-        const int c0 = ecol[e];
-        const uint32_t code = eidx[e];
-        float sc = 1.f;
-        if constexpr (AT == VNM_DT_FP8) sc = blk_scale[eblk[e]];
-        a0 = load_aval<AT>(evals, 2 * e) * sc;
-        if constexpr (BT == VNM_DT_FP8) a0 *= bsc[c0 >> 7];
-        off0 = c0 * rowb;
-        if (code & 0x10u) {
-            const int c1 = (c0 & ~(M - 1)) | (int)(code & 15u);
-            a1 = load_aval<AT>(evals, 2 * e + 1) * sc;
-            if constexpr (BT == VNM_DT_FP8) a1 *= bsc[c1 >> 7];
-            off1 = c1 * rowb;
-        }
-    }
+    const int rr = lane < cnt ? vrow[beg + lane] : 0;
     const int first_row = lane_bcast(rr, 0);
-    // entries that start a new row (entry 0 does not: cur starts there)
+    // values that start a new row (value 0 does not: cur starts there)
     const int prev_rr = __shfl_up(rr, 1, 64);
     const uint64_t starts = __ballot(lane > 0 && lane < cnt && rr != prev_rr);
-    const int col0 = tile * (64 * VEC) + lane * VEC;
+    // the T column tiles interleave in 128-byte pieces (one L2 line of a B
+    // row): tile t takes pieces t, t + T, t + 2T, ... of every row, so each
+    // XCD's L2 holds its 1/T of B spread over the whole row
+    constexpr int LB = VEC * SB;  // bytes per lane
+    const int col0 = (((lane * LB) >> 7) * T + tile) * (128 / SB) + (((lane * LB) & 127) / SB);
     const int voff = col0 * SB;
     const float *bias_c = bias ? bias + col0 : nullptr;
     float acc[VEC];
-#pragma unroll
-    for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
     int cur = first_row;
     // rows whose V-row block has a dense part hold its sum already (written by
-    // the dense kernel): row-entry sums add to it
+    // the dense kernel): row sums add to it
     auto dense_row = [&](int r) { return dvb != nullptr && dvb[r / V] != 0; };
-    // rows [r0, r1) -- a maximal run without row entries -- get the bias or
-    // zero (plus a dense sum); runs longer than GAP_MAX are the fix-up
-    // pass's, so no wave walks thousands of empty rows
+    // rows [r0, r1) -- a maximal run without values -- get the bias or zero
+    // (plus a dense sum); runs longer than GAP_MAX are the fix-up pass's, so
+    // no wave walks thousands of empty rows
     auto fill = [&](int r0, int r1) {
         if (r1 - r0 > GAP_MAX) return;
         float z[VEC];
@@ -340,113 +326,79 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ erow, cons
             store_vec<VEC>(cp, acc, bias_c);
         }
     };
-    // U entries: broadcast, gather, then fold in (flushing on a row change);
-    // CHECK only for the item's last, partial group
-    auto group = [&](int k, auto check) {
-        constexpr bool CHECK = decltype(check)::value;
-        BRaw<BT, VEC> raw[U][2];
-        float w[U][2];
-        int o1[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int kk = k + u;  // wave-uniform
-            const bool ok = !CHECK || kk < cnt;
-            const int ks = kk & 63;
-            w[u][0] = ok ? lane_bcast(a0, ks) : 0.f;
-            raw[u][0] = ok ? load_row<BT, VEC>(rs, voff, lane_bcast(off0, ks)) : zero_raw<BT, VEC>();
-            o1[u] = ok ? lane_bcast(off1, ks) : -1;
-            w[u][1] = o1[u] >= 0 ? lane_bcast(a1, ks) : 0.f;
-            raw[u][1] = o1[u] >= 0 ? load_row<BT, VEC>(rs, voff, o1[u]) : zero_raw<BT, VEC>();
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int kk = k + u;
-            if ((starts >> (kk & 63)) & 1ull) {  // wave-uniform: a new row starts here
-                flush();
-#pragma unroll
-                for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
-                const int nxt = lane_bcast(rr, kk & 63);
-                fill(cur + 1, nxt);
-                cur = nxt;
-            }
-            if (!CHECK || kk < cnt) {
-                fma_raw<BT, VEC>(acc, w[u][0], raw[u][0]);
-                if (o1[u] >= 0) fma_raw<BT, VEC>(acc, w[u][1], raw[u][1]);
-            }
-        }
+    for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
+    const int32_t *__restrict__ cp = vcol + beg;
+    const float *__restrict__ wp = vw + beg;
+    // the B row of one value (the weight picks up the B row's FP8 scale)
+    auto gather = [&](int c, float &w) {
+        if constexpr (BT == VNM_DT_FP8) w *= bsc[c >> 7];
+        return load_row<BT, VEC>(rs, voff, (uint32_t)c * (uint32_t)rowb);
     };
-    if constexpr (!PIPE) {
-        const int full = cnt & ~(U - 1);
-        for (int k = 0; k < full; k += U) group(k, std::false_type{});
-        if (full < cnt) group(full, std::true_type{});
-    } else {
-        // the same gathers and folds as group(), split so that a group's
-        // loads are issued before the previous group is folded in; entries
-        // past cnt gather and fold nothing
-        using Raw = BRaw<BT, VEC>[U][2];
-        auto issue = [&](int k, Raw &raw, auto check) {
-            constexpr bool CHECK = decltype(check)::value;
+    // blocks of 8 values (the item's last may hold fewer, m): their B rows and
+    // weights by scalar loads, all gathers in flight, then the FMAs in value
+    // order, one row segment of the block at a time (a block without a row
+    // start is one segment)
+    for (int k0 = 0; k0 < cnt; k0 += 8) {
+        const int m = min(8, cnt - k0);
+        int c[8];
+        float w[8];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int kk = k + u;  // wave-uniform
-                const bool ok = !CHECK || kk < cnt;
-                const int ks = kk & 63;
-                raw[u][0] = ok ? load_row<BT, VEC>(rs, voff, lane_bcast(off0, ks)) : zero_raw<BT, VEC>();
-                const int o1 = ok ? lane_bcast(off1, ks) : -1;
-                raw[u][1] = o1 >= 0 ? load_row<BT, VEC>(rs, voff, o1) : zero_raw<BT, VEC>();
-            }
-        };
-        auto issue_any = [&](int k, Raw &raw) {
-            if (k + U <= cnt) issue(k, raw, std::false_type{});
-            else issue(k, raw, std::true_type{});
-        };
-        auto fold = [&](int k, const Raw &raw) {
+        for (int t = 0; t < 8; ++t) {  // past the item: later items or the stream's zero tail
+            c[t] = cp[k0 + t];
+            w[t] = wp[k0 + t];
+        }
+        BRaw<BT, VEC> raw[8];
+        if (m == 8) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int kk = k + u;
-                const int ks = kk & 63;
-                if ((starts >> ks) & 1ull) {  // wave-uniform: a new row starts here
-                    flush();
+            for (int t = 0; t < 8; ++t) raw[t] = gather(c[t], w[t]);
+        } else {
 #pragma unroll
-                    for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
-                    const int nxt = lane_bcast(rr, ks);
-                    fill(cur + 1, nxt);
-                    cur = nxt;
-                }
-                if (kk < cnt) {
-                    fma_raw<BT, VEC>(acc, lane_bcast(a0, ks), raw[u][0]);
-                    if (lane_bcast(off1, ks) >= 0) fma_raw<BT, VEC>(acc, lane_bcast(a1, ks), raw[u][1]);
-                }
-            }
-        };
-        Raw ra, rb;
-        issue_any(0, ra);
-        for (int k = 0;; k += 2 * U) {
-            if (k + U < cnt) issue_any(k + U, rb);
-            fold(k, ra);
-            if (k + U >= cnt) break;
-            if (k + 2 * U < cnt) issue_any(k + 2 * U, ra);
-            fold(k + U, rb);
-            if (k + 2 * U >= cnt) break;
+            for (int t = 0; t < 8; ++t)
+                if (t < m) raw[t] = gather(c[t], w[t]);
+        }
+        // values of this block that start a row (value 0's bit is clear)
+        uint32_t rem = (uint32_t)(starts >> k0) & 0xFFu;
+        if (m == 8 && rem == 0) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) fma_raw<BT, VEC>(acc, w[t], raw[t]);
+            continue;
+        }
+        for (int t0 = 0;;) {
+            // values [t0, t1) of the block belong to row `cur`
+            const int t1 = rem ? (int)__builtin_ctz(rem) : m;
+            const uint32_t seg = ((1u << t1) - 1u) & ~((1u << t0) - 1u);
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if ((seg >> t) & 1u) fma_raw<BT, VEC>(acc, w[t], raw[t]);
+            if (!rem) break;
+            // a new row starts at value k0 + t1
+            flush();
+#pragma unroll
+            for (int i = 0; i < VEC; ++i) acc[i] = 0.f;
+            const int nxt = lane_bcast(rr, k0 + t1);
+            fill(cur + 1, nxt);
+            cur = nxt;
+            t0 = t1;
+            rem &= rem - 1u;
         }
     }
     flush();
     fill(cur + 1, next_row);
 }
 
-// Work item `item` (wave-uniform), column tile `tile` of one packed matrix.
-template <int AT, int BT, int VEC, bool PIPE = false>
-__device__ __forceinline__ void spmm_unit(const SpmmArgs &a, int item, int tile) {
+// Work item `item` (wave-uniform), column tile `tile` of T of one packed matrix.
+template <int BT, int VEC>
+__device__ __forceinline__ void spmm_unit(const SpmmArgs &a, int item, int tile, int T) {
     const int beg = a.item_beg[item];
     const int end = a.item_beg[item + 1];
     // rows that keep nothing: item 0 fills those before its first row, every
     // item those between its rows and after its last, up to the next item's
     // first row (the last item: up to the row count)
     const int lead = item == 0 ? 0 : a.c_rows;
-    const int next_row = item + 1 < a.items ? a.ent_row[end] : a.c_rows;
-    spmm_item<AT, BT, VEC, PIPE>(a.ent_row, a.ent_col, a.ent_idx, a.ent_val, a.ent_blk, a.blk_scale,
-                                 a.b_scales, a.B, a.C, a.part, a.bias, a.dvb, a.V, beg, end - beg,
-                                 a.item_slot[item], lead, next_row, a.M, a.ldb, a.ldc, a.pn, a.b_rows, tile);
+    const int next_row = item + 1 < a.items ? a.vrow[end] : a.c_rows;
+    spmm_item<BT, VEC>(a.vrow, a.vcol, a.vw, a.b_scales, a.B, a.C, a.part, a.bias, a.dvb, a.V, beg, end - beg,
+                       a.item_slot[item], lead, next_row, a.ldb, a.ldc, a.pn, a.b_rows, tile, T);
 }
 
 // Fix-up of one listed row (a wave): the sum of its `ns` partial-sum slots
