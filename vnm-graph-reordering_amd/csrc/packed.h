@@ -280,9 +280,17 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ vrow, cons
     // the T column tiles interleave in 128-byte pieces (one L2 line of a B
     // row): tile t takes pieces t, t + T, t + 2T, ... of every row, so each
     // XCD's L2 holds its 1/T of B spread over the whole row
-    constexpr int LB = VEC * SB;  // bytes per lane
-    const int col0 = (((lane * LB) >> 7) * T + tile) * (128 / SB) + (((lane * LB) & 127) / SB);
-    const int voff = col0 * SB;
+    constexpr int LB = VEC * SB;  // bytes per lane and B row
+    auto tile_byte = [&](int b) { return ((b >> 7) * T + tile) * 128 + (b & 127); };
+    // 8 bytes per lane: two B rows per load instruction, 16 bytes per lane --
+    // lanes 0-31 read the tile of one row, lanes 32-63 that of the next, and
+    // two lane-half swaps (v_permlane32_swap) then give every lane both rows'
+    // 8 bytes for the columns it owns (lane i and i + 32 own the two halves
+    // of bytes [16i, 16i + 16))
+    constexpr bool PAIR = LB == 8;
+    const int upper = lane >> 5;
+    const int col0 = tile_byte(PAIR ? 16 * (lane & 31) + 8 * upper : lane * LB) / SB;
+    const int voff = PAIR ? tile_byte(16 * (lane & 31)) : col0 * SB;
     const float *bias_c = bias ? bias + col0 : nullptr;
     float acc[VEC];
     int cur = first_row;
@@ -335,6 +343,24 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ vrow, cons
         if constexpr (BT == VNM_DT_FP8) w *= bsc[c >> 7];
         return load_row<BT, VEC>(rs, voff, (uint32_t)c * (uint32_t)rowb);
     };
+    // the B rows of two values (PAIR), each lane's offset in full in the
+    // per-lane operand (in range, like every B row)
+    auto gather2 = [&](int c0, float &w0, int c1, float &w1, BRaw<BT, VEC> &r0, BRaw<BT, VEC> &r1) {
+        if constexpr (BT == VNM_DT_FP8) {
+            w0 *= bsc[c0 >> 7];
+            w1 *= bsc[c1 >> 7];
+        }
+        const uint32_t o0 = __builtin_amdgcn_readfirstlane((uint32_t)c0 * (uint32_t)rowb);
+        const uint32_t o1 = __builtin_amdgcn_readfirstlane((uint32_t)c1 * (uint32_t)rowb);
+        const uint32_t vo = (uint32_t)voff + (upper ? o1 : o0);
+        const auto t = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)vo, 0, 0);
+        const auto s0 = __builtin_amdgcn_permlane32_swap(t[0], t[2], false, false);
+        const auto s1 = __builtin_amdgcn_permlane32_swap(t[1], t[3], false, false);
+        r0.w[0] = s0[0];
+        r0.w[1] = s1[0];
+        r1.w[0] = s0[1];
+        r1.w[1] = s1[1];
+    };
     // blocks of 8 values (the item's last may hold fewer, m): their B rows and
     // weights by scalar loads, all gathers in flight, then the FMAs in value
     // order, one row segment of the block at a time (a block without a row
@@ -349,7 +375,20 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ vrow, cons
             w[t] = wp[k0 + t];
         }
         BRaw<BT, VEC> raw[8];
-        if (m == 8) {
+        if constexpr (PAIR) {
+            if (m == 8) {
+#pragma unroll
+                for (int t = 0; t < 8; t += 2) gather2(c[t], w[t], c[t + 1], w[t + 1], raw[t], raw[t + 1]);
+            } else {
+                float wd = 0.f;
+                BRaw<BT, VEC> rd;
+#pragma unroll
+                for (int t = 0; t < 8; t += 2) {
+                    if (t + 1 < m) gather2(c[t], w[t], c[t + 1], w[t + 1], raw[t], raw[t + 1]);
+                    else if (t < m) gather2(c[t], w[t], c[t], wd, raw[t], rd);  // the last, unpaired
+                }
+            }
+        } else if (m == 8) {
 #pragma unroll
             for (int t = 0; t < 8; ++t) raw[t] = gather(c[t], w[t]);
         } else {
