@@ -415,14 +415,17 @@ typedef struct {
 int vnm_lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window,
                    int maxiter, uint64_t seed, vnm_lp_stats *st);
 
-/* vnm_lp_reorder of count graphs of one context: `workers` (1..16) host
- * threads, each with its own stream, take the graphs in turn, so the small
- * kernels and host waits of separate runs overlap.  st[i] and graph i's
- * relabelling are those of vnm_lp_reorder on graph i alone with the same
- * seed.  No counterpart in the reference; for batches of graphs such as the
- * inputs of one GCN training set. */
+/* vnm_lp_reorder of count graphs of one context as one launch chain: one
+ * iteration of every graph is captured as a HIP graph of 8 parallel branches
+ * (every kernel reads its sizes and flags from its graph's device-side
+ * control block) and replayed until every run is over or maxiter iterations
+ * have run, with one host read of the runs' done flags per replay (and a new
+ * capture of the running graphs once at most half of the captured ones
+ * run).  st[i] and graph i's relabelling are those of vnm_lp_reorder on
+ * graph i alone with the same seed.  No counterpart in the reference; for
+ * batches of graphs such as the inputs of one GCN training set. */
 int vnm_lp_reorder_batch(vnm_graph *const *graphs, int count, const vnm_lp_model *m, int N, int mode,
-                         int topk, int window, int maxiter, uint64_t seed, int workers, vnm_lp_stats *st);
+                         int topk, int window, int maxiter, uint64_t seed, vnm_lp_stats *st);
 
 /* hybrid_reordering (main.cu:195-337) with vnm_lp_reorder (mode, topk,
  * window, maxiter) as its N:M step; VENOM steps and stats as vnm_hybrid

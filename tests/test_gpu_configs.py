@@ -242,7 +242,7 @@ def test_chunglu64_exact_batch_one_chain(ctx):
 
 
 def test_chunglu64_learned_batch_gcn_aggregation(ctx):
-    """All 64 graphs: learned reorder on worker streams (each graph's result
+    """All 64 graphs: learned reorder in one launch chain (each graph's result
     its own single run's, checked on a sample), V:N:M 64:2:8 BF16 packing,
     then Y_i = A_i X_i with X_i n_i x 256 BF16 in one captured launch: every
     product bit-identical to its own vnm_spmm, a sample against the oracle."""
@@ -250,7 +250,7 @@ def test_chunglu64_learned_batch_gcn_aggregation(ctx):
     coos = [_chunglu(i) for i in range(64)]
     model = vnm.LPModel.load(ctx, WEIGHTS)
     graphs = [vnm.Graph(ctx, m, 8) for m in coos]
-    sts = vnm.Graph.lp_reorder_batch(graphs, model, N, topk=4096, window=16, maxiter=40, workers=8)
+    sts = vnm.Graph.lp_reorder_batch(graphs, model, N, topk=4096, window=16, maxiter=40)
     for i in CHUNGLU_SAMPLE:
         g1 = vnm.Graph(ctx, coos[i], 8)
         s1 = g1.lp_reorder(model, N, topk=4096, window=16, maxiter=40)

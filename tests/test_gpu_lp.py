@@ -256,9 +256,10 @@ def test_trained_proposer_beats_random_and_untrained(ctx):
 
 @pytest.mark.parametrize("mode", [vnm.LP_LEARNED, vnm.LP_RANDOM, vnm.LP_EXHAUSTIVE])
 def test_lp_reorder_batch_matches_single_runs(ctx, mode):
-    """Uneven graphs reordered by the batched entry point on four worker
-    streams: stats and relabelled COO equal each graph's own lp_reorder with
-    the same seed, and a second batch continues from the reordered graphs."""
+    """Uneven graphs reordered by the batched entry point (one captured
+    iteration of all graphs, replayed): stats and relabelled COO equal each
+    graph's own lp_reorder with the same seed, and a second batch continues
+    from the reordered graphs."""
     model = vnm.LPModel.load(ctx, WEIGHTS) if mode == vnm.LP_LEARNED else None
     rng = np.random.default_rng(77)
     coos = [vnm.gen_chung_lu(int(n), 2.1, 12.0, 3000 + i, M=M)
@@ -269,7 +270,7 @@ def test_lp_reorder_batch_matches_single_runs(ctx, mode):
     batch = [vnm.Graph(ctx, m, M) for m in coos]
     single = [vnm.Graph(ctx, m, M) for m in coos]
     for rep in range(2):
-        got = vnm.Graph.lp_reorder_batch(batch, model, N, mode=mode, workers=4, **kw)
+        got = vnm.Graph.lp_reorder_batch(batch, model, N, mode=mode, **kw)
         for i, (gb, gs) in enumerate(zip(batch, single)):
             want = gs.lp_reorder(model, N, mode=mode, **kw)
             a, b = dict(got[i].as_dict()), dict(want.as_dict())
@@ -288,6 +289,6 @@ def test_lp_reorder_batch_rejects_bad_batches(ctx):
     with pytest.raises(vnm.VnmError):
         vnm.Graph.lp_reorder_batch([a, b], None, N, mode=vnm.LP_LEARNED)  # learned needs a model
     with pytest.raises(vnm.VnmError):
-        vnm.Graph.lp_reorder_batch([a, b], None, N, mode=vnm.LP_RANDOM, workers=0)
+        vnm.Graph.lp_reorder_batch([a, b], None, N, mode=vnm.LP_RANDOM, topk=0)
     st = vnm.Graph.lp_reorder_batch([a, b], None, N, mode=vnm.LP_RANDOM, maxiter=4, seed=3)
     assert st[0].final_cnt == st[1].final_cnt  # same graph, same seed

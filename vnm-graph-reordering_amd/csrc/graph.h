@@ -118,7 +118,7 @@ int venom_eval(vnm_graph *g, int V, int32_t *inv, int32_t *inv_strict);
 int lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window, int maxiter,
                uint64_t seed, vnm_lp_stats *stats);
 int lp_reorder_batch(vnm_graph *const *gs, int count, const vnm_lp_model *m, int N, int mode, int topk,
-                     int window, int maxiter, uint64_t seed, int workers, vnm_lp_stats *stats);
+                     int window, int maxiter, uint64_t seed, vnm_lp_stats *stats);
 int venom_reorder(vnm_graph *g, int V, int maxiter, vnm_venom_stats *st);
 int reload_sort(vnm_graph *g, bool quantize);
 int graph_clone(vnm_graph *src, vnm_graph **out);

@@ -169,7 +169,10 @@ __global__ __launch_bounds__(SS_T) void k_lp_seg_stats(
     int R, int M, int N, int nseg, const int32_t *__restrict__ segs, int nsel,
     const int32_t *__restrict__ inv, const int32_t *__restrict__ ptr, const int32_t *__restrict__ idx,
     int32_t *__restrict__ ccls, int32_t *__restrict__ segcls, int32_t *__restrict__ segnnz,
-    int32_t *__restrict__ queue) {
+    int32_t *__restrict__ queue, const int *__restrict__ d_nsel, const int *__restrict__ skip) {
+    // a device-side count and skip flag, when given, replace nsel
+    if (skip && *skip) return;
+    if (d_nsel) nsel = *d_nsel;
     __shared__ ColSpan cols[16];
     __shared__ int pre[17];
     __shared__ int sel[1], cur;
@@ -330,7 +333,8 @@ __global__ __launch_bounds__(SS_T) void k_lp_seg_stats(
 // tot[0] = sum over segments of rows with more than N ones, tot[1] = number
 // of segments with at least one such row.
 __global__ __launch_bounds__(1024) void k_lp_totals(int nseg, const int32_t *__restrict__ segcls,
-                                                     int32_t *__restrict__ tot) {
+                                                     int32_t *__restrict__ tot, const int *__restrict__ skip) {
+    if (skip && *skip) return;
     __shared__ int sc[16], ss[16];
     int c = 0, s = 0;
     for (int i = threadIdx.x; i < nseg; i += 1024) {
@@ -356,7 +360,9 @@ template <typename T>
 __global__ void k_lp_features(int R, int M, const int32_t *__restrict__ perm,
                               const int32_t *__restrict__ ptr, const int32_t *__restrict__ ccls,
                               const int32_t *__restrict__ segcls, const int32_t *__restrict__ segnnz,
-                              const int32_t *__restrict__ tot, T *__restrict__ X) {
+                              const int32_t *__restrict__ tot, T *__restrict__ X, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // not wanted this iteration (a device-side flag)
+
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= R) return;
     const int p = perm[u] / M;
@@ -442,7 +448,9 @@ template <typename TI, typename TO, int F, bool BWD>
 __global__ __launch_bounds__(256) void k_lp_agg_nodes(int R, const int32_t *__restrict__ ptr,
                                                       const int32_t *__restrict__ idx,
                                                       const float *__restrict__ inv_deg,
-                                                      const TI *__restrict__ H, TO *__restrict__ out) {
+                                                      const TI *__restrict__ H, TO *__restrict__ out, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // not wanted this iteration (a device-side flag)
+
     const int u = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (u >= R) return;
     const int b = ptr[u], e = ptr[u + 1];
@@ -469,7 +477,9 @@ __global__ __launch_bounds__(256) void k_lp_agg_chunks(int nch, const int32_t *_
                                                        const int32_t *__restrict__ ch_end,
                                                        const int32_t *__restrict__ idx,
                                                        const float *__restrict__ inv_deg,
-                                                       const TI *__restrict__ H, float *__restrict__ part) {
+                                                       const TI *__restrict__ H, float *__restrict__ part, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // not wanted this iteration (a device-side flag)
+
     const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (c >= nch) return;
     const float4 acc = agg_range<TI, F, BWD>(ch_beg[c], ch_end[c], idx, inv_deg, H);
@@ -480,7 +490,9 @@ __global__ __launch_bounds__(256) void k_lp_agg_chunks(int nch, const int32_t *_
 template <typename TO, int F, bool BWD>
 __global__ void k_lp_agg_combine(int nheavy, const int32_t *__restrict__ heavy,
                                  const int32_t *__restrict__ ch0, const float *__restrict__ part,
-                                 const float *__restrict__ inv_deg, TO *__restrict__ out) {
+                                 const float *__restrict__ inv_deg, TO *__restrict__ out, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // not wanted this iteration (a device-side flag)
+
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nheavy * F) return;
     const int h = t / F, f = t % F;
@@ -507,7 +519,9 @@ __global__ __launch_bounds__(256) void k_lp_dense_mfma(int R, const uint16_t *__
                                                        const uint16_t *__restrict__ B,
                                                        const uint16_t *__restrict__ Wt,
                                                        const float *__restrict__ bias,
-                                                       TO *__restrict__ out) {
+                                                       TO *__restrict__ out, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // not wanted this iteration (a device-side flag)
+
     constexpr int K = KA + KB, KP = K + 4;  // padded LDS row (bank spread)
     __shared__ uint16_t w[64 * KP];
     __shared__ float bs[64];
@@ -560,7 +574,9 @@ __global__ __launch_bounds__(256) void k_lp_dense_f32(int R, const float *__rest
                                                       const float *__restrict__ B,
                                                       const float *__restrict__ W,
                                                       const float *__restrict__ bias,
-                                                      float *__restrict__ out) {
+                                                      float *__restrict__ out, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // not wanted this iteration (a device-side flag)
+
     constexpr int K = KA + KB;
     __shared__ float w[K * 64];
     for (int i = threadIdx.x; i < K * 64; i += 256) w[i] = W[i];
@@ -733,19 +749,116 @@ __global__ void k_lp_wt(const float *__restrict__ w, uint16_t *__restrict__ wt) 
 
 // ------------------------------------------------------------ candidates
 
+// Device-side control of one learned run.  The host enqueues iterations
+// without reading anything back; the control kernels take the decisions of
+// the iteration loop (stop, fall back to the exhaustive window, sizes), and
+// every kernel of an iteration reads its counts and its skip flag here.
+struct LPCtl {
+    int done;       // the run has ended (converged, no donors, nothing to apply)
+    int exh;        // this iteration scores every window entry exactly
+    int skip_fwd;   // no forward pass this iteration
+    int skip_norm;  // no sort of the one-per-donor candidates (done || exh)
+    int skip_exh;   // no sort of the window candidates (done || !exh)
+    int skip_swap;  // no swaps to apply (done || none accepted)
+    int it;         // iterations run (the proposals' seeds follow it)
+    int ndonor;     // donors counted by k_lp_pick
+    int nd, nc, stride;
+    int nacc, gain;  // accepted swaps and their summed gain (k_lp_select)
+    int nsel;        // segments the accepted swaps touched
+    int fallbacks, swaps;
+    long long gain_total, exact_pairs;
+};
+
+// A run starts at iteration 0, in the exhaustive mode for mode 2, and is
+// over at once when no segment is invalid.
+__global__ void k_lp_ctl_init(LPCtl *ctl, const int32_t *__restrict__ tot, int mode) {
+    if (threadIdx.x != 0) return;
+    LPCtl z{};
+    z.exh = mode == 2;
+    z.done = tot[0] <= 0;
+    *ctl = z;
+}
+
+__global__ void k_lp_ctl_begin(LPCtl *ctl, int mode) {
+    if (threadIdx.x != 0) return;
+    ctl->ndonor = 0;
+    ctl->skip_fwd = ctl->done || ctl->exh || mode != 0;
+}
+
+// After the donors are counted: this iteration's sizes, or the end of the run
+// when there is no donor.
+__global__ void k_lp_ctl_mid(LPCtl *ctl, int topk, int W, int nseg) {
+    if (threadIdx.x != 0) return;
+    LPCtl c = *ctl;
+    c.nd = min(topk, c.ndonor);
+    if (!c.done && c.nd == 0) c.done = 1;
+    if (c.done) {
+        c.nc = 0;
+        c.skip_norm = c.skip_exh = c.skip_swap = 1;
+    } else {
+        c.stride = max(1, nseg / c.nd);
+        c.nc = c.exh ? c.nd * W : c.nd;
+        c.exact_pairs += c.nc;
+        c.skip_norm = c.exh;
+        c.skip_exh = !c.exh;
+    }
+    *ctl = c;
+}
+
+// After the selection: nothing accepted falls back to the exhaustive window
+// once (mode 0) or ends the run; otherwise the swaps are applied and the
+// next iteration proposes again.
+__global__ void k_lp_ctl_after(LPCtl *ctl, int mode, int32_t *__restrict__ queue) {
+    if (threadIdx.x != 0) return;
+    LPCtl c = *ctl;
+    if (c.done) return;
+    c.it++;
+    if (c.nacc == 0) {
+        c.skip_swap = 1;
+        c.nsel = 0;
+        if (mode == 0 && !c.exh) {
+            c.exh = 1;
+            c.fallbacks++;
+        } else {
+            c.done = 1;
+        }
+    } else {
+        c.skip_swap = 0;
+        c.nsel = 2 * c.nacc;
+        c.swaps += c.nacc;
+        c.gain_total += c.gain;
+        if (mode == 0) c.exh = 0;
+        *queue = 0;  // k_lp_seg_stats' work counter
+    }
+    *ctl = c;
+}
+
+// After the touched segments are recounted: over when nothing is invalid.
+__global__ void k_lp_ctl_end(LPCtl *ctl, const int32_t *__restrict__ tot) {
+    if (threadIdx.x == 0 && !ctl->done && !ctl->skip_swap && tot[0] <= 0) ctl->done = 1;
+}
+
+// The done flags of a batch's runs, in one array the host reads.
+__global__ void k_lp_done_flags(LPCtl *const *__restrict__ ctls, int G, int32_t *__restrict__ done) {
+    for (int i = threadIdx.x; i < G; i += blockDim.x) done[i] = ctls[i]->done;
+}
+
 // Per segment: the donor (a node of an invalid segment that sits in a row
 // with N+1 ones, best donor score, ties to the lower bit) and the acceptor
 // (best acceptor score).  mode 0: network scores Y; 1: seeded hash scores;
 // 2: counts (donor = its rows with N+1 ones; acceptor = minus its degree and
 // its segment's rows with at least N ones).  Keys sort ascending = best first.
-__global__ void k_lp_pick(int nseg, int M, int mode, uint64_t seed, const int32_t *__restrict__ inv,
+__global__ void k_lp_pick(LPCtl *ctl, int nseg, int M, int mode0, uint64_t seed0, const int32_t *__restrict__ inv,
                           const int32_t *__restrict__ ptr, const int32_t *__restrict__ ccls,
                           const int32_t *__restrict__ segcls, const float *__restrict__ Y,
                           uint64_t *__restrict__ dkey, uint64_t *__restrict__ akey,
-                          int8_t *__restrict__ dbest, int8_t *__restrict__ abest,
-                          int32_t *__restrict__ ndonor, int sbits) {
+                          int8_t *__restrict__ dbest, int8_t *__restrict__ abest, int sbits) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nseg) return;
+    if (s >= nseg || ctl->done) return;
+    // the exhaustive window ranks by counts; the seed follows the iteration
+    const int mode = ctl->exh ? 2 : mode0;
+    const uint64_t seed = seed0 + (uint64_t)ctl->it * 0x9E3779B97F4A7C15ull;
+    int32_t *ndonor = &ctl->ndonor;
     const int4 S = *(const int4 *)&segcls[s * 4];
     const bool invalid = S.z + S.w > 0;
     float bd = 0.f, ba = 0.f;
@@ -785,13 +898,15 @@ struct Cand { int p, q, bu, bv; };
 // nseg, w < W.  all == 0: one candidate per donor, the window entry with the
 // best pair score (network: Y0[u] + Y1[v] + P[u].Q[v]; hash in mode 1; the
 // first in mode 2); all == 1: every window entry is a candidate.
-__global__ void k_lp_match(int nd, int W, int nseg, int stride, int M, int mode, int all, uint64_t seed,
+__global__ void k_lp_match(const LPCtl *__restrict__ ctl, int W, int nseg, int M, int mode0, uint64_t seed0,
                            const uint64_t *__restrict__ sdkey, const uint64_t *__restrict__ sakey,
                            const int8_t *__restrict__ dbest, const int8_t *__restrict__ abest,
                            const int32_t *__restrict__ inv, const float *__restrict__ Y,
                            Cand *__restrict__ cand, uint64_t smask) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nd) return;
+    if (ctl->done || i >= ctl->nd) return;
+    const int stride = ctl->stride, all = ctl->exh, mode = ctl->exh ? 2 : mode0;
+    const uint64_t seed = seed0 ^ ((uint64_t)ctl->it << 20);
     const int p = (int)(sdkey[i] & smask);
     const int bu = dbest[p];
     const int u = inv[p * M + bu];
@@ -844,7 +959,8 @@ __global__ void k_lp_match(int nd, int W, int nseg, int stride, int M, int mode,
 // Writes the best pair (max lgain + rgain, then max rgain, then the lower
 // index b1*M+b2) and, when full != nullptr, the whole matrix in the layout
 // of vnm_swap_score.
-__global__ __launch_bounds__(256) void k_lp_pair_gain(int R, int M, int N, const Cand *__restrict__ cand,
+__global__ __launch_bounds__(256) void k_lp_pair_gain(const int *__restrict__ d_nc, int R, int M, int N,
+                                                      const Cand *__restrict__ cand,
                                                       const int32_t *__restrict__ inv,
                                                       const int32_t *__restrict__ ptr,
                                                       const int32_t *__restrict__ idx,
@@ -863,6 +979,7 @@ __global__ __launch_bounds__(256) void k_lp_pair_gain(int R, int M, int N, const
         uint32_t tile[PAIR_TILE_ROWS / 2];
     } L;
     const int i = blockIdx.x;
+    if (d_nc && i >= *d_nc) return;  // past this iteration's candidates (device count)
     const Cand c = cand[i];
     const int tid = threadIdx.x, lane = tid & 63;
     const int MM = M * M;
@@ -1039,13 +1156,20 @@ __global__ __launch_bounds__(256) void k_lp_pair_gain(int R, int M, int N, const
 // largest candidate index, gmax = 2^gbits - 1 at least the largest gain (a
 // gain counts rows of the pair's two segments, each row at most once per
 // segment, so it is at most twice the row count)
-__global__ void k_lp_gain_keys(int nc, const Cand *__restrict__ cand, const int4 *__restrict__ best,
-                               uint64_t *__restrict__ keys, int ibits, uint32_t gmax) {
+// Keys for the first nkeys slots; slots past this iteration's candidates get
+// the sentinel ~0 (it sorts last), so the sort runs at a size fixed ahead.
+__global__ void k_lp_gain_keys(const LPCtl *__restrict__ ctl, int nkeys, const Cand *__restrict__ cand,
+                               const int4 *__restrict__ best, uint64_t *__restrict__ keys, int ibits,
+                               uint32_t gmax) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nc) return;
-    const int4 b = best[i];
-    const int gain = b.z + b.w;
-    keys[i] = (cand[i].q >= 0 && gain > 0) ? ((uint64_t)(gmax - (uint32_t)gain) << ibits) | (uint32_t)i : ~0ull;
+    if (i >= nkeys || ctl->done) return;
+    uint64_t key = ~0ull;
+    if (i < ctl->nc) {
+        const int4 b = best[i];
+        const int gain = b.z + b.w;
+        if (cand[i].q >= 0 && gain > 0) key = ((uint64_t)(gmax - (uint32_t)gain) << ibits) | (uint32_t)i;
+    }
+    keys[i] = key;
 }
 
 constexpr int SEL_T = 1024, SEL_B = 1024;
@@ -1063,10 +1187,12 @@ constexpr int32_t OWNER_NONE = 0x7F7F7F7F;    // empty owner (bytes 0x7F, set by
 // finishes the chunk sequentially.  Chunks of SEL_B candidates in order.
 // out[0] = accepted, out[1] = their total gain; acc = candidate indices in
 // order.  owner: nseg ints, OWNER_NONE on entry and on exit.
-__global__ __launch_bounds__(SEL_T) void k_lp_select(int nc, int nseg, const uint64_t *__restrict__ skeys,
+__global__ __launch_bounds__(SEL_T) void k_lp_select(LPCtl *ctl, int nseg, const uint64_t *__restrict__ skeys,
                                                      const Cand *__restrict__ cand,
                                                      const int4 *__restrict__ best, int32_t *__restrict__ acc,
-                                                     int32_t *__restrict__ out, int32_t *owner, uint64_t imask) {
+                                                     int32_t *owner, uint64_t imask) {
+    if (ctl->done) return;
+    const int nc = ctl->nc;
     extern __shared__ uint32_t used[];
     __shared__ int bp[SEL_B], bq[SEL_B], bg[SEL_B], bc[SEL_B];
     __shared__ uint8_t live[SEL_B], took[SEL_B];
@@ -1163,16 +1289,19 @@ __global__ __launch_bounds__(SEL_T) void k_lp_select(int nc, int nseg, const uin
         }
         __syncthreads();
     }
-    if (tid == 0) { out[0] = nacc; out[1] = gsum; }
+    if (tid == 0) {
+        ctl->nacc = nacc;
+        ctl->gain = gsum;
+    }
 }
 
 // Apply the accepted swaps (segment-disjoint, so in parallel) and list the
 // touched segments.
-__global__ void k_lp_swap(int nsw, int M, const int32_t *__restrict__ acc, const Cand *__restrict__ cand,
-                          const int4 *__restrict__ best, int32_t *__restrict__ perm,
+__global__ void k_lp_swap(const LPCtl *__restrict__ ctl, int M, const int32_t *__restrict__ acc,
+                          const Cand *__restrict__ cand, const int4 *__restrict__ best, int32_t *__restrict__ perm,
                           int32_t *__restrict__ inv, int32_t *__restrict__ touched) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nsw) return;
+    if (ctl->skip_swap || t >= ctl->nacc) return;
     const int ci = acc[t];
     const Cand c = cand[ci];
     const int4 b = best[ci];
@@ -1324,8 +1453,9 @@ static int lp_stats(vnm_graph *g, LPCache *c, int N, const int32_t *segs, int ns
     if (!g->queue.p && (rc = g->queue.alloc(4))) return rc;
     VNM_HIP_TRY(hipMemsetAsync(g->queue.p, 0, 4, st));
     hipLaunchKernelGGL(k_lp_seg_stats, dim3(grid), dim3(SS_T), 0, st, g->nrows, g->M, N, c->nseg, segs, nsel,
-                       g->inv.p, c->ptr.p, c->idx.p, c->ccls.p, c->segcls.p, c->segnnz.p, g->queue.p);
-    hipLaunchKernelGGL(k_lp_totals, dim3(1), dim3(1024), 0, st, c->nseg, c->segcls.p, c->tot.p);
+                       g->inv.p, c->ptr.p, c->idx.p, c->ccls.p, c->segcls.p, c->segnnz.p, g->queue.p, nullptr,
+                       nullptr);
+    hipLaunchKernelGGL(k_lp_totals, dim3(1), dim3(1024), 0, st, c->nseg, c->segcls.p, c->tot.p, nullptr);
     g->ctx->launches += 2;
     VNM_LAUNCH_CHECK("k_lp_seg_stats");
     return VNM_OK;
@@ -1357,7 +1487,7 @@ static int lp_prepare_bwd(vnm_graph *g, LPCache *c) {
 }
 
 template <typename TI, typename TO, int F, bool BWD>
-static int lp_aggregate(vnm_graph *g, LPCache *c, const TI *H, TO *out) {
+static int lp_aggregate(vnm_graph *g, LPCache *c, const TI *H, TO *out, const int *skip = nullptr) {
     hipStream_t st = g->ctx->stream;
     const int R = c->R;
     if (BWD) {
@@ -1368,13 +1498,13 @@ static int lp_aggregate(vnm_graph *g, LPCache *c, const TI *H, TO *out) {
     const int32_t *ptr = BWD ? c->tptr.p : c->ptr.p;
     const int32_t *idx = BWD ? c->tidx.p : c->idx.p;
     hipLaunchKernelGGL((k_lp_agg_nodes<TI, TO, F, BWD>), dim3((R + 3) / 4), dim3(256), 0, st, R, ptr, idx,
-                       c->inv_deg.p, H, out);
+                       c->inv_deg.p, H, out, skip);
     g->ctx->launches++;
     if (P.nch) {
         hipLaunchKernelGGL((k_lp_agg_chunks<TI, F, BWD>), dim3((P.nch + 3) / 4), dim3(256), 0, st, P.nch,
-                           P.beg.p, P.end.p, idx, c->inv_deg.p, H, P.part.p);
+                           P.beg.p, P.end.p, idx, c->inv_deg.p, H, P.part.p, skip);
         hipLaunchKernelGGL((k_lp_agg_combine<TO, F, BWD>), dim3(grid_of((int64_t)P.nheavy * F, 256)),
-                           dim3(256), 0, st, P.nheavy, P.heavy.p, P.ch0.p, P.part.p, c->inv_deg.p, out);
+                           dim3(256), 0, st, P.nheavy, P.heavy.p, P.ch0.p, P.part.p, c->inv_deg.p, out, skip);
         g->ctx->launches += 2;
     }
     VNM_LAUNCH_CHECK("k_lp_agg");
@@ -1382,7 +1512,8 @@ static int lp_aggregate(vnm_graph *g, LPCache *c, const TI *H, TO *out) {
 }
 
 // Features, three layers and the head at the current perm/inv; Y is FP32.
-static int lp_forward(vnm_graph *g, LPCache *c, const vnm_lp_model *m, int bf16) {
+// skip: a device flag; while it is nonzero every kernel returns at once.
+static int lp_forward(vnm_graph *g, LPCache *c, const vnm_lp_model *m, int bf16, const int *skip = nullptr) {
     hipStream_t st = g->ctx->stream;
     const int R = c->R;
     int rc;
@@ -1392,36 +1523,36 @@ static int lp_forward(vnm_graph *g, LPCache *c, const vnm_lp_model *m, int bf16)
         T *X = (T *)c->X.p, *A0 = (T *)c->A0.p, *H1 = (T *)c->H1.p, *A1 = (T *)c->A1.p;
         T *H2 = (T *)c->H2.p, *A2 = (T *)c->A2.p, *H3 = (T *)c->H3.p;
         hipLaunchKernelGGL(k_lp_features<T>, dim3(grid_of(R, 256)), dim3(256), 0, st, R, g->M, g->perm.p,
-                           c->ptr.p, c->ccls.p, c->segcls.p, c->segnnz.p, c->tot.p, X);
-        if ((rc = lp_aggregate<T, T, 16, false>(g, c, X, A0))) return rc;
+                           c->ptr.p, c->ccls.p, c->segcls.p, c->segnnz.p, c->tot.p, X, skip);
+        if ((rc = lp_aggregate<T, T, 16, false>(g, c, X, A0, skip))) return rc;
         const int gr = (R + 63) / 64;
         hipLaunchKernelGGL((k_lp_dense_mfma<16, 16, true, T>), dim3(gr), dim3(256), 0, st, R, X, A0,
-                           m->wt.p + LP_WT_OFF[0], m->w.p + LP_OFF_B[0], H1);
-        if ((rc = lp_aggregate<T, T, 64, false>(g, c, H1, A1))) return rc;
+                           m->wt.p + LP_WT_OFF[0], m->w.p + LP_OFF_B[0], H1, skip);
+        if ((rc = lp_aggregate<T, T, 64, false>(g, c, H1, A1, skip))) return rc;
         hipLaunchKernelGGL((k_lp_dense_mfma<64, 64, true, T>), dim3(gr), dim3(256), 0, st, R, H1, A1,
-                           m->wt.p + LP_WT_OFF[1], m->w.p + LP_OFF_B[1], H2);
-        if ((rc = lp_aggregate<T, T, 64, false>(g, c, H2, A2))) return rc;
+                           m->wt.p + LP_WT_OFF[1], m->w.p + LP_OFF_B[1], H2, skip);
+        if ((rc = lp_aggregate<T, T, 64, false>(g, c, H2, A2, skip))) return rc;
         hipLaunchKernelGGL((k_lp_dense_mfma<64, 64, true, T>), dim3(gr), dim3(256), 0, st, R, H2, A2,
-                           m->wt.p + LP_WT_OFF[2], m->w.p + LP_OFF_B[2], H3);
+                           m->wt.p + LP_WT_OFF[2], m->w.p + LP_OFF_B[2], H3, skip);
         hipLaunchKernelGGL((k_lp_dense_mfma<64, 0, false, float>), dim3(gr), dim3(256), 0, st, R, H3,
-                           (const T *)nullptr, m->wt.p + LP_WT_OFF[3], m->w.p + LP_OFF_BH, c->Y.p);
+                           (const T *)nullptr, m->wt.p + LP_WT_OFF[3], m->w.p + LP_OFF_BH, c->Y.p, skip);
     } else {
         float *X = c->X.p, *A0 = c->A0.p, *H1 = c->H1.p, *A1 = c->A1.p, *H2 = c->H2.p, *A2 = c->A2.p,
               *H3 = c->H3.p;
         hipLaunchKernelGGL(k_lp_features<float>, dim3(grid_of(R, 256)), dim3(256), 0, st, R, g->M,
-                           g->perm.p, c->ptr.p, c->ccls.p, c->segcls.p, c->segnnz.p, c->tot.p, X);
+                           g->perm.p, c->ptr.p, c->ccls.p, c->segcls.p, c->segnnz.p, c->tot.p, X, skip);
         const int gr = grid_of(R, 4, 4096);
-        if ((rc = lp_aggregate<float, float, 16, false>(g, c, X, A0))) return rc;
+        if ((rc = lp_aggregate<float, float, 16, false>(g, c, X, A0, skip))) return rc;
         hipLaunchKernelGGL((k_lp_dense_f32<16, 16, true>), dim3(gr), dim3(256), 0, st, R, X, A0,
-                           m->w.p + LP_OFF_W[0], m->w.p + LP_OFF_B[0], H1);
-        if ((rc = lp_aggregate<float, float, 64, false>(g, c, H1, A1))) return rc;
+                           m->w.p + LP_OFF_W[0], m->w.p + LP_OFF_B[0], H1, skip);
+        if ((rc = lp_aggregate<float, float, 64, false>(g, c, H1, A1, skip))) return rc;
         hipLaunchKernelGGL((k_lp_dense_f32<64, 64, true>), dim3(gr), dim3(256), 0, st, R, H1, A1,
-                           m->w.p + LP_OFF_W[1], m->w.p + LP_OFF_B[1], H2);
-        if ((rc = lp_aggregate<float, float, 64, false>(g, c, H2, A2))) return rc;
+                           m->w.p + LP_OFF_W[1], m->w.p + LP_OFF_B[1], H2, skip);
+        if ((rc = lp_aggregate<float, float, 64, false>(g, c, H2, A2, skip))) return rc;
         hipLaunchKernelGGL((k_lp_dense_f32<64, 64, true>), dim3(gr), dim3(256), 0, st, R, H2, A2,
-                           m->w.p + LP_OFF_W[2], m->w.p + LP_OFF_B[2], H3);
+                           m->w.p + LP_OFF_W[2], m->w.p + LP_OFF_B[2], H3, skip);
         hipLaunchKernelGGL((k_lp_dense_f32<64, 0, false>), dim3(gr), dim3(256), 0, st, R, H3,
-                           (const float *)nullptr, m->w.p + LP_OFF_WH, m->w.p + LP_OFF_BH, c->Y.p);
+                           (const float *)nullptr, m->w.p + LP_OFF_WH, m->w.p + LP_OFF_BH, c->Y.p, skip);
     }
     kt.stop();
     g->ctx->launches += 6;
@@ -1554,10 +1685,12 @@ struct LPRound {
 };
 
 // Score candidates [0, nc) exactly: a workgroup per candidate.
-static int lp_score_cands(vnm_graph *g, LPCache *c, int N, Cand *cand, int4 *best, int nc, int32_t *full) {
+// nc candidates, or (d_nc given) the first *d_nc of nc launched blocks.
+static int lp_score_cands(vnm_graph *g, LPCache *c, int N, Cand *cand, int4 *best, int nc, int32_t *full,
+                          const int *d_nc = nullptr) {
     if (nc <= 0) return VNM_OK;
     KernelTimer kt(g->ctx, "k_lp_pair_gain");
-    hipLaunchKernelGGL(k_lp_pair_gain, dim3(nc), dim3(256), 0, g->ctx->stream, g->nrows, g->M, N, cand,
+    hipLaunchKernelGGL(k_lp_pair_gain, dim3(nc), dim3(256), 0, g->ctx->stream, d_nc, g->nrows, g->M, N, cand,
                        g->inv.p, c->ptr.p, c->idx.p, best, full);
     kt.stop();
     g->ctx->launches++;
@@ -1574,104 +1707,165 @@ static int lp_reorder_args(vnm_graph *g, const vnm_lp_model *m, int mode, int to
     return VNM_OK;
 }
 
-// The reorder itself, on the stream of g->ctx (arguments checked).
-static int lp_reorder_run(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window,
-                          int maxiter, uint64_t seed, vnm_lp_stats *stats) {
-    auto t0 = std::chrono::steady_clock::now();
-    LPCache *c;
-    int rc = lp_begin(g, N, &c);
+// One learned run, split so that its iterations can be enqueued without the
+// host reading anything back: begin (cache, buffers, control block), any
+// number of iterations, end (statistics, relabel).
+struct LPRun {
+    vnm_graph *g = nullptr;
+    const vnm_lp_model *m = nullptr;
+    LPCache *c = nullptr;
+    int N = 0, mode = 0, topk = 0, window = 0, ns = 0, M = 0;
+    int nnorm = 0;  // keys sorted in an iteration with one candidate per donor
+    int ncap = 0;   // ... and with the whole window (the most candidates)
+    int sbits = 0, ibits = 0, gbits = 0;
+    int init_cnt = 0, init_size = 0;
+    uint32_t gmax = 0;
+    uint64_t smask = 0, imask = 0, seed = 0;
+    LPRound w;
+    DBuf<LPCtl> ctl;
+    std::chrono::steady_clock::time_point t0;
+};
+
+static int lp_run_begin(LPRun &r, vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window,
+                        uint64_t seed) {
+    r.t0 = std::chrono::steady_clock::now();
+    r.g = g;
+    r.m = m;
+    r.N = N;
+    r.mode = mode;
+    r.topk = topk;
+    r.window = window;
+    r.seed = seed;
+    int rc = lp_begin(g, N, &r.c);
     if (rc) return rc;
     hipStream_t st = g->ctx->stream;
-    const int ns = c->nseg, M = g->M;
-    const int ncmax = std::max(topk * window, topk);
+    const int ns = r.ns = r.c->nseg;
+    r.M = g->M;
+    r.nnorm = std::max(1, std::min(topk, ns));
+    r.ncap = mode == 1 ? r.nnorm : r.nnorm * window;
     // sort keys hold a segment or candidate index in their low bits
     auto bits_of = [](int64_t x) { int b = 1; while ((1ll << b) <= x) ++b; return b; };
-    const int sbits = bits_of(ns - 1), ibits = bits_of(ncmax - 1), gbits = bits_of(2 * (int64_t)g->nrows);
-    const uint32_t gmax = (uint32_t)((1ull << gbits) - 1);
-    const uint64_t smask = (1ull << sbits) - 1, imask = (1ull << ibits) - 1;
-    LPRound w;
+    r.sbits = bits_of(ns - 1);
+    r.ibits = bits_of(r.ncap - 1);
+    r.gbits = bits_of(2 * (int64_t)g->nrows);
+    r.gmax = (uint32_t)((1ull << r.gbits) - 1);
+    r.smask = (1ull << r.sbits) - 1;
+    r.imask = (1ull << r.ibits) - 1;
+    LPRound &w = r.w;
     if ((rc = w.dkey.alloc(ns)) || (rc = w.akey.alloc(ns)) || (rc = w.sdkey.alloc(ns)) ||
         (rc = w.sakey.alloc(ns)) || (rc = w.dbest.alloc(ns)) || (rc = w.abest.alloc(ns)) ||
-        (rc = w.cand.alloc(ncmax)) || (rc = w.best.alloc(ncmax)) || (rc = w.gkey.alloc(ncmax)) ||
-        (rc = w.sgkey.alloc(ncmax)) || (rc = w.acc.alloc(ncmax)) || (rc = w.out.alloc(8)) ||
-        (rc = w.touched.alloc(2 * (size_t)ncmax)) || (rc = w.owner.alloc(ns)))
+        (rc = w.cand.alloc(r.ncap)) || (rc = w.best.alloc(r.ncap)) || (rc = w.gkey.alloc(r.ncap)) ||
+        (rc = w.sgkey.alloc(r.ncap)) || (rc = w.acc.alloc(r.ncap)) ||
+        (rc = w.touched.alloc(2 * (size_t)std::min(r.ncap, ns))) || (rc = w.owner.alloc(ns)) ||
+        (rc = r.ctl.alloc(1)))
         return rc;
+    if (!g->queue.p && (rc = g->queue.alloc(4))) return rc;
+    // the sorts of an iteration allocate nothing (they may run under capture)
+    if ((rc = sort_scratch_reserve(std::max(ns, r.ncap)))) return rc;
     VNM_HIP_TRY(hipMemsetAsync(w.owner.p, 0x7F, (size_t)ns * 4, st));  // OWNER_NONE in every slot
     int32_t tot[2];
-    VNM_HIP_TRY(hipMemcpyAsync(tot, c->tot.p, 8, hipMemcpyDeviceToHost, st));
+    VNM_HIP_TRY(hipMemcpyAsync(tot, r.c->tot.p, 8, hipMemcpyDeviceToHost, st));
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    r.init_cnt = tot[0];
+    r.init_size = tot[1];
+    hipLaunchKernelGGL(k_lp_ctl_init, dim3(1), dim3(64), 0, st, r.ctl.p, r.c->tot.p, mode);
+    g->ctx->launches++;
+    VNM_LAUNCH_CHECK("k_lp_ctl_init");
+    return VNM_OK;
+}
+
+// Enqueue one iteration; every kernel reads its sizes and flags from the
+// control block, so nothing here waits for the device.
+static int lp_run_iter(LPRun &r) {
+    vnm_graph *g = r.g;
+    LPCache *c = r.c;
+    LPRound &w = r.w;
+    LPCtl *ctl = r.ctl.p;
+    hipStream_t st = g->ctx->stream;
+    const int ns = r.ns, M = r.M;
+    int rc;
+    hipLaunchKernelGGL(k_lp_ctl_begin, dim3(1), dim3(64), 0, st, ctl, r.mode);
+    if (r.mode == 0 && (rc = lp_forward(g, c, r.m, 1, &ctl->skip_fwd))) return rc;
+    hipLaunchKernelGGL(k_lp_pick, dim3(grid_of(ns, 256)), dim3(256), 0, st, ctl, ns, M, r.mode, r.seed, g->inv.p,
+                       c->ptr.p, c->ccls.p, c->segcls.p, c->Y.p, w.dkey.p, w.akey.p, w.dbest.p, w.abest.p, r.sbits);
+    VNM_LAUNCH_CHECK("k_lp_pick");
+    if ((rc = sort_keys_u64_if(w.dkey.p, w.sdkey.p, ns, 32 + r.sbits, st, &ctl->done))) return rc;
+    if ((rc = sort_keys_u64_if(w.akey.p, w.sakey.p, ns, 32 + r.sbits, st, &ctl->done))) return rc;
+    hipLaunchKernelGGL(k_lp_ctl_mid, dim3(1), dim3(64), 0, st, ctl, r.topk, r.window, ns);
+    hipLaunchKernelGGL(k_lp_match, dim3(grid_of(r.nnorm, 256)), dim3(256), 0, st, ctl, r.window, ns, M, r.mode,
+                       r.seed, w.sdkey.p, w.sakey.p, w.dbest.p, w.abest.p, g->inv.p, c->Y.p, w.cand.p, r.smask);
+    VNM_LAUNCH_CHECK("k_lp_match");
+    if ((rc = lp_score_cands(g, c, r.N, w.cand.p, w.best.p, r.ncap, nullptr, &ctl->nc))) return rc;
+    hipLaunchKernelGGL(k_lp_gain_keys, dim3(grid_of(r.ncap, 256)), dim3(256), 0, st, ctl, r.ncap, w.cand.p,
+                       w.best.p, w.gkey.p, r.ibits, r.gmax);
+    // one candidate per donor: the first nnorm keys; the whole window: ncap
+    const int kbits = r.gbits + r.ibits;
+    if (r.mode != 2 && (rc = sort_keys_u64_if(w.gkey.p, w.sgkey.p, r.nnorm, kbits, st, &ctl->skip_norm))) return rc;
+    if (r.mode != 1 && (rc = sort_keys_u64_if(w.gkey.p, w.sgkey.p, r.ncap, kbits, st, &ctl->skip_exh))) return rc;
+    const size_t shm = (size_t)((ns + 31) / 32) * 4;
+    hipLaunchKernelGGL(k_lp_select, dim3(1), dim3(SEL_T), shm, st, ctl, ns, w.sgkey.p, w.cand.p, w.best.p,
+                       w.acc.p, w.owner.p, r.imask);
+    VNM_LAUNCH_CHECK("k_lp_select");
+    hipLaunchKernelGGL(k_lp_ctl_after, dim3(1), dim3(64), 0, st, ctl, r.mode, g->queue.p);
+    const int maxsw = std::min(r.ncap, ns);
+    hipLaunchKernelGGL(k_lp_swap, dim3(grid_of(maxsw, 256)), dim3(256), 0, st, ctl, M, w.acc.p, w.cand.p,
+                       w.best.p, g->perm.p, g->inv.p, w.touched.p);
+    hipLaunchKernelGGL(k_lp_seg_stats, dim3(std::max(1, std::min(2 * maxsw, 1024))), dim3(SS_T), 0, st, g->nrows,
+                       M, r.N, ns, w.touched.p, 0, g->inv.p, c->ptr.p, c->idx.p, c->ccls.p, c->segcls.p,
+                       c->segnnz.p, g->queue.p, &ctl->nsel, &ctl->skip_swap);
+    hipLaunchKernelGGL(k_lp_totals, dim3(1), dim3(1024), 0, st, ns, c->segcls.p, c->tot.p, &ctl->skip_swap);
+    hipLaunchKernelGGL(k_lp_ctl_end, dim3(1), dim3(64), 0, st, ctl, c->tot.p);
+    g->ctx->launches += 11;
+    VNM_LAUNCH_CHECK("k_lp_ctl_end");
+    return VNM_OK;
+}
+
+static int lp_run_done(LPRun &r, bool *done) {
+    int32_t d = 0;
+    VNM_HIP_TRY(hipMemcpyAsync(&d, &r.ctl.p->done, 4, hipMemcpyDeviceToHost, r.g->ctx->stream));
+    VNM_HIP_TRY(hipStreamSynchronize(r.g->ctx->stream));
+    *done = d != 0;
+    return VNM_OK;
+}
+
+static int lp_run_end(LPRun &r, vnm_lp_stats *stats) {
+    hipStream_t st = r.g->ctx->stream;
+    LPCtl h;
+    int32_t tot[2];
+    VNM_HIP_TRY(hipMemcpyAsync(&h, r.ctl.p, sizeof(LPCtl), hipMemcpyDeviceToHost, st));
+    VNM_HIP_TRY(hipMemcpyAsync(tot, r.c->tot.p, 8, hipMemcpyDeviceToHost, st));
+    int rc = relabel(r.g, r.g->perm.p);
+    if (rc) return rc;
     VNM_HIP_TRY(hipStreamSynchronize(st));
     vnm_lp_stats S{};
-    S.init_cnt = tot[0];
-    S.init_size = tot[1];
-    bool exhaustive_now = (mode == 2);
-    int it = 0;
-    while (it < maxiter && tot[0] > 0) {
-        const int pick_mode = exhaustive_now ? 2 : mode;
-        if (pick_mode == 0 && (rc = lp_forward(g, c, m, 1))) return rc;
-        VNM_HIP_TRY(hipMemsetAsync(w.out.p, 0, 32, st));
-        hipLaunchKernelGGL(k_lp_pick, dim3(grid_of(ns, 256)), dim3(256), 0, st, ns, M, pick_mode,
-                           seed + (uint64_t)it * 0x9E3779B97F4A7C15ull, g->inv.p, c->ptr.p, c->ccls.p,
-                           c->segcls.p, c->Y.p, w.dkey.p, w.akey.p, w.dbest.p, w.abest.p, w.out.p + 4, sbits);
-        g->ctx->launches++;
-        VNM_LAUNCH_CHECK("k_lp_pick");
-        if ((rc = sort_keys_u64(w.dkey.p, w.sdkey.p, ns, 32 + sbits, st))) return rc;
-        if ((rc = sort_keys_u64(w.akey.p, w.sakey.p, ns, 32 + sbits, st))) return rc;
-        int32_t nd_h = 0;
-        VNM_HIP_TRY(hipMemcpyAsync(&nd_h, w.out.p + 4, 4, hipMemcpyDeviceToHost, st));
-        VNM_HIP_TRY(hipStreamSynchronize(st));
-        const int nd = std::min(topk, nd_h);
-        if (nd == 0) break;
-        const int all = exhaustive_now ? 1 : 0;
-        const int W = window;
-        const int stride = std::max(1, ns / std::max(nd, 1));
-        hipLaunchKernelGGL(k_lp_match, dim3(grid_of(nd, 256)), dim3(256), 0, st, nd, W, ns, stride, M, pick_mode,
-                           all, seed ^ ((uint64_t)it << 20), w.sdkey.p, w.sakey.p, w.dbest.p, w.abest.p, g->inv.p,
-                           c->Y.p, w.cand.p, smask);
-        g->ctx->launches++;
-        VNM_LAUNCH_CHECK("k_lp_match");
-        const int nc = all ? nd * W : nd;
-        if ((rc = lp_score_cands(g, c, N, w.cand.p, w.best.p, nc, nullptr))) return rc;
-        S.exact_pairs += nc;
-        hipLaunchKernelGGL(k_lp_gain_keys, dim3(grid_of(nc, 256)), dim3(256), 0, st, nc, w.cand.p, w.best.p,
-                           w.gkey.p, ibits, gmax);
-        if ((rc = sort_keys_u64(w.gkey.p, w.sgkey.p, nc, gbits + ibits, st))) return rc;
-        const size_t shm = (size_t)((ns + 31) / 32) * 4;
-        hipLaunchKernelGGL(k_lp_select, dim3(1), dim3(SEL_T), shm, st, nc, ns, w.sgkey.p, w.cand.p, w.best.p,
-                           w.acc.p, w.out.p, w.owner.p, imask);
-        g->ctx->launches += 2;
-        VNM_LAUNCH_CHECK("k_lp_select");
-        int32_t o[2];
-        VNM_HIP_TRY(hipMemcpyAsync(o, w.out.p, 8, hipMemcpyDeviceToHost, st));
-        VNM_HIP_TRY(hipStreamSynchronize(st));
-        ++it;
-        if (o[0] == 0) {
-            if (mode == 0 && !exhaustive_now) {
-                // nothing the proposals found pays: score the whole window exactly
-                exhaustive_now = true;
-                S.fallbacks++;
-                continue;
-            }
-            break;
-        }
-        hipLaunchKernelGGL(k_lp_swap, dim3(grid_of(o[0], 256)), dim3(256), 0, st, o[0], M, w.acc.p, w.cand.p,
-                           w.best.p, g->perm.p, g->inv.p, w.touched.p);
-        g->ctx->launches++;
-        VNM_LAUNCH_CHECK("k_lp_swap");
-        if ((rc = lp_stats(g, c, N, w.touched.p, 2 * o[0]))) return rc;
-        VNM_HIP_TRY(hipMemcpyAsync(tot, c->tot.p, 8, hipMemcpyDeviceToHost, st));
-        VNM_HIP_TRY(hipStreamSynchronize(st));
-        S.swaps += o[0];
-        S.gain += o[1];
-        if (mode == 0) exhaustive_now = false;
-    }
-    S.iters = it;
+    S.init_cnt = r.init_cnt;
+    S.init_size = r.init_size;
+    S.iters = h.it;
+    S.swaps = h.swaps;
+    S.gain = h.gain_total;
+    S.exact_pairs = h.exact_pairs;
+    S.fallbacks = h.fallbacks;
     S.final_cnt = tot[0];
     S.final_size = tot[1];
-    if ((rc = relabel(g, g->perm.p))) return rc;
-    VNM_HIP_TRY(hipStreamSynchronize(st));
-    S.ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    S.ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r.t0).count();
     *stats = S;
     return VNM_OK;
+}
+
+// The reorder itself, on the stream of g->ctx (arguments checked): one host
+// read per iteration, to stop once the run is over.
+static int lp_reorder_run(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window,
+                          int maxiter, uint64_t seed, vnm_lp_stats *stats) {
+    LPRun r;
+    int rc = lp_run_begin(r, g, m, N, mode, topk, window, seed);
+    if (rc) return rc;
+    bool done = false;
+    if ((rc = lp_run_done(r, &done))) return rc;
+    for (int it = 0; it < maxiter && !done; ++it) {
+        if ((rc = lp_run_iter(r))) return rc;
+        if ((rc = lp_run_done(r, &done))) return rc;
+    }
+    return lp_run_end(r, stats);
 }
 
 int lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, int window, int maxiter,
@@ -1681,15 +1875,22 @@ int lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int topk, i
     return lp_reorder_run(g, m, N, mode, topk, window, maxiter, seed, stats);
 }
 
-// The learned reorder of many graphs of one context: `workers` host threads,
-// each with a stream of its own, take the graphs in turn (a graph's run
-// launches dozens of small kernels and waits on the host three times per
-// iteration, so runs on separate streams overlap).  A graph's run, and so
-// its result, is the one lp_reorder makes alone with the same seed.
+// The learned reorder of many graphs of one context as one launch chain:
+// one iteration of every graph is captured as a HIP graph -- LP_BRANCHES
+// parallel branches (forked from the context stream inside the capture),
+// the graphs dealt round-robin over them, each branch with its own sort
+// scratch; every kernel reads its sizes and flags from its graph's control
+// block -- and the graph is replayed until every run is over or maxiter
+// iterations have run; the host reads the runs' done flags after each
+// replay and captures again with the running graphs once at most half of
+// the captured ones still run.  A graph's
+// result is the one lp_reorder makes alone with the same seed: the same
+// kernels take the same decisions.
+constexpr int LP_BRANCHES = 8;
+
 int lp_reorder_batch(vnm_graph *const *gs, int count, const vnm_lp_model *m, int N, int mode, int topk,
-                     int window, int maxiter, uint64_t seed, int workers, vnm_lp_stats *stats) {
+                     int window, int maxiter, uint64_t seed, vnm_lp_stats *stats) {
     if (count < 1) return set_error(VNM_E_ARG, "batch of %d graphs", count);
-    if (workers < 1 || workers > 16) return set_error(VNM_E_ARG, "workers must be 1..16 (got %d)", workers);
     vnm_ctx *ctx = gs[0] ? gs[0]->ctx : nullptr;
     for (int i = 0; i < count; ++i) {
         if (!gs[i]) return set_error(VNM_E_ARG, "graph %d is null", i);
@@ -1701,67 +1902,145 @@ int lp_reorder_batch(vnm_graph *const *gs, int count, const vnm_lp_model *m, int
         for (int j = 0; j < i; ++j)
             if (gs[j] == gs[i]) return set_error(VNM_E_ARG, "graph %d appears twice in the batch", i);
     }
-    VNM_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    const int W = std::min(workers, count);
-    // one stream per worker, default priority, released on every path
-    struct Aux {
-        std::vector<vnm_ctx> c;
-        explicit Aux(int n) : c(n) {}
-        ~Aux() {
-            for (auto &x : c)
+    hipStream_t st = ctx->stream;
+    const int B = std::min(LP_BRANCHES, count);
+    // branch streams (default priority) and fork/join events, released on
+    // every path; the graphs get their context back on every path
+    struct Res {
+        std::vector<vnm_ctx> aux;
+        std::vector<hipEvent_t> ev;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        vnm_graph *const *gs = nullptr;
+        int count = 0;
+        vnm_ctx *ctx = nullptr;
+        int slot = 0;
+        ~Res() {
+            for (int i = 0; i < count; ++i) gs[i]->ctx = ctx;
+            (void)sort_scratch_slot(slot);
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            for (auto e : ev) (void)hipEventDestroy(e);
+            for (auto &x : aux)
                 if (x.stream) {
                     (void)hipStreamSynchronize(x.stream);
                     (void)hipStreamDestroy(x.stream);
                 }
         }
-    } aux(W);
-    for (int w = 0; w < W; ++w) {
-        aux.c[w].device = ctx->device;
-        hipError_t e = hipStreamCreateWithFlags(&aux.c[w].stream, hipStreamNonBlocking);
+    } res;
+    res.gs = gs;
+    res.count = count;
+    res.ctx = ctx;
+    res.slot = sort_scratch_slot(0);
+    res.aux.resize(B);
+    for (int b = 0; b < B; ++b) {
+        res.aux[b].device = ctx->device;
+        hipError_t e = hipStreamCreateWithFlags(&res.aux[b].stream, hipStreamNonBlocking);
         if (e != hipSuccess) {
-            aux.c[w].stream = nullptr;
+            res.aux[b].stream = nullptr;
             return set_error(VNM_E_HIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
         }
     }
-    std::atomic<int> next{0}, failed{0};
-    std::vector<int> wrc(W, VNM_OK), wat(W, -1);
-    std::vector<std::string> wmsg(W);
-    auto work = [&](int w) {
-        if (hipSetDevice(ctx->device) != hipSuccess) {
-            wrc[w] = VNM_E_HIP;
-            wmsg[w] = "hipSetDevice failed in a worker";
-            failed.store(1);
-            return;
-        }
-        for (;;) {
-            const int i = next.fetch_add(1);
-            if (i >= count || failed.load()) return;
-            vnm_graph *g = gs[i];
-            g->ctx = &aux.c[w];
-            const int rc = lp_reorder_run(g, m, N, mode, topk, window, maxiter, seed, &stats[i]);
-            g->ctx = ctx;
-            if (rc) {
-                wrc[w] = rc;
-                wat[w] = i;
-                wmsg[w] = last_error();
-                failed.store(1);
-                return;
-            }
-        }
-    };
-    std::vector<std::thread> th;
-    try {  // workers that cannot be started leave their graphs to the others
-        for (int w = 1; w < W; ++w) th.emplace_back(work, w);
-    } catch (const std::system_error &) {
+    for (int k = 0; k < B + 1; ++k) {
+        hipEvent_t e = nullptr;
+        VNM_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        res.ev.push_back(e);
     }
-    work(0);
-    for (auto &t : th) t.join();
-    for (auto &x : aux.c) ctx->launches += x.launches;
-    // the failure of the lowest graph index is reported
-    int bad = -1;
-    for (int w = 0; w < W; ++w)
-        if (wrc[w] != VNM_OK && (bad < 0 || wat[w] < wat[bad] || wat[bad] < 0)) bad = w;
-    if (bad >= 0) return set_error(wrc[bad], "graph %d: %s", wat[bad], wmsg[bad].c_str());
+    // per graph: cache, buffers, control block (on the context stream), sort
+    // scratch reserved in its branch's slot
+    std::vector<LPRun> runs(count);
+    std::vector<LPCtl *> hctl(count);
+    int rc;
+    for (int i = 0; i < count; ++i) {
+        (void)sort_scratch_slot(i % B);
+        if ((rc = lp_run_begin(runs[i], gs[i], m, N, mode, topk, window, seed)))
+            return set_error(rc, "graph %d: %s", i, last_error());
+        hctl[i] = runs[i].ctl.p;
+    }
+    DBuf<LPCtl *> dctl;
+    DBuf<int32_t> dflags;
+    if ((rc = dctl.alloc(count)) || (rc = dflags.alloc(count))) return rc;
+    VNM_HIP_TRY(hipMemcpyAsync(dctl.p, hctl.data(), count * sizeof(LPCtl *), hipMemcpyHostToDevice, st));
+    std::vector<int32_t> done(count, 0);
+    VNM_HIP_TRY(hipStreamSynchronize(st));
+    // capture one iteration of the graphs in `act` (no event timing inside)
+    auto capture = [&](const std::vector<int> &act) -> int {
+        if (res.exec) (void)hipGraphExecDestroy(res.exec);
+        if (res.graph) (void)hipGraphDestroy(res.graph);
+        res.exec = nullptr;
+        res.graph = nullptr;
+        const bool timing = ctx->timing;
+        ctx->timing = false;
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            ctx->timing = timing;
+            return set_error(VNM_E_HIP, "capture of the learned iteration failed: %s", hipGetErrorString(e));
+        }
+        int rc = VNM_OK;
+        // the branches the running graphs use join the capture (a stream
+        // that did not would run its kernels at once instead of recording)
+        std::vector<char> used(B, 0);
+        for (int i : act) used[i % B] = 1;
+        if (hipEventRecord(res.ev[B], st) != hipSuccess) rc = set_error(VNM_E_HIP, "fork event failed");
+        for (int b = 0; b < B && rc == VNM_OK; ++b)
+            if (used[b] && hipStreamWaitEvent(res.aux[b].stream, res.ev[B], 0) != hipSuccess)
+                rc = set_error(VNM_E_HIP, "fork failed");
+        for (size_t k = 0; k < act.size() && rc == VNM_OK; ++k) {
+            const int i = act[k], b = i % B;  // a graph keeps its branch: its sort scratch slot
+            (void)sort_scratch_slot(b);
+            gs[i]->ctx = &res.aux[b];
+            rc = lp_run_iter(runs[i]);
+            gs[i]->ctx = ctx;
+            ctx->launches += res.aux[b].launches;
+            res.aux[b].launches = 0;
+        }
+        for (int b = 0; b < B && rc == VNM_OK; ++b)
+            if (used[b] && (hipEventRecord(res.ev[b], res.aux[b].stream) != hipSuccess ||
+                            hipStreamWaitEvent(st, res.ev[b], 0) != hipSuccess))
+                rc = set_error(VNM_E_HIP, "join failed");
+        if (rc == VNM_OK) hipLaunchKernelGGL(k_lp_done_flags, dim3(1), dim3(256), 0, st, dctl.p, count, dflags.p);
+        e = hipStreamEndCapture(st, &res.graph);
+        ctx->timing = timing;
+        if (rc) return rc;
+        if (e != hipSuccess)
+            return set_error(VNM_E_HIP, "capture of the learned iteration failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&res.exec, res.graph, nullptr, nullptr, 0);
+        if (e != hipSuccess)
+            return set_error(VNM_E_HIP, "instantiating the learned iteration failed: %s", hipGetErrorString(e));
+        return VNM_OK;
+    };
+    // replay; once at most half of the captured graphs still run, capture
+    // again with those only (a finished graph's kernels would all return at
+    // once, but each still costs a launch)
+    std::vector<int> act;
+    for (int i = 0; i < count; ++i) act.push_back(i);
+    size_t captured = 0;
+    const bool dbg = getenv("VNM_DEBUG") != nullptr;
+    auto tnow = [] { return std::chrono::steady_clock::now(); };
+    auto tms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    auto tb = tnow();
+    for (int it = 0; it < maxiter && !act.empty(); ++it) {
+        if (2 * act.size() <= captured || captured == 0) {
+            auto tc = tnow();
+            if ((rc = capture(act))) return rc;
+            captured = act.size();
+            if (dbg)
+                fprintf(stderr, "[vnm] lp batch: iteration %d, capture of %zu graphs %.2f ms (%.2f ms so far)\n", it,
+                        captured, tms(tc, tnow()), tms(tb, tnow()));
+        }
+        VNM_HIP_TRY(hipGraphLaunch(res.exec, st));
+        VNM_HIP_TRY(hipMemcpyAsync(done.data(), dflags.p, count * 4, hipMemcpyDeviceToHost, st));
+        VNM_HIP_TRY(hipStreamSynchronize(st));
+        std::vector<int> next;
+        for (int i : act)
+            if (!done[i]) next.push_back(i);
+        act.swap(next);
+    }
+    if (dbg) fprintf(stderr, "[vnm] lp batch: iterations done %.2f ms\n", tms(tb, tnow()));
+    for (int i = 0; i < count; ++i)
+        if ((rc = lp_run_end(runs[i], &stats[i]))) return set_error(rc, "graph %d: %s", i, last_error());
     return VNM_OK;
 }
 
@@ -1837,7 +2116,7 @@ int vnm_lp_features(vnm_graph *g, int N, float *X) {
     if (rc) return rc;
     hipStream_t st = g->ctx->stream;
     hipLaunchKernelGGL(k_lp_features<float>, dim3(grid_of(g->nrows, 256)), dim3(256), 0, st, g->nrows, g->M,
-                       g->perm.p, c->ptr.p, c->ccls.p, c->segcls.p, c->segnnz.p, c->tot.p, c->X.p);
+                       g->perm.p, c->ptr.p, c->ccls.p, c->segcls.p, c->segnnz.p, c->tot.p, c->X.p, nullptr);
     g->ctx->launches++;
     VNM_LAUNCH_CHECK("k_lp_features");
     VNM_HIP_TRY(hipMemcpyAsync(X, c->X.p, (size_t)g->nrows * LP_F * 4, hipMemcpyDeviceToHost, st));
@@ -1933,9 +2212,9 @@ int vnm_lp_reorder(vnm_graph *g, const vnm_lp_model *m, int N, int mode, int top
 }
 
 int vnm_lp_reorder_batch(vnm_graph *const *graphs, int count, const vnm_lp_model *m, int N, int mode, int topk,
-                         int window, int maxiter, uint64_t seed, int workers, vnm_lp_stats *st) {
+                         int window, int maxiter, uint64_t seed, vnm_lp_stats *st) {
     if (!graphs || !st) return set_error(VNM_E_ARG, "bad arguments");
-    return lp_reorder_batch(graphs, count, m, N, mode, topk, window, maxiter, seed, workers, st);
+    return lp_reorder_batch(graphs, count, m, N, mode, topk, window, maxiter, seed, st);
 }
 
 }  // extern "C"

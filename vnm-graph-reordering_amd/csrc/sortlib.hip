@@ -64,7 +64,13 @@ struct TempBuf {
         if (p) (void)hipFree(p);
     }
 };
-thread_local TempBuf g_tmp;
+// Scratch slots per host thread; the current one serves every sort and scan
+// (separate slots let launch chains that may run concurrently, such as the
+// branches of one captured graph, sort at the same time).
+constexpr int SCRATCH_SLOTS = 16;
+thread_local TempBuf g_slots[SCRATCH_SLOTS];
+thread_local int g_slot = 0;
+#define g_tmp (g_slots[g_slot])
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -122,7 +128,8 @@ __device__ __forceinline__ int digit_of(uint64_t k, int shift, int bits) {
 }
 
 __global__ __launch_bounds__(WG) void k_radix_hist(const uint64_t *__restrict__ keys, int64_t n, int shift,
-                                                   int bits, int tiles, int32_t *__restrict__ hist) {
+                                                   int bits, int tiles, int32_t *__restrict__ hist, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // the caller's flag: this sort is not wanted
     __shared__ int32_t cnt[RADIX];
     cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -141,7 +148,8 @@ __global__ __launch_bounds__(WG) void k_radix_scatter(const uint64_t *__restrict
                                                       const int32_t *__restrict__ vin,
                                                       uint64_t *__restrict__ kout, int32_t *__restrict__ vout,
                                                       int64_t n, int shift, int bits, int tiles,
-                                                      const int32_t *__restrict__ offs) {
+                                                      const int32_t *__restrict__ offs, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // the caller's flag: this sort is not wanted
     __shared__ int32_t run[NWAVE][RADIX];   // per wave: keys of each digit so far
     __shared__ int32_t start[RADIX];        // tile position of each digit's first key
     __shared__ int32_t sh[NWAVE];
@@ -220,7 +228,8 @@ __global__ __launch_bounds__(WG) void k_radix_scatter(const uint64_t *__restrict
 template <bool VALS>
 __global__ __launch_bounds__(WG) void k_radix_small(const uint64_t *__restrict__ kin, const int32_t *__restrict__ vin,
                                                     uint64_t *__restrict__ kout, int32_t *__restrict__ vout, int n,
-                                                    int end_bit) {
+                                                    int end_bit, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // the caller's flag: this sort is not wanted
     __shared__ int32_t run[NWAVE][RADIX];
     __shared__ int32_t start[RADIX];
     __shared__ int32_t sh[NWAVE];
@@ -307,7 +316,8 @@ __global__ __launch_bounds__(WG) void k_radix_small(const uint64_t *__restrict__
 // or scan launch is needed between passes.
 constexpr int FUSED_TILES = 16;
 
-__global__ void k_fill_zero_i32(int32_t *p, int n) {
+__global__ void k_fill_zero_i32(int32_t *p, int n, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // the caller's flag: this sort is not wanted
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
@@ -316,7 +326,8 @@ __global__ __launch_bounds__(WG) void k_radix_fused(const uint64_t *__restrict__
                                                     uint64_t *__restrict__ kout, int32_t *__restrict__ vout,
                                                     int n, int shift, int bits, int tiles,
                                                     const int32_t *__restrict__ hist, int next_shift,
-                                                    int next_bits, int32_t *__restrict__ next_hist) {
+                                                    int next_bits, int32_t *__restrict__ next_hist, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // the caller's flag: this sort is not wanted
     __shared__ int32_t run[NWAVE][RADIX];
     __shared__ int32_t start[RADIX];
     __shared__ int32_t offs[RADIX];
@@ -415,7 +426,8 @@ __device__ __forceinline__ int32_t load_as(const uint8_t *p, int64_t i) { return
 
 template <typename In, typename Op>
 __global__ __launch_bounds__(WG) void k_scan_reduce(const In *__restrict__ in, int64_t n,
-                                                    int32_t *__restrict__ totals) {
+                                                    int32_t *__restrict__ totals, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // the caller's flag: this sort is not wanted
     __shared__ int32_t sh[NWAVE];
     const Op op;
     const int64_t t0 = (int64_t)blockIdx.x * TILE;
@@ -431,7 +443,8 @@ __global__ __launch_bounds__(WG) void k_scan_reduce(const In *__restrict__ in, i
 
 // One workgroup: exclusive scan of the tile totals in place.
 template <typename Op>
-__global__ __launch_bounds__(WG) void k_scan_totals(int32_t *__restrict__ totals, int tiles) {
+__global__ __launch_bounds__(WG) void k_scan_totals(int32_t *__restrict__ totals, int tiles, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // the caller's flag: this sort is not wanted
     __shared__ int32_t sh[NWAVE];
     const Op op;
     int32_t carry = Op::id;
@@ -448,7 +461,8 @@ __global__ __launch_bounds__(WG) void k_scan_totals(int32_t *__restrict__ totals
 // Tile scan: elements staged in LDS, each thread scans 16 consecutive ones.
 template <typename In, typename Op, bool EXCL>
 __global__ __launch_bounds__(WG) void k_scan_tile(const In *__restrict__ in, int32_t *__restrict__ out, int64_t n,
-                                                  const int32_t *__restrict__ offsets) {
+                                                  const int32_t *__restrict__ offsets, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // the caller's flag: this sort is not wanted
     __shared__ int32_t s[TILE + TILE / 32];  // one pad word per 32 to spread the banks
     __shared__ int32_t sh[NWAVE];
     const Op op;
@@ -483,7 +497,8 @@ __global__ __launch_bounds__(WG) void k_scan_tile(const In *__restrict__ in, int
 
 template <typename In, typename Op, bool EXCL>
 // A whole scan of at most TILE elements in one workgroup (one launch).
-__global__ __launch_bounds__(WG) void k_scan_small(const In *__restrict__ in, int32_t *__restrict__ out, int n) {
+__global__ __launch_bounds__(WG) void k_scan_small(const In *__restrict__ in, int32_t *__restrict__ out, int n, const int *__restrict__ skip) {
+    if (skip && *skip) return;  // the caller's flag: this sort is not wanted
     __shared__ int32_t s[TILE + TILE / 32];
     __shared__ int32_t sh[NWAVE];
     const Op op;
@@ -529,17 +544,18 @@ int tiles_of(int64_t n) { return (int)((n + TILE - 1) / TILE); }
 
 // Scan of n elements with `totals` (>= tiles_of(n) ints) as scratch.
 template <typename In, typename Op, bool EXCL>
-int scan_with(const In *in, int32_t *out, int64_t n, int32_t *totals, hipStream_t st) {
+int scan_with(const In *in, int32_t *out, int64_t n, int32_t *totals, hipStream_t st,
+              const int *skip = nullptr) {
     if (n <= TILE) {
-        hipLaunchKernelGGL((k_scan_small<In, Op, EXCL>), dim3(1), dim3(WG), 0, st, in, out, (int)n);
+        hipLaunchKernelGGL((k_scan_small<In, Op, EXCL>), dim3(1), dim3(WG), 0, st, in, out, (int)n, skip);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return set_error(VNM_E_HIP, "scan launch failed: %s", hipGetErrorString(e));
         return VNM_OK;
     }
     const int tiles = tiles_of(n);
-    hipLaunchKernelGGL((k_scan_reduce<In, Op>), dim3(tiles), dim3(WG), 0, st, in, n, totals);
-    hipLaunchKernelGGL((k_scan_totals<Op>), dim3(1), dim3(WG), 0, st, totals, tiles);
-    hipLaunchKernelGGL((k_scan_tile<In, Op, EXCL>), dim3(tiles), dim3(WG), 0, st, in, out, n, totals);
+    hipLaunchKernelGGL((k_scan_reduce<In, Op>), dim3(tiles), dim3(WG), 0, st, in, n, totals, skip);
+    hipLaunchKernelGGL((k_scan_totals<Op>), dim3(1), dim3(WG), 0, st, totals, tiles, skip);
+    hipLaunchKernelGGL((k_scan_tile<In, Op, EXCL>), dim3(tiles), dim3(WG), 0, st, in, out, n, totals, skip);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(VNM_E_HIP, "scan launch failed: %s", hipGetErrorString(e));
     return VNM_OK;
@@ -547,12 +563,12 @@ int scan_with(const In *in, int32_t *out, int64_t n, int32_t *totals, hipStream_
 
 template <bool VALS>
 int radix_sort(const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t *vout, int64_t n, int end_bit,
-               hipStream_t st) {
+               hipStream_t st, const int *skip = nullptr) {
     if (n <= 0) return VNM_OK;
     if (n > (int64_t)INT32_MAX) return set_error(VNM_E_ARG, "radix sort of %lld keys", (long long)n);
     end_bit = std::max(1, std::min(end_bit, 64));
     if (n <= TILE) {
-        hipLaunchKernelGGL((k_radix_small<VALS>), dim3(1), dim3(WG), 0, st, kin, vin, kout, vout, (int)n, end_bit);
+        hipLaunchKernelGGL((k_radix_small<VALS>), dim3(1), dim3(WG), 0, st, kin, vin, kout, vout, (int)n, end_bit, skip);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return set_error(VNM_E_HIP, "radix sort launch failed: %s", hipGetErrorString(e));
         return VNM_OK;
@@ -569,8 +585,8 @@ int radix_sort(const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t 
         uint64_t *ktmp = (uint64_t *)g_tmp.p;
         int32_t *vtmp = (int32_t *)((char *)g_tmp.p + kb);
         int32_t *hists = (int32_t *)((char *)g_tmp.p + kb + vb);
-        hipLaunchKernelGGL(k_fill_zero_i32, dim3((tab * passes + 255) / 256), dim3(256), 0, st, hists, tab * passes);
-        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(WG), 0, st, kin, n, 0, std::min(8, end_bit), tiles, hists);
+        hipLaunchKernelGGL(k_fill_zero_i32, dim3((tab * passes + 255) / 256), dim3(256), 0, st, hists, tab * passes, skip);
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(WG), 0, st, kin, n, 0, std::min(8, end_bit), tiles, hists, skip);
         const uint64_t *ks = kin;
         const int32_t *vs = vin;
         for (int p = 0; p < passes; ++p) {
@@ -580,7 +596,7 @@ int radix_sort(const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t 
             int32_t *vd = ((passes - 1 - p) % 2 == 0) ? vout : vtmp;
             hipLaunchKernelGGL((k_radix_fused<VALS>), dim3(tiles), dim3(WG), 0, st, ks, vs, kd, vd, (int)n, shift, bits,
                                tiles, hists + (size_t)tab * p, last ? 0 : shift + 8,
-                               last ? 1 : std::min(8, end_bit - shift - 8), last ? nullptr : hists + (size_t)tab * (p + 1));
+                               last ? 1 : std::min(8, end_bit - shift - 8), last ? nullptr : hists + (size_t)tab * (p + 1), skip);
             ks = kd;
             vs = vd;
         }
@@ -608,10 +624,10 @@ int radix_sort(const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t 
         // the last pass writes the output
         uint64_t *kd = ((passes - 1 - p) % 2 == 0) ? kout : ktmp;
         int32_t *vd = ((passes - 1 - p) % 2 == 0) ? vout : vtmp;
-        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(WG), 0, st, ks, n, shift, bits, tiles, hist);
-        if ((rc = scan_with<int32_t, OpSum, true>(hist, offs, nh, tot, st))) return rc;
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(WG), 0, st, ks, n, shift, bits, tiles, hist, skip);
+        if ((rc = scan_with<int32_t, OpSum, true>(hist, offs, nh, tot, st, skip))) return rc;
         hipLaunchKernelGGL((k_radix_scatter<VALS>), dim3(tiles), dim3(WG), 0, st, ks, vs, kd, vd, n, shift, bits,
-                           tiles, offs);
+                           tiles, offs, skip);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return set_error(VNM_E_HIP, "radix sort launch failed: %s", hipGetErrorString(e));
         ks = kd;
@@ -624,6 +640,27 @@ int radix_sort(const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t 
 
 int sort_keys_u64(const uint64_t *in, uint64_t *out, int64_t n, int end_bit, hipStream_t st) {
     return radix_sort<false>(in, out, nullptr, nullptr, n, end_bit, st);
+}
+
+int sort_keys_u64_if(const uint64_t *in, uint64_t *out, int64_t n, int end_bit, hipStream_t st, const int *skip) {
+    return radix_sort<false>(in, out, nullptr, nullptr, n, end_bit, st, skip);
+}
+
+int sort_scratch_slot(int slot) {
+    const int prev = g_slot;
+    if (slot >= 0 && slot < SCRATCH_SLOTS) g_slot = slot;
+    return prev;
+}
+
+int sort_scratch_reserve(int64_t n) {
+    if (n <= TILE) return VNM_OK;
+    // the larger of the two multi-tile paths' needs, keys and values, 8 passes
+    const int tiles = tiles_of(n);
+    const size_t kb = align256((size_t)n * 8), vb = align256((size_t)n * 4);
+    const size_t fused = kb + vb + align256((size_t)RADIX * tiles * 8 * 4);
+    const int64_t nh = (int64_t)RADIX * tiles;
+    const size_t multi = kb + vb + 2 * align256((size_t)nh * 4) + align256((size_t)tiles_of(nh) * 4);
+    return g_tmp.ensure(std::max(fused, multi));
 }
 
 int sort_pairs_u64_i32(const uint64_t *kin, uint64_t *kout, const int32_t *vin, int32_t *vout, int64_t n,

@@ -193,7 +193,7 @@ def lib():
     L.vnm_lp_reorder.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_int, ctypes.c_uint64, P(LPStats)]
     L.vnm_lp_reorder_batch.argtypes = [P(vp), ctypes.c_int, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                       ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, P(LPStats)]
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_uint64, P(LPStats)]
     L.vnm_hybrid_learned.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, P(HybridStats)]
     L.vnm_graph_clone.argtypes = [vp, P(vp)]
@@ -518,15 +518,15 @@ class Graph:
         return st
 
     @staticmethod
-    def lp_reorder_batch(graphs, model, N, mode=LP_LEARNED, topk=4096, window=16, maxiter=20, seed=1,
-                         workers=4):
-        """lp_reorder of several graphs of one context on `workers` streams;
-        each result is lp_reorder's on the graph alone.  Returns the stats."""
+    def lp_reorder_batch(graphs, model, N, mode=LP_LEARNED, topk=4096, window=16, maxiter=20, seed=1):
+        """lp_reorder of several graphs of one context as one launch chain (a
+        captured iteration of all graphs, replayed); each result is
+        lp_reorder's on the graph alone.  Returns the stats."""
         n = len(graphs)
         hs = (ctypes.c_void_p * n)(*[g.h for g in graphs])
         st = (LPStats * n)()
         _check(lib().vnm_lp_reorder_batch(hs, n, model.h if model is not None else None, N, mode, topk,
-                                          window, maxiter, seed, workers, st))
+                                          window, maxiter, seed, st))
         return list(st)
 
     def venom_eval(self, V):
