@@ -415,7 +415,6 @@ struct GreedyShared {
     int maxk;             // precomputed pops the greedy used, most over the ordered pairs
     int swv1[64], swv2[64], swlg[64], swrg[64], swp[64], swt[64], swpc[64], swtc[64];
     int nsw;
-    uint64_t wtop[NW][KMAX + 1];
     uint64_t wbest[NW];
     uint64_t topk[KMAX + 1];
     int selthr[KMAX + 1], nselthr;  // threads whose segments hold the round's top keys
@@ -461,35 +460,35 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Top-k (k <= KMAX + 1) keys of the block into S.topk (0 = none); tk holds
-// each thread's descending local list.
-__device__ __forceinline__ void block_topk(uint64_t (&tk)[KMAX + 1], int k, GreedyShared &S) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+// Top k (k <= KMAX + 1) of n keys in LDS (0 = none) into S.topk, by wave 0
+// alone: each lane keeps a descending list of its keys (j = lane, lane + 64,
+// ...), then k wave maxima merge the lists.  The other waves skip it (one
+// wave's short chains instead of sixteen waves sharing four SIMDs).
+__device__ __forceinline__ void wave0_topk(const uint64_t *keys, int n, int k, GreedyShared &S) {
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x >= 64) return;
+    uint64_t lst[KMAX + 1];
+#pragma unroll
+    for (int i = 0; i <= KMAX; ++i) lst[i] = 0;
+    for (int j = lane; j < n; j += 64) {
+        uint64_t x = keys[j];
+        if (x <= lst[KMAX]) continue;
+#pragma unroll
+        for (int i = 0; i <= KMAX; ++i) {
+            const uint64_t hi = lst[i] > x ? lst[i] : x;
+            x = lst[i] > x ? x : lst[i];
+            lst[i] = hi;
+        }
+    }
     for (int it = 0; it < k; ++it) {
-        uint64_t m = wave_max_u64(tk[0]);
-        if (m != 0 && tk[0] == m) {
+        const uint64_t m = wave_max_u64(lst[0]);
+        if (m != 0 && lst[0] == m) {
 #pragma unroll
-            for (int i = 0; i < KMAX; ++i) tk[i] = tk[i + 1];
-            tk[KMAX] = 0;
+            for (int i = 0; i < KMAX; ++i) lst[i] = lst[i + 1];
+            lst[KMAX] = 0;
         }
-        if (lane == 0) S.wtop[w][it] = m;
+        if (lane == 0) S.topk[it] = m;
     }
-    __syncthreads();
-    if (w == 0) {
-        uint64_t lst[KMAX + 1];
-#pragma unroll
-        for (int i = 0; i <= KMAX; ++i) lst[i] = (lane < NW && i < k) ? S.wtop[lane][i] : 0ull;
-        for (int it = 0; it < k; ++it) {
-            uint64_t m = wave_max_u64(lst[0]);
-            if (m != 0 && lst[0] == m) {
-#pragma unroll
-                for (int i = 0; i < KMAX; ++i) lst[i] = lst[i + 1];
-                lst[KMAX] = 0;
-            }
-            if (lane == 0) S.topk[it] = m;
-        }
-    }
-    __syncthreads();
 }
 
 __device__ __forceinline__ uint64_t block_max_u64(uint64_t v, GreedyShared &S) {
@@ -1036,7 +1035,12 @@ __global__ __launch_bounds__(GT) void k_gr_select(NMArgs a0, const NMArgs *__res
             }
         }
         uint64_t bl = block_max_u64(best_l, S);
-        block_topk(tk, k, S);  // the k largest thread maxima
+        // the k largest thread maxima (one key per thread in LDS)
+        uint64_t *keys = (uint64_t *)&S.wkeys[0][0];
+        keys[tid] = tk[0];
+        __syncthreads();
+        wave0_topk(keys, GT, k, S);
+        __syncthreads();
         if (tid == 0) {
             int n = 0;
             for (int i = 0; i < k; ++i)
@@ -1044,23 +1048,36 @@ __global__ __launch_bounds__(GT) void k_gr_select(NMArgs a0, const NMArgs *__res
             S.nselthr = n;
         }
         __syncthreads();
-#pragma unroll
-        for (int i = 0; i <= KMAX; ++i) tk[i] = 0;
+        // those threads' segments scored again by wave 0, then their top k
         const int per = (nseg + GT - 1) / GT;  // segments per thread, at most
         const int ncand = S.nselthr * per;
-        for (int j = tid; j < ncand; j += GT) {
-            const int s = S.selthr[j / per] + (j % per) * GT;
-            const int c = s < nseg ? ld_l2(&A.cnt[s]) : 0;
-            if (c == 0) continue;
-            uint64_t x = tfwd ? key_fwd(s, c) : key_rev(s, c);
+        if (tid < 64) {
+            uint64_t lst[KMAX + 1];
 #pragma unroll
-            for (int i = 0; i <= KMAX; ++i) {
-                uint64_t hi = tk[i] > x ? tk[i] : x;
-                x = tk[i] > x ? x : tk[i];
-                tk[i] = hi;
+            for (int i = 0; i <= KMAX; ++i) lst[i] = 0;
+            for (int j = tid; j < ncand; j += 64) {
+                const int s = S.selthr[j / per] + (j % per) * GT;
+                const int c = s < nseg ? ld_l2(&A.cnt[s]) : 0;
+                if (c == 0) continue;
+                uint64_t x = tfwd ? key_fwd(s, c) : key_rev(s, c);
+#pragma unroll
+                for (int i = 0; i <= KMAX; ++i) {
+                    const uint64_t hi = lst[i] > x ? lst[i] : x;
+                    x = lst[i] > x ? x : lst[i];
+                    lst[i] = hi;
+                }
+            }
+            for (int it = 0; it < k; ++it) {
+                const uint64_t m = wave_max_u64(lst[0]);
+                if (m != 0 && lst[0] == m) {
+#pragma unroll
+                    for (int i = 0; i < KMAX; ++i) lst[i] = lst[i + 1];
+                    lst[KMAX] = 0;
+                }
+                if (tid == 0) S.topk[it] = m;
             }
         }
-        block_topk(tk, k, S);
+        __syncthreads();
         if (tid == 0) {
             int primary = seg_of_key(bl, !rev);
             S.size1 = 0;
