@@ -883,8 +883,11 @@ __global__ void k_lp_pick(LPCtl *ctl, int nseg, int M, int mode0, uint64_t seed0
     abest[s] = (int8_t)bai;
     // (score descending, segment) in 32 + sbits bits, sbits the bits of the
     // largest segment index: the keys' order, sorted on those bits only
-    dkey[s] = bdi >= 0 ? ((uint64_t)(~ord_f32(bd)) << sbits) | (uint32_t)s : ~0ull;
-    akey[s] = ((uint64_t)(~ord_f32(ba)) << sbits) | (uint32_t)s;
+    // one array for one sort: donors first, acceptors after them (bit
+    // 32 + sbits set), a segment without a donor the last of the donors
+    const uint64_t low = (1ull << (32 + sbits)) - 1;
+    dkey[s] = bdi >= 0 ? ((uint64_t)(~ord_f32(bd)) << sbits) | (uint32_t)s : low;
+    akey[s] = (low + 1) | ((uint64_t)(~ord_f32(ba)) << sbits) | (uint32_t)s;
     // donors counted once per wave (lanes past nseg have returned)
     const uint64_t dm = __ballot(bdi >= 0);
     const uint64_t live = __ballot(1);
@@ -1677,7 +1680,7 @@ static int lp_begin(vnm_graph *g, int N, LPCache **out) {
 // ------------------------------------------------------------ host: reorder
 
 struct LPRound {
-    DBuf<uint64_t> dkey, akey, sdkey, sakey, gkey, sgkey;
+    DBuf<uint64_t> dkey, sdkey, gkey, sgkey;  // dkey/sdkey: donors then acceptors (2 x nseg)
     DBuf<int8_t> dbest, abest;
     DBuf<Cand> cand;
     DBuf<int4> best;
@@ -1752,8 +1755,8 @@ static int lp_run_begin(LPRun &r, vnm_graph *g, const vnm_lp_model *m, int N, in
     r.smask = (1ull << r.sbits) - 1;
     r.imask = (1ull << r.ibits) - 1;
     LPRound &w = r.w;
-    if ((rc = w.dkey.alloc(ns)) || (rc = w.akey.alloc(ns)) || (rc = w.sdkey.alloc(ns)) ||
-        (rc = w.sakey.alloc(ns)) || (rc = w.dbest.alloc(ns)) || (rc = w.abest.alloc(ns)) ||
+    if ((rc = w.dkey.alloc(2 * (size_t)ns)) || (rc = w.sdkey.alloc(2 * (size_t)ns)) ||
+        (rc = w.dbest.alloc(ns)) || (rc = w.abest.alloc(ns)) ||
         (rc = w.cand.alloc(r.ncap)) || (rc = w.best.alloc(r.ncap)) || (rc = w.gkey.alloc(r.ncap)) ||
         (rc = w.sgkey.alloc(r.ncap)) || (rc = w.acc.alloc(r.ncap)) ||
         (rc = w.touched.alloc(2 * (size_t)std::min(r.ncap, ns))) || (rc = w.owner.alloc(ns)) ||
@@ -1761,7 +1764,7 @@ static int lp_run_begin(LPRun &r, vnm_graph *g, const vnm_lp_model *m, int N, in
         return rc;
     if (!g->queue.p && (rc = g->queue.alloc(4))) return rc;
     // the sorts of an iteration allocate nothing (they may run under capture)
-    if ((rc = sort_scratch_reserve(std::max(ns, r.ncap)))) return rc;
+    if ((rc = sort_scratch_reserve(std::max(2 * ns, r.ncap)))) return rc;
     VNM_HIP_TRY(hipMemsetAsync(w.owner.p, 0x7F, (size_t)ns * 4, st));  // OWNER_NONE in every slot
     int32_t tot[2];
     VNM_HIP_TRY(hipMemcpyAsync(tot, r.c->tot.p, 8, hipMemcpyDeviceToHost, st));
@@ -1787,13 +1790,13 @@ static int lp_run_iter(LPRun &r) {
     hipLaunchKernelGGL(k_lp_ctl_begin, dim3(1), dim3(64), 0, st, ctl, r.mode);
     if (r.mode == 0 && (rc = lp_forward(g, c, r.m, 1, &ctl->skip_fwd))) return rc;
     hipLaunchKernelGGL(k_lp_pick, dim3(grid_of(ns, 256)), dim3(256), 0, st, ctl, ns, M, r.mode, r.seed, g->inv.p,
-                       c->ptr.p, c->ccls.p, c->segcls.p, c->Y.p, w.dkey.p, w.akey.p, w.dbest.p, w.abest.p, r.sbits);
+                       c->ptr.p, c->ccls.p, c->segcls.p, c->Y.p, w.dkey.p, w.dkey.p + ns, w.dbest.p, w.abest.p,
+                       r.sbits);
     VNM_LAUNCH_CHECK("k_lp_pick");
-    if ((rc = sort_keys_u64_if(w.dkey.p, w.sdkey.p, ns, 32 + r.sbits, st, &ctl->done))) return rc;
-    if ((rc = sort_keys_u64_if(w.akey.p, w.sakey.p, ns, 32 + r.sbits, st, &ctl->done))) return rc;
+    if ((rc = sort_keys_u64_if(w.dkey.p, w.sdkey.p, 2 * (int64_t)ns, 33 + r.sbits, st, &ctl->done))) return rc;
     hipLaunchKernelGGL(k_lp_ctl_mid, dim3(1), dim3(64), 0, st, ctl, r.topk, r.window, ns);
     hipLaunchKernelGGL(k_lp_match, dim3(grid_of(r.nnorm, 256)), dim3(256), 0, st, ctl, r.window, ns, M, r.mode,
-                       r.seed, w.sdkey.p, w.sakey.p, w.dbest.p, w.abest.p, g->inv.p, c->Y.p, w.cand.p, r.smask);
+                       r.seed, w.sdkey.p, w.sdkey.p + ns, w.dbest.p, w.abest.p, g->inv.p, c->Y.p, w.cand.p, r.smask);
     VNM_LAUNCH_CHECK("k_lp_match");
     if ((rc = lp_score_cands(g, c, r.N, w.cand.p, w.best.p, r.ncap, nullptr, &ctl->nc))) return rc;
     hipLaunchKernelGGL(k_lp_gain_keys, dim3(grid_of(r.ncap, 256)), dim3(256), 0, st, ctl, r.ncap, w.cand.p,
