@@ -737,16 +737,16 @@ int pack(vnm_graph *g, int V, int dtype, int dense_min, vnm_packed **out, int64_
     VNM_HIP_TRY(hipMemcpyAsync(&NV, voff.p + E, 4, hipMemcpyDeviceToHost, st));
     VNM_HIP_TRY(hipStreamSynchronize(st));
     p->nval = NV;
-    // 8 zero records past the end: the SpMM reads B rows and weights 8 values
-    // at a time
-    const size_t vn = (size_t)NV + 8;
+    // VTAIL zero records past the end: the SpMM reads B rows and weights a
+    // block of values at a time
+    const size_t vn = (size_t)NV + VTAIL;
     if ((rc = p->vrow.alloc(vn)) || (rc = p->vcol.alloc(vn)) || (rc = p->vw.alloc(vn))) {
         delete p;
         return rc;
     }
-    VNM_HIP_TRY(hipMemsetAsync(p->vrow.p + NV, 0, 8 * 4, st));
-    VNM_HIP_TRY(hipMemsetAsync(p->vcol.p + NV, 0, 8 * 4, st));
-    VNM_HIP_TRY(hipMemsetAsync(p->vw.p + NV, 0, 8 * 4, st));
+    VNM_HIP_TRY(hipMemsetAsync(p->vrow.p + NV, 0, VTAIL * 4, st));
+    VNM_HIP_TRY(hipMemsetAsync(p->vcol.p + NV, 0, VTAIL * 4, st));
+    VNM_HIP_TRY(hipMemsetAsync(p->vw.p + NV, 0, VTAIL * 4, st));
     if (E > 0)
         hipLaunchKernelGGL(k_values, dim3(grid_of(E, 256)), dim3(256), 0, st, E, M, dtype, voff.p, p->ent_row.p,
                            p->ent_col.p, p->ent_idx.p, p->ent_val.p, p->ent_blk.p, p->blk_scale.p,

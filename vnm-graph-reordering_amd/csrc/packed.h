@@ -24,7 +24,7 @@ struct vnm_packed {
     vnm::DBuf<uint16_t> blk_sel;
     vnm::DBuf<float> blk_scale;
     // the SpMM's value stream: every value of the row entries, row-major, as
-    // (row, B row, FP32 weight), and 8 zero records past its end
+    // (row, B row, FP32 weight), and VTAIL zero records past its end
     int32_t nval = 0;
     vnm::DBuf<int32_t> vrow, vcol;
     vnm::DBuf<float> vw;
@@ -51,6 +51,8 @@ struct vnm_packed {
 namespace vnm {
 
 constexpr int ITEM = 64;      // values per work item
+constexpr int SBLK = 8;       // values per block of the SpMM's item walk
+constexpr int VTAIL = SBLK;   // zero records past the end of the value stream
 constexpr int GAP_MAX = 8;    // runs of rows without entries up to this long are written by the
                               // item walk; longer runs by the fix-up pass (one wave per row)
 
@@ -365,42 +367,43 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ vrow, cons
     // weights by scalar loads, all gathers in flight, then the FMAs in value
     // order, one row segment of the block at a time (a block without a row
     // start is one segment)
-    for (int k0 = 0; k0 < cnt; k0 += 8) {
-        const int m = min(8, cnt - k0);
-        int c[8];
-        float w[8];
+    constexpr int BLK = SBLK;
+    for (int k0 = 0; k0 < cnt; k0 += BLK) {
+        const int m = min(BLK, cnt - k0);
+        int c[BLK];
+        float w[BLK];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {  // past the item: later items or the stream's zero tail
+        for (int t = 0; t < BLK; ++t) {  // past the item: later items or the stream's zero tail
             c[t] = cp[k0 + t];
             w[t] = wp[k0 + t];
         }
-        BRaw<BT, VEC> raw[8];
+        BRaw<BT, VEC> raw[BLK];
         if constexpr (PAIR) {
-            if (m == 8) {
+            if (m == BLK) {
 #pragma unroll
-                for (int t = 0; t < 8; t += 2) gather2(c[t], w[t], c[t + 1], w[t + 1], raw[t], raw[t + 1]);
+                for (int t = 0; t < BLK; t += 2) gather2(c[t], w[t], c[t + 1], w[t + 1], raw[t], raw[t + 1]);
             } else {
                 float wd = 0.f;
                 BRaw<BT, VEC> rd;
 #pragma unroll
-                for (int t = 0; t < 8; t += 2) {
+                for (int t = 0; t < BLK; t += 2) {
                     if (t + 1 < m) gather2(c[t], w[t], c[t + 1], w[t + 1], raw[t], raw[t + 1]);
                     else if (t < m) gather2(c[t], w[t], c[t], wd, raw[t], rd);  // the last, unpaired
                 }
             }
-        } else if (m == 8) {
+        } else if (m == BLK) {
 #pragma unroll
-            for (int t = 0; t < 8; ++t) raw[t] = gather(c[t], w[t]);
+            for (int t = 0; t < BLK; ++t) raw[t] = gather(c[t], w[t]);
         } else {
 #pragma unroll
-            for (int t = 0; t < 8; ++t)
+            for (int t = 0; t < BLK; ++t)
                 if (t < m) raw[t] = gather(c[t], w[t]);
         }
         // values of this block that start a row (value 0's bit is clear)
-        uint32_t rem = (uint32_t)(starts >> k0) & 0xFFu;
-        if (m == 8 && rem == 0) {
+        uint32_t rem = (uint32_t)(starts >> k0) & ((1u << BLK) - 1u);
+        if (m == BLK && rem == 0) {
 #pragma unroll
-            for (int t = 0; t < 8; ++t) fma_raw<BT, VEC>(acc, w[t], raw[t]);
+            for (int t = 0; t < BLK; ++t) fma_raw<BT, VEC>(acc, w[t], raw[t]);
             continue;
         }
         for (int t0 = 0;;) {
@@ -408,7 +411,7 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ vrow, cons
             const int t1 = rem ? (int)__builtin_ctz(rem) : m;
             const uint32_t seg = ((1u << t1) - 1u) & ~((1u << t0) - 1u);
 #pragma unroll
-            for (int t = 0; t < 8; ++t)
+            for (int t = 0; t < BLK; ++t)
                 if ((seg >> t) & 1u) fma_raw<BT, VEC>(acc, w[t], raw[t]);
             if (!rem) break;
             // a new row starts at value k0 + t1
