@@ -164,6 +164,30 @@ def test_rmat262144_schedules(ctx):
     _check_sched_batch(ctx, coo, 8, 128, [f"rmat262144_sched{s}" for s in range(4)])
 
 
+# The learned runs' trajectories (iterations, final invalid rows, swaps,
+# exactly scored pairs, fallbacks to the whole window) as the iteration loop
+# produced them when it read its counts back on the host after every step;
+# the device-driven loop takes the same decisions, so it must reproduce them.
+LEARNED_TRAJECTORY = {
+    "rmat65536": (9, 281, 2627, 3372, 0),
+    "rmat262144": (13, 1015, 11694, 14211, 1),
+}
+
+
+def _trajectory(st):
+    return (st.iters, st.final_cnt, st.swaps, st.exact_pairs, st.fallbacks)
+
+
+def test_rmat65536_learned_trajectory(ctx):
+    coo = vnm.gen_rmat(16, 524288, 2, M=8)
+    model = vnm.LPModel.load(ctx, WEIGHTS)
+    g = vnm.Graph(ctx, coo, 8)
+    st = g.lp_reorder(model, N, topk=4096, window=16, maxiter=40)
+    assert _trajectory(st) == LEARNED_TRAJECTORY["rmat65536"], st.as_dict()
+    assert (st.final_cnt, st.final_size) == _recount(g) and _is_perm(g)
+    g.close()
+
+
 def test_rmat262144_learned_reaches_exact(ctx):
     """The proposer's run: exact final counts, a permutation, the same bits on
     a repeat run, and at least 99.5 % of the best exact schedule's retained
@@ -177,6 +201,7 @@ def test_rmat262144_learned_reaches_exact(ctx):
         st = g.lp_reorder(model, N, topk=4096, window=16, maxiter=40)
         assert (st.final_cnt, st.final_size) == _recount(g)
         assert st.final_cnt == st.init_cnt - st.gain and _is_perm(g)
+        assert _trajectory(st) == LEARNED_TRAJECTORY["rmat262144"], st.as_dict()
         r = g.retained_nnz(128)
         print(f"learned rep {rep}: {st.as_dict()} retained {r} best exact {best}", flush=True)
         assert r >= 0.995 * best
