@@ -270,12 +270,12 @@ __global__ __launch_bounds__(SC_ROWS) void k_score_rows(int R, int M, int N, int
 // heap code as the greedy pushes every pair and pops them all.
 __global__ void k_pq_pop_order(const int32_t *__restrict__ scores, int count, int M,
                                int32_t *__restrict__ order) {
-    extern __shared__ uint16_t hsm[];  // slot s of thread t at [s * blockDim.x + t]
+    extern __shared__ uint32_t hsm[];  // thread t's heap at [t * HEAP_WORDS, (t + 1) * HEAP_WORDS)
     const int MM = M * M;
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= count) return;
     const int32_t *sc = scores + (size_t)q * 2 * MM;
-    HeapRef H{hsm + threadIdx.x, 0, (int)blockDim.x};
+    HeapRef H{reinterpret_cast<uint16_t *>(hsm + threadIdx.x * HEAP_WORDS), 0};
     for (int i = 0; i < MM; ++i) {
         const int64_t ki = swap_key(sc[2 * i], sc[2 * i + 1]);
         int r = 0;
@@ -405,9 +405,9 @@ struct GreedyShared {
     //   Bl[b1*M+b2] = sum over rows in both of y_b2 * (x_b1 ? A : B)
     //   Br[b1*M+b2] = sum over rows in both of x_b1 * (y_b2 ? D : C)
     int both[NPAIR_T][2][MMAX * MMAX];
-    // per ordered pair q: the heap (slot-major, lists interleaved), its first
-    // first npre pops, the next of those not yet consumed, and the heap's size
-    uint16_t heap[MMAX * MMAX][NQ];
+    // per ordered pair q: the heap, its first
+    // npre pops, the next of those not yet consumed, and the heap's size
+    uint32_t heap[NQ][HEAP_WORDS];
     uint8_t popped[MMAX * MMAX][NQ];
     int qcur[NQ];
     int heapn[NQ];
@@ -844,7 +844,7 @@ __device__ __forceinline__ int first_free(GreedyShared &S, int q, int a, int b, 
         }
         S.qcur[q] = npre;
     }
-    HeapRef H{&S.heap[0][q], S.heapn[q], NQ};
+    HeapRef H{reinterpret_cast<uint16_t *>(S.heap[q]), S.heapn[q]};
     while (H.n > 0) {
         const int e = (int)(heap_pop(H) & 0xFFu);
         S.heap_pops += 1;
@@ -1189,7 +1189,10 @@ __global__ __launch_bounds__(GT) void k_gr_greedy(NMArgs a0, const NMArgs *__res
                 for (int j = 0; j < MM; ++j) rnk += S.wkeys[w][j] < key;
             }
             // the push order is element order: stage entry e in slot e
-            if (lane < MM) S.heap[lane][q] = (uint16_t)((rnk << 8) | lane);
+            if (lane < MM) {
+                HeapRef Hq{reinterpret_cast<uint16_t *>(S.heap[q]), 0};
+                hset(Hq, lane, (uint32_t)((rnk << 8) | lane));
+            }
             __builtin_amdgcn_wave_barrier();
         }
         __syncthreads();
@@ -1203,7 +1206,7 @@ __global__ __launch_bounds__(GT) void k_gr_greedy(NMArgs a0, const NMArgs *__res
             S.maxk = 0;
         }
         if (tid < nq) {
-            HeapRef H{&S.heap[0][tid], 0, NQ};
+            HeapRef H{reinterpret_cast<uint16_t *>(S.heap[tid]), 0};
             for (int e = 0; e < MM; ++e) heap_push(H, hget(H, e));
             for (int k = 0; k < npre; ++k) S.popped[k][tid] = (uint8_t)(heap_pop(H) & 0xFFu);
             S.heapn[tid] = H.n;
@@ -1827,7 +1830,7 @@ int vnm_pq_pop_order(vnm_ctx *ctx, const int32_t *scores, int count, int M, int3
     hipStream_t st = ctx->stream;
     VNM_HIP_TRY(hipMemcpyAsync(ds.p, scores, (size_t)count * 2 * MM * 4, hipMemcpyHostToDevice, st));
     const int bt = 64;
-    hipLaunchKernelGGL(k_pq_pop_order, dim3((count + bt - 1) / bt), dim3(bt), (size_t)bt * MM * 2, st, ds.p,
+    hipLaunchKernelGGL(k_pq_pop_order, dim3((count + bt - 1) / bt), dim3(bt), (size_t)bt * HEAP_WORDS * 4, st, ds.p,
                        count, M, dord.p);
     ctx->launches++;
     VNM_LAUNCH_CHECK("k_pq_pop_order");
