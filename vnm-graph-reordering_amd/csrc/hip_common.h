@@ -77,10 +77,11 @@ __host__ __device__ __forceinline__ bool heap_le(uint32_t a, uint32_t b) { retur
 // ... is fixed (it stays at the root once there), so every ancestor on it is
 // read at once; the passed ancestors then move down one position and the
 // value takes the last hole.  Every position of the path is written, from
-// the top down, so the writes need no branches: position ip[j] gets its
-// parent's entry anc[j] for j < t, the value for j == t and its own entry
-// anc[j - 1] back for j > t; the root, repeated at the path's top, ends with
-// its first occurrence's entry.
+// the top down, so the writes need no branches: with go[j] true for the
+// passed levels j < t, position ip[j] gets its parent's entry anc[j] for
+// j < t, the value for j == t and its own entry anc[j - 1] back for j > t;
+// the root, repeated at the path's top, ends with its first occurrence's
+// entry.
 __host__ __device__ __forceinline__ void heap_sift_up1(HeapRef &H, int i, uint32_t value) {
     int ip[HEAP_DEPTH + 1];
     uint32_t anc[HEAP_DEPTH];
@@ -89,18 +90,13 @@ __host__ __device__ __forceinline__ void heap_sift_up1(HeapRef &H, int i, uint32
     for (int j = 0; j < HEAP_DEPTH; ++j) ip[j + 1] = (ip[j] >> 1) > 1 ? ip[j] >> 1 : 1;
 #pragma unroll
     for (int j = 0; j < HEAP_DEPTH; ++j) anc[j] = H.h[ip[j + 1]];
-    int t = 0;
-    bool go = true;
+    bool go[HEAP_DEPTH];
 #pragma unroll
-    for (int j = 0; j < HEAP_DEPTH; ++j) {
-        go = go && ip[j] > 1 && heap_le(anc[j], value);
-        t += go ? 1 : 0;
-    }
-    H.h[ip[HEAP_DEPTH]] = (uint16_t)(t == HEAP_DEPTH ? value : anc[HEAP_DEPTH - 1]);
+    for (int j = 0; j < HEAP_DEPTH; ++j) go[j] = (j == 0 || go[j - 1]) && ip[j] > 1 && heap_le(anc[j], value);
+    H.h[ip[HEAP_DEPTH]] = (uint16_t)(go[HEAP_DEPTH - 1] ? value : anc[HEAP_DEPTH - 1]);
 #pragma unroll
-    for (int j = HEAP_DEPTH - 1; j >= 1; --j)
-        H.h[ip[j]] = (uint16_t)(j < t ? anc[j] : j == t ? value : anc[j - 1]);
-    H.h[ip[0]] = (uint16_t)(t > 0 ? anc[0] : value);
+    for (int j = HEAP_DEPTH - 1; j >= 1; --j) H.h[ip[j]] = (uint16_t)(go[j] ? anc[j] : go[j - 1] ? value : anc[j - 1]);
+    H.h[ip[0]] = (uint16_t)(go[0] ? anc[0] : value);
 }
 
 __host__ __device__ __forceinline__ void heap_push(HeapRef &H, uint32_t value) {
