@@ -188,23 +188,6 @@ def test_rmat65536_learned_trajectory(ctx):
     g.close()
 
 
-def test_rmat65536_learned_owner_slots_in_memory(ctx, monkeypatch):
-    """The learned selection with its owner slots in device memory instead of
-    LDS: the same trajectory and the same permutation."""
-    coo = vnm.gen_rmat(16, 524288, 2, M=8)
-    model = vnm.LPModel.load(ctx, WEIGHTS)
-    digests = []
-    for glob in (False, True):
-        if glob:
-            monkeypatch.setenv("VNM_SELECT_OWNER_GLOBAL", "1")
-        g = vnm.Graph(ctx, coo, 8)
-        st = g.lp_reorder(model, N, topk=4096, window=16, maxiter=40)
-        assert _trajectory(st) == LEARNED_TRAJECTORY["rmat65536"], (glob, st.as_dict())
-        digests.append(_digest(g.download()))
-        g.close()
-    assert digests[0] == digests[1]
-
-
 def test_rmat262144_learned_reaches_exact(ctx):
     """The proposer's run: exact final counts, a permutation, the same bits on
     a repeat run, and at least 99.5 % of the best exact schedule's retained

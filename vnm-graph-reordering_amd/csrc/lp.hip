@@ -1176,8 +1176,7 @@ __global__ void k_lp_gain_keys(const LPCtl *__restrict__ ctl, int nkeys, const C
 }
 
 constexpr int SEL_T = 1024, SEL_B = 1024;
-constexpr int SEL_ROUNDS = 24;                 // parallel rounds before the sequential finish (owners in memory)
-constexpr int SEL_ROUNDS_LDS = 64;             // ... with the owners in LDS (a round is a few barriers)
+constexpr int SEL_ROUNDS = 24;                 // parallel rounds before the sequential finish
 constexpr int32_t OWNER_NONE = 0x7F7F7F7F;    // empty owner (bytes 0x7F, set by a memset)
 
 // Accept candidates in (gain desc, index asc) order when both segments are
@@ -1187,14 +1186,10 @@ constexpr int32_t OWNER_NONE = 0x7F7F7F7F;    // empty owner (bytes 0x7F, set by
 // k is accepted when it owns both (no live candidate before it shares a
 // segment, so the sequential walk would accept it too), and candidates
 // touching a segment used by now are dropped.  The first live candidate is
-// always accepted, so rounds make progress; after a fixed number of rounds
-// thread 0 finishes the chunk sequentially (any number of rounds gives the
-// same result).  Chunks of SEL_B candidates in order.  ctl->nacc = accepted,
-// ctl->gain = their total gain; acc = candidate indices in order.  The owner
-// slots (nseg ints) are in LDS after the used-segment bits when LDS_OWNER
-// (set to OWNER_NONE here), else `owner` in device memory, OWNER_NONE on
-// entry and on exit.
-template <bool LDS_OWNER>
+// always accepted, so rounds make progress; after SEL_ROUNDS rounds thread 0
+// finishes the chunk sequentially.  Chunks of SEL_B candidates in order.
+// out[0] = accepted, out[1] = their total gain; acc = candidate indices in
+// order.  owner: nseg ints, OWNER_NONE on entry and on exit.
 __global__ __launch_bounds__(SEL_T) void k_lp_select(LPCtl *ctl, int nseg, const uint64_t *__restrict__ skeys,
                                                      const Cand *__restrict__ cand,
                                                      const int4 *__restrict__ best, int32_t *__restrict__ acc,
@@ -1202,7 +1197,6 @@ __global__ __launch_bounds__(SEL_T) void k_lp_select(LPCtl *ctl, int nseg, const
     if (ctl->done) return;
     const int nc = ctl->nc;
     extern __shared__ uint32_t used[];
-    constexpr int ROUNDS = LDS_OWNER ? SEL_ROUNDS_LDS : SEL_ROUNDS;
     __shared__ int bp[SEL_B], bq[SEL_B], bg[SEL_B], bc[SEL_B];
     __shared__ uint8_t live[SEL_B], took[SEL_B];
     __shared__ int wsum[SEL_T / 64], wg[SEL_T / 64];
@@ -1210,10 +1204,6 @@ __global__ __launch_bounds__(SEL_T) void k_lp_select(LPCtl *ctl, int nseg, const
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int words = (nseg + 31) / 32;
     for (int i = tid; i < words; i += SEL_T) used[i] = 0;
-    if constexpr (LDS_OWNER) {
-        owner = reinterpret_cast<int32_t *>(used + words);
-        for (int i = tid; i < nseg; i += SEL_T) owner[i] = OWNER_NONE;
-    }
     if (tid == 0) { nacc = 0; gsum = 0; stop = 0; }
     __syncthreads();
     for (int base = 0; base < nc && !stop; base += SEL_B) {
@@ -1239,35 +1229,27 @@ __global__ __launch_bounds__(SEL_T) void k_lp_select(LPCtl *ctl, int nseg, const
             }
         }
         bool taken = false;
-        for (int r = 0; r < ROUNDS; ++r) {
+        for (int r = 0; r < SEL_ROUNDS; ++r) {
             if (!__syncthreads_or(alive)) break;
             if (alive) {
                 atomicMin(&owner[p], k);
                 atomicMin(&owner[q], k);
             }
-            if constexpr (!LDS_OWNER) __threadfence();
+            __threadfence();
             __syncthreads();
             bool win = false;
-            if (alive) {
-                if constexpr (LDS_OWNER) win = owner[p] == k && owner[q] == k;
-                else win = ld_l2(&owner[p]) == k && ld_l2(&owner[q]) == k;
-            }
+            if (alive) win = ld_l2(&owner[p]) == k && ld_l2(&owner[q]) == k;
             __syncthreads();
             if (alive) {
-                if constexpr (LDS_OWNER) {
-                    owner[p] = OWNER_NONE;
-                    owner[q] = OWNER_NONE;
-                } else {
-                    st_l2(&owner[p], OWNER_NONE);
-                    st_l2(&owner[q], OWNER_NONE);
-                }
+                st_l2(&owner[p], OWNER_NONE);
+                st_l2(&owner[q], OWNER_NONE);
                 if (win) {
                     taken = true;
                     atomicOr(&used[p >> 5], 1u << (p & 31));
                     atomicOr(&used[q >> 5], 1u << (q & 31));
                 }
             }
-            if constexpr (!LDS_OWNER) __threadfence();
+            __threadfence();
             __syncthreads();
             if (alive) alive = !taken && !((used[p >> 5] >> (p & 31)) & 1u) && !((used[q >> 5] >> (q & 31)) & 1u);
         }
@@ -1739,7 +1721,6 @@ struct LPRun {
     int nnorm = 0;  // keys sorted in an iteration with one candidate per donor
     int ncap = 0;   // ... and with the whole window (the most candidates)
     int sbits = 0, ibits = 0, gbits = 0;
-    bool sel_lds = false;  // k_lp_select keeps its owner slots in LDS
     int init_cnt = 0, init_size = 0;
     uint32_t gmax = 0;
     uint64_t smask = 0, imask = 0, seed = 0;
@@ -1785,26 +1766,6 @@ static int lp_run_begin(LPRun &r, vnm_graph *g, const vnm_lp_model *m, int N, in
     // the sorts of an iteration allocate nothing (they may run under capture)
     if ((rc = sort_scratch_reserve(std::max(2 * ns, r.ncap)))) return rc;
     VNM_HIP_TRY(hipMemsetAsync(w.owner.p, 0x7F, (size_t)ns * 4, st));  // OWNER_NONE in every slot
-    // the selection's owner slots go to LDS when they fit beside its static
-    // arrays; the kernel's dynamic LDS limit is raised to all that is left
-    // (once, here, before any capture of the iteration).  The environment
-    // variable VNM_SELECT_OWNER_GLOBAL keeps them in device memory (the two
-    // paths give the same result; tests compare them).
-    {
-        int lds_max = 0;
-        hipFuncAttributes fa{};
-        const size_t dyn = (size_t)((ns + 31) / 32) * 4 + (size_t)ns * 4;
-        if (!getenv("VNM_SELECT_OWNER_GLOBAL") &&
-            hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, g->ctx->device) ==
-                hipSuccess &&
-            hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(k_lp_select<true>)) == hipSuccess &&
-            fa.sharedSizeBytes + dyn <= (size_t)lds_max &&
-            hipFuncSetAttribute(reinterpret_cast<const void *>(k_lp_select<true>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                lds_max - (int)fa.sharedSizeBytes) == hipSuccess)
-            r.sel_lds = true;
-        (void)hipGetLastError();
-    }
     int32_t tot[2];
     VNM_HIP_TRY(hipMemcpyAsync(tot, r.c->tot.p, 8, hipMemcpyDeviceToHost, st));
     VNM_HIP_TRY(hipStreamSynchronize(st));
@@ -1844,13 +1805,9 @@ static int lp_run_iter(LPRun &r) {
     const int kbits = r.gbits + r.ibits;
     if (r.mode != 2 && (rc = sort_keys_u64_if(w.gkey.p, w.sgkey.p, r.nnorm, kbits, st, &ctl->skip_norm))) return rc;
     if (r.mode != 1 && (rc = sort_keys_u64_if(w.gkey.p, w.sgkey.p, r.ncap, kbits, st, &ctl->skip_exh))) return rc;
-    const size_t shm = (size_t)((ns + 31) / 32) * 4 + (r.sel_lds ? (size_t)ns * 4 : 0);
-    if (r.sel_lds)
-        hipLaunchKernelGGL(k_lp_select<true>, dim3(1), dim3(SEL_T), shm, st, ctl, ns, w.sgkey.p, w.cand.p,
-                           w.best.p, w.acc.p, nullptr, r.imask);
-    else
-        hipLaunchKernelGGL(k_lp_select<false>, dim3(1), dim3(SEL_T), shm, st, ctl, ns, w.sgkey.p, w.cand.p,
-                           w.best.p, w.acc.p, w.owner.p, r.imask);
+    const size_t shm = (size_t)((ns + 31) / 32) * 4;
+    hipLaunchKernelGGL(k_lp_select, dim3(1), dim3(SEL_T), shm, st, ctl, ns, w.sgkey.p, w.cand.p, w.best.p,
+                       w.acc.p, w.owner.p, r.imask);
     VNM_LAUNCH_CHECK("k_lp_select");
     hipLaunchKernelGGL(k_lp_ctl_after, dim3(1), dim3(64), 0, st, ctl, r.mode, g->queue.p);
     const int maxsw = std::min(r.ncap, ns);
