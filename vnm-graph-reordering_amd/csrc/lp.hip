@@ -1833,13 +1833,16 @@ static int lp_run_done(LPRun &r, bool *done) {
 
 static int lp_run_end(LPRun &r, vnm_lp_stats *stats) {
     hipStream_t st = r.g->ctx->stream;
-    LPCtl h;
-    int32_t tot[2];
-    VNM_HIP_TRY(hipMemcpyAsync(&h, r.ctl.p, sizeof(LPCtl), hipMemcpyDeviceToHost, st));
-    VNM_HIP_TRY(hipMemcpyAsync(tot, r.c->tot.p, 8, hipMemcpyDeviceToHost, st));
     int rc = relabel(r.g, r.g->perm.p);
     if (rc) return rc;
-    VNM_HIP_TRY(hipStreamSynchronize(st));
+    // the copies land in this frame: wait for them on every path
+    LPCtl h;
+    int32_t tot[2];
+    hipError_t e = hipMemcpyAsync(&h, r.ctl.p, sizeof(LPCtl), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(tot, r.c->tot.p, 8, hipMemcpyDeviceToHost, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = es;
+    if (e != hipSuccess) return set_error(VNM_E_HIP, "reading the run's results failed: %s", hipGetErrorString(e));
     vnm_lp_stats S{};
     S.init_cnt = r.init_cnt;
     S.init_size = r.init_size;
