@@ -173,6 +173,27 @@ __device__ __forceinline__ void store_vec(float *__restrict__ cp, const float (&
     }
 }
 
+// store_vec for rows of C: nontemporal stores, so that the output stream
+// does not push the gathered B rows out of L2 (the same values land)
+template <int NV>
+__device__ __forceinline__ void store_vec_nt(float *__restrict__ cp, const float (&acc)[NV],
+                                             const float *__restrict__ bias) {
+    float v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = bias ? acc[i] + bias[i] : acc[i];
+    if constexpr (NV % 4 == 0) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+        for (int i = 0; i < NV; i += 4)
+            __builtin_nontemporal_store(f4{v[i], v[i + 1], v[i + 2], v[i + 3]}, (f4 *)(cp + i));
+    } else if constexpr (NV == 2) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        __builtin_nontemporal_store(f2{v[0], v[1]}, (f2 *)cp);
+    } else {
+        __builtin_nontemporal_store(v[0], cp);
+    }
+}
+
 template <int NV>
 __device__ __forceinline__ void load_vec(const float *__restrict__ cp, float (&v)[NV]) {
     if constexpr (NV % 4 == 0) {
@@ -316,7 +337,7 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ vrow, cons
                     store_vec<VEC>(cp, d, bias_c);
                 }
             } else {
-                store_vec<VEC>(cp, z, bias_c);
+                store_vec_nt<VEC>(cp, z, bias_c);
             }
         }
     };
@@ -333,7 +354,7 @@ __device__ __forceinline__ void spmm_item(const int32_t *__restrict__ vrow, cons
 #pragma unroll
                 for (int i = 0; i < VEC; ++i) acc[i] += d[i];
             }
-            store_vec<VEC>(cp, acc, bias_c);
+            store_vec_nt<VEC>(cp, acc, bias_c);
         }
     };
 #pragma unroll
